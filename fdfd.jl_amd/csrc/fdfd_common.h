@@ -1,0 +1,188 @@
+// Internal declarations shared by the HIP kernels and the host runtime of
+// libfdfd.  Complex values are interleaved (re, im) pairs: double2 for FP64,
+// float2 for FP32.  Field-shaped arrays use the natural layout of the
+// reference (x fastest: index ix + iy*Nx).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fdfd {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+template <typename R> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename C> __host__ __device__ inline C mk(decltype(C{}.x) re, decltype(C{}.x) im) {
+    C z; z.x = re; z.y = im; return z;
+}
+template <typename C> __host__ __device__ inline C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __host__ __device__ inline C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __host__ __device__ inline C cmul(C a, C b) {
+    return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a*conj(b)
+template <typename C> __host__ __device__ inline C cmulc(C a, C b) {
+    return mk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// conj(a)*b
+template <typename C> __host__ __device__ inline C cconjmul(C a, C b) {
+    return mk<C>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+template <typename C> __host__ __device__ inline C cneg(C a) { return mk<C>(-a.x, -a.y); }
+// component-wise select (a struct-valued ?: can be lowered to a stack array)
+template <typename C> __host__ __device__ inline C csel(bool p, C a, C b) {
+    return mk<C>(p ? a.x : b.x, p ? a.y : b.y);
+}
+template <typename C> __host__ __device__ inline C cscale(C a, decltype(C{}.x) s) { return mk<C>(a.x * s, a.y * s); }
+// 1/z with scaling against overflow (Smith's method)
+template <typename C> __host__ __device__ inline C crcp(C z) {
+    using R = decltype(C{}.x);
+    R ar = z.x < 0 ? -z.x : z.x, ai = z.y < 0 ? -z.y : z.y;
+    if (ar >= ai) {
+        R t = z.y / z.x, d = z.x + z.y * t;
+        return mk<C>(R(1) / d, -t / d);
+    } else {
+        R t = z.x / z.y, d = z.y + z.x * t;
+        return mk<C>(t / d, R(-1) / d);
+    }
+}
+template <typename C> __host__ __device__ inline C cdiv(C a, C b) { return cmul(a, crcp(b)); }
+// LAPACK-style pivot magnitude |re| + |im| (izamax / cabs1)
+template <typename C> __host__ __device__ inline decltype(C{}.x) cabs1(C z) {
+    return (z.x < 0 ? -z.x : z.x) + (z.y < 0 ? -z.y : z.y);
+}
+template <typename C> __host__ __device__ inline decltype(C{}.x) cnorm2(C z) { return z.x * z.x + z.y * z.y; }
+
+__host__ __device__ inline float2 to_f(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__host__ __device__ inline double2 to_d(float2 a) { return make_double2((double)a.x, (double)a.y); }
+__host__ __device__ inline double2 to_d(double2 a) { return a; }
+template <typename C> __host__ __device__ inline C from_d(double2 a);
+template <> __host__ __device__ inline double2 from_d<double2>(double2 a) { return a; }
+template <> __host__ __device__ inline float2 from_d<float2>(double2 a) { return to_f(a); }
+
+// ---------------------------------------------------------------------------
+// physical constants: src/types.jl:6-9 (SI, as in the reference)
+// ---------------------------------------------------------------------------
+constexpr double EPS0 = 8.85418782e-12;
+constexpr double MU0 = 1.25663706e-6;
+constexpr double PML_M = 3.5;    // src/pml.jl:1
+constexpr double PML_LNR = -12;  // src/pml.jl:1
+
+// ---------------------------------------------------------------------------
+// operator coefficients.  The engine works with the scaled operator
+// A' = sigma_s A (sigma_s = mu0' dx^2 for TM, eps0' dx^2 for TE) so that the
+// entries are O(1); a 5-point periodic stencil
+//   (A'x)_ij = cW x_{i-1,j} + cE x_{i+1,j} + cS x_{i,j-1} + cN x_{i,j+1} + cC x_ij
+// with the five coefficient planes stored contiguously (plane k at coef+k*N).
+// ---------------------------------------------------------------------------
+enum CoefPlane { CW = 0, CE = 1, CS = 2, CN = 3, CC = 4 };
+
+struct GridDesc {
+    int Nx, Ny;
+    double dx, dy;
+    int npx, npy;
+    double L0;
+};
+
+// One line-block inversion in flight (Gauss-Jordan with partial pivoting,
+// in place, pivot rows tracked by index instead of swapped).
+struct LineJob {
+    void* Y;          // n x n row-major complex (R); input S', output Y
+    int* used;        // n: row already chosen as a pivot
+    int* prow;        // n: prow[c] = pivot row of column c      (= rho of the line)
+    int* iprow;       // n: inverse of prow (-1 until pivoted)   (= rho^-1)
+    const int* q;     // n: row map of the work matrix, S'[r] = S[q(r)]
+    int* kap;         // n: column map, X[a,b] = Y[rho(a), kap(b)]
+    int* kapinv;      // n: inverse of kap
+    void* Vbuf;       // n x PMAX panel copy (original panel values)
+    void* Vpinv;      // PMAX x PMAX inverse of the pivot block
+    int* piv;         // PMAX: pivot rows of the current panel
+    void* Z;          // PMAX x n: Vpinv * [A(piv, :) | I]
+    double* pivstat;  // [0] = min |pivot|, [1] = max |pivot|, [2] = zero-pivot flag
+    int n;
+};
+
+// Formation of the work matrix S' of one line from its neighbours' factors.
+//   S'[r, b] = D[q(r), b] - l[q(r)] u[b] X1[q(r), b] - l2[q(r)] u2[b] X2[q(r), b]
+// with X1[a,b] = Y1[rho1(a), kap1(b)] and q = rho1^-1 (so the X1 term reads
+// Y1 row r directly).  Chain starts have no neighbour (Y1 == nullptr).
+struct SchurJob {
+    void* S;                  // destination n x n (line slot)
+    const void* Y1; const int* rhoinv1; const int* kap1;
+    const void* Y2; const int* rho2; const int* kap2;
+    const double2* dsub; const double2* ddiag; const double2* dsup;  // line coefficient planes
+    const double2* l1; const double2* u1;   // row / column scaling of X1 term
+    const double2* l2; const double2* u2;   // row / column scaling of X2 term
+    long si;                  // element stride along the line in natural layout
+    long su1, su2;            // stride of the u vectors (neighbour line)
+    int* q;                   // out: row map of S'
+    int* used; int* iprow;    // reset for the new line
+    int n;
+};
+
+// One GEMV of a block sweep (forward, backward or middle step of one chain).
+struct SweepJob {
+    const void* Y;              // n x n factor (R)
+    const int* rho; const int* rhoinv; const int* kap; const int* kapinv;
+    int mode;                   // 0 forward, 1 backward, 2 middle
+    const double2* b;           // rhs line (natural layout, stride si)
+    const double2* nb1; const double2* c1; long snb1;  // neighbour line values and coupling
+    const double2* nb2; const double2* c2; long snb2;
+    const double2* g;           // backward: forward-sweep value of this line
+    double2* out;               // output line
+    long si;
+    int n;
+};
+
+// ---------------------------------------------------------------------------
+// launch wrappers (implemented in the .hip files)
+// ---------------------------------------------------------------------------
+void launch_sfactor(hipStream_t s, double2* sxf, double2* sxb, double2* syf, double2* syb,
+                    const GridDesc& g, double omega, double eps0n, double eta0);
+void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const double2* sxf,
+                   const double2* sxb, const double2* syf, const double2* syb,
+                   const GridDesc& g, int pol, double kdiag);
+void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef,
+                  int Nx, int Ny, int transpose, const double2* b /*nullable: r=b-Ax*/);
+void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n);
+void launch_zero(hipStream_t s, double2* y, long n);
+void launch_multidot(hipStream_t s, const double2* V, long ldv, int k, const double2* w,
+                     long n, double2* partial, int nblk);
+void launch_reduce_partials(hipStream_t s, const double2* partial, int nblk, int k, double2* out);
+void launch_gemv_update(hipStream_t s, double2* w, const double2* V, long ldv, int k,
+                        const double2* h, double sign, long n);
+void launch_norm2(hipStream_t s, const double2* x, long n, double* partial, int nblk);
+void launch_reduce_real(hipStream_t s, const double* partial, int nblk, double* out);
+void launch_fields(hipStream_t s, double2* out, const double2* x, const double2* eps,
+                   const double2* sxf, const double2* sxb, const double2* syf,
+                   const double2* syb, const GridDesc& g, int pol, double2 w,
+                   int eigen_mode);
+void launch_mul_eps(hipStream_t s, double2* y, const double2* x, const double2* eps,
+                    double2 a, long n);
+void launch_grad(hipStream_t s, double* grad, const double2* e, const double2* lam,
+                 double2 coef, long n);
+
+template <typename R>
+void launch_schur(hipStream_t s, const SchurJob* jobs_d, int njobs, int n);
+template <typename R>
+void launch_panel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
+template <typename R>
+void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
+template <typename R>
+void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
+void launch_maps(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
+template <typename R>
+void launch_sweep(hipStream_t s, const SweepJob* jobs_d, int njobs, int n, int transpose,
+                  double2* partial);
+
+// largest line-block size the panel kernel supports (one row per thread)
+constexpr int MAX_LINE = 1024;
+// widest Gauss-Jordan panel; row stride of the panel copy and of Vpinv
+constexpr int PMAX = 16;
+
+}  // namespace fdfd

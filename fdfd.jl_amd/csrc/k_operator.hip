@@ -1,0 +1,358 @@
+// Operator-side kernels of libfdfd (all FP64):
+//   * SC-PML inverse stretch factors        (src/pml.jl:1-63)
+//   * 5-point coefficient assembly TM / TE   (src/solver/driven.jl:21-46,
+//                                             src/grid.jl:128-162)
+//   * matrix-free apply / residual, A and A^T
+//   * BLAS-1 reductions for the FP64 Krylov refinement
+//   * field recovery (driven.jl:40-41,50-51; eigen.jl:90-91,108-109)
+//   * adjoint gradient product
+// Field-shaped arrays are Nx x Ny complex, x fastest.  These kernels are
+// HBM-bound streaming kernels: one thread per cell, coalesced along x.
+#include "fdfd_common.h"
+
+#include <algorithm>
+
+namespace fdfd {
+
+// ---------------------------------------------------------------------------
+// SC-PML: 1/s(l), s = 1 - i sigma(l)/(omega eps0'),
+// sigma(l) = sigma_max (l/T)^m, sigma_max = -(m+1) lnR / (2 eta0 T)
+// (pml.jl:2-10); l at cell centres for the forward factor (:15-19) and at
+// cell edges for the backward factor (:21-25); 1 outside the PML.  The
+// reference stores the inverse 1/s in S_create (pml.jl:47-53).
+// ---------------------------------------------------------------------------
+__device__ inline double2 inv_sfactor(int i /*1-based*/, int Nw, int npml, double dw, bool fwd,
+                                      double omega, double eps0n, double eta0) {
+    if (npml == 0) return make_double2(1.0, 0.0);
+    double T = npml * dw;
+    double l;
+    if (fwd) {
+        if (i <= npml) l = dw * (npml - i + 0.5);
+        else if (i > Nw - npml) l = dw * (i - (Nw - npml) - 0.5);
+        else return make_double2(1.0, 0.0);
+    } else {
+        if (i <= npml) l = dw * (npml - i + 1);
+        else if (i > Nw - npml) l = dw * (i - (Nw - npml) - 1);
+        else return make_double2(1.0, 0.0);
+    }
+    double smax = -(PML_M + 1.0) * PML_LNR / (2.0 * eta0 * T);
+    double sig = smax * pow(l / T, PML_M);
+    double2 s = make_double2(1.0, -sig / (omega * eps0n));
+    return crcp(s);
+}
+
+__global__ void k_sfactor(double2* sxf, double2* sxb, double2* syf, double2* syb, GridDesc g,
+                          double omega, double eps0n, double eta0) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < g.Nx) {
+        sxf[t] = inv_sfactor(t + 1, g.Nx, g.npx, g.dx, true, omega, eps0n, eta0);
+        sxb[t] = inv_sfactor(t + 1, g.Nx, g.npx, g.dx, false, omega, eps0n, eta0);
+    } else if (t < g.Nx + g.Ny) {
+        int j = t - g.Nx;
+        syf[j] = inv_sfactor(j + 1, g.Ny, g.npy, g.dy, true, omega, eps0n, eta0);
+        syb[j] = inv_sfactor(j + 1, g.Ny, g.npy, g.dy, false, omega, eps0n, eta0);
+    }
+}
+
+void launch_sfactor(hipStream_t s, double2* sxf, double2* sxb, double2* syf, double2* syb,
+                    const GridDesc& g, double omega, double eps0n, double eta0) {
+    int tot = g.Nx + g.Ny;
+    k_sfactor<<<(tot + 255) / 256, 256, 0, s>>>(sxf, sxb, syf, syb, g, omega, eps0n, eta0);
+}
+
+// ---------------------------------------------------------------------------
+// Coefficients of A' (scaled operator).
+// TM (driven.jl:35), scale mu0' dx^2:
+//   cE = sxf_i sxb_{i+1}, cW = sxf_i sxb_i, cN = r syf_j syb_{j+1},
+//   cS = r syf_j syb_j (r = dx^2/dy^2), cC = -(cE+cW+cN+cS) + kdiag*eps_ij
+// TE (driven.jl:45), scale eps0' dx^2, tx_ij = 2/(eps_ij+eps_{i-1,j})
+// (= eps0'/grid_average(eps0' eps, x), grid.jl:157-162, periodic):
+//   cE = sxf_i tx_{i+1,j} sxb_{i+1}, cW = sxf_i tx_ij sxb_i, (y analogous),
+//   cC = -(sum) + kdiag.   Indices wrap periodically (grid.jl:138-150).
+// ---------------------------------------------------------------------------
+__global__ void k_coeffs(double2* coef, const double2* eps, const double2* sxf,
+                         const double2* sxb, const double2* syf, const double2* syb, int Nx,
+                         int Ny, double ry, int pol, double kdiag) {
+    long N = (long)Nx * Ny;
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int i = (int)(k % Nx), j = (int)(k / Nx);
+    int ip = (i + 1 == Nx) ? 0 : i + 1, im = (i == 0) ? Nx - 1 : i - 1;
+    int jp = (j + 1 == Ny) ? 0 : j + 1, jm = (j == 0) ? Ny - 1 : j - 1;
+    double2 cE, cW, cN, cS, cC;
+    double2 ryc = make_double2(ry, 0.0);
+    if (pol == 1) {
+        cE = cmul(sxf[i], sxb[ip]);
+        cW = cmul(sxf[i], sxb[i]);
+        cN = cmul(ryc, cmul(syf[j], syb[jp]));
+        cS = cmul(ryc, cmul(syf[j], syb[j]));
+        cC = cadd(cneg(cadd(cadd(cE, cW), cadd(cN, cS))), cscale(eps[k], kdiag));
+    } else {
+        double2 e0 = eps[k];
+        double2 two = make_double2(2.0, 0.0);
+        double2 tx_i = cdiv(two, cadd(e0, eps[im + (long)j * Nx]));
+        double2 tx_ip = cdiv(two, cadd(eps[ip + (long)j * Nx], e0));
+        double2 ty_j = cdiv(two, cadd(e0, eps[i + (long)jm * Nx]));
+        double2 ty_jp = cdiv(two, cadd(eps[i + (long)jp * Nx], e0));
+        cE = cmul(sxf[i], cmul(tx_ip, sxb[ip]));
+        cW = cmul(sxf[i], cmul(tx_i, sxb[i]));
+        cN = cmul(ryc, cmul(syf[j], cmul(ty_jp, syb[jp])));
+        cS = cmul(ryc, cmul(syf[j], cmul(ty_j, syb[j])));
+        cC = cadd(cneg(cadd(cadd(cE, cW), cadd(cN, cS))), make_double2(kdiag, 0.0));
+    }
+    coef[CW * N + k] = cW;
+    coef[CE * N + k] = cE;
+    coef[CS * N + k] = cS;
+    coef[CN * N + k] = cN;
+    coef[CC * N + k] = cC;
+}
+
+void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const double2* sxf,
+                   const double2* sxb, const double2* syf, const double2* syb,
+                   const GridDesc& g, int pol, double kdiag) {
+    long N = (long)g.Nx * g.Ny;
+    double ry = (g.dx * g.dx) / (g.dy * g.dy);
+    k_coeffs<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(coef, eps, sxf, sxb, syf, syb, g.Nx,
+                                                         g.Ny, ry, pol, kdiag);
+}
+
+// ---------------------------------------------------------------------------
+// y = A' x  (or A'^T x); if b != nullptr: y = b - A' x.
+// A'^T x at (i,j) = cE_{i-1} x_{i-1} + cW_{i+1} x_{i+1} + cN_{j-1} x_{j-1}
+//                 + cS_{j+1} x_{j+1} + cC x.
+// ---------------------------------------------------------------------------
+__global__ void k_apply(double2* y, const double2* x, const double2* coef, int Nx, int Ny,
+                        int transpose, const double2* b) {
+    long N = (long)Nx * Ny;
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int i = (int)(k % Nx), j = (int)(k / Nx);
+    long kw = (i == 0) ? k + Nx - 1 : k - 1;
+    long ke = (i + 1 == Nx) ? k - (Nx - 1) : k + 1;
+    long ks = (j == 0) ? k + N - Nx : k - Nx;
+    long kn = (j + 1 == Ny) ? k - (N - Nx) : k + Nx;
+    double2 acc;
+    if (!transpose) {
+        acc = cmul(coef[CC * N + k], x[k]);
+        acc = cadd(acc, cmul(coef[CW * N + k], x[kw]));
+        acc = cadd(acc, cmul(coef[CE * N + k], x[ke]));
+        acc = cadd(acc, cmul(coef[CS * N + k], x[ks]));
+        acc = cadd(acc, cmul(coef[CN * N + k], x[kn]));
+    } else {
+        acc = cmul(coef[CC * N + k], x[k]);
+        acc = cadd(acc, cmul(coef[CE * N + kw], x[kw]));
+        acc = cadd(acc, cmul(coef[CW * N + ke], x[ke]));
+        acc = cadd(acc, cmul(coef[CN * N + ks], x[ks]));
+        acc = cadd(acc, cmul(coef[CS * N + kn], x[kn]));
+    }
+    y[k] = b ? csub(b[k], acc) : acc;
+}
+
+void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef, int Nx,
+                  int Ny, int transpose, const double2* b) {
+    long N = (long)Nx * Ny;
+    k_apply<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(y, x, coef, Nx, Ny, transpose, b);
+}
+
+// y = a * x
+__global__ void k_scale_copy(double2* y, const double2* x, double2 a, long n) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; k < n; k += (long)gridDim.x * blockDim.x) y[k] = cmul(a, x[k]);
+}
+void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n) {
+    unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 4096);
+    k_scale_copy<<<nb, 256, 0, s>>>(y, x, a, n);
+}
+
+// y = 0
+__global__ void k_zero(double2* y, long n) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; k < n; k += (long)gridDim.x * blockDim.x) y[k] = make_double2(0.0, 0.0);
+}
+void launch_zero(hipStream_t s, double2* y, long n) {
+    unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 4096);
+    k_zero<<<nb, 256, 0, s>>>(y, n);
+}
+
+// y = a * eps .* x
+__global__ void k_mul_eps(double2* y, const double2* x, const double2* eps, double2 a, long n) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    for (; k < n; k += (long)gridDim.x * blockDim.x) y[k] = cmul(a, cmul(eps[k], x[k]));
+}
+void launch_mul_eps(hipStream_t s, double2* y, const double2* x, const double2* eps, double2 a,
+                    long n) {
+    unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 4096);
+    k_mul_eps<<<nb, 256, 0, s>>>(y, x, eps, a, n);
+}
+
+// ---------------------------------------------------------------------------
+// reductions (deterministic two-pass: per-block partials, then one block)
+// ---------------------------------------------------------------------------
+template <typename T> __device__ inline T wave_sum(T v);
+template <> __device__ inline double wave_sum<double>(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// partial[blk*k + i] = sum_{e in blk} conj(V_i[e]) w[e]   (k <= 32)
+constexpr int MD_MAXK = 32;
+__global__ __launch_bounds__(256) void k_multidot(const double2* V, long ldv, int k,
+                                                  const double2* w, long n, double2* partial) {
+    __shared__ double2 red[4][MD_MAXK];
+    double2 acc[MD_MAXK];
+    for (int i = 0; i < MD_MAXK; ++i) acc[i] = make_double2(0, 0);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        double2 we = w[e];
+#pragma unroll
+        for (int i = 0; i < MD_MAXK; ++i)
+            if (i < k) acc[i] = cadd(acc[i], cconjmul(V[i * ldv + e], we));
+    }
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < MD_MAXK; ++i) {
+        if (i < k) {
+            double re = wave_sum(acc[i].x), im = wave_sum(acc[i].y);
+            if (lane == 0) red[wv][i] = make_double2(re, im);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double2 s = red[0][threadIdx.x];
+        for (int q = 1; q < 4; ++q) s = cadd(s, red[q][threadIdx.x]);
+        partial[blockIdx.x * (long)k + threadIdx.x] = s;
+    }
+}
+
+void launch_multidot(hipStream_t s, const double2* V, long ldv, int k, const double2* w, long n,
+                     double2* partial, int nblk) {
+    k_multidot<<<nblk, 256, 0, s>>>(V, ldv, k, w, n, partial);
+}
+
+__global__ void k_reduce_partials(const double2* partial, int nblk, int k, double2* out) {
+    // one block; thread i sums column i in a fixed order
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        double2 s = make_double2(0, 0);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[b * (long)k + i]);
+        out[i] = s;
+    }
+}
+void launch_reduce_partials(hipStream_t s, const double2* partial, int nblk, int k,
+                            double2* out) {
+    k_reduce_partials<<<1, 64, 0, s>>>(partial, nblk, k, out);
+}
+
+// w += sign * sum_i h_i V_i
+__global__ void k_gemv_update(double2* w, const double2* V, long ldv, int k, const double2* h,
+                              double sign, long n) {
+    __shared__ double2 hs[64];
+    if (threadIdx.x < k) hs[threadIdx.x] = cscale(h[threadIdx.x], sign);
+    __syncthreads();
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        double2 acc = w[e];
+        for (int i = 0; i < k; ++i) acc = cadd(acc, cmul(hs[i], V[i * ldv + e]));
+        w[e] = acc;
+    }
+}
+void launch_gemv_update(hipStream_t s, double2* w, const double2* V, long ldv, int k,
+                        const double2* h, double sign, long n) {
+    unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 2048);
+    k_gemv_update<<<nb, 256, 0, s>>>(w, V, ldv, k, h, sign, n);
+}
+
+__global__ __launch_bounds__(256) void k_norm2(const double2* x, long n, double* partial) {
+    __shared__ double red[4];
+    double acc = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        acc += cnorm2(x[e]);
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+void launch_norm2(hipStream_t s, const double2* x, long n, double* partial, int nblk) {
+    k_norm2<<<nblk, 256, 0, s>>>(x, n, partial);
+}
+__global__ void k_reduce_real(const double* partial, int nblk, double* out) {
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int b = 0; b < nblk; ++b) s += partial[b];
+        *out = s;
+    }
+}
+void launch_reduce_real(hipStream_t s, const double* partial, int nblk, double* out) {
+    k_reduce_real<<<1, 64, 0, s>>>(partial, nblk, out);
+}
+
+// ---------------------------------------------------------------------------
+// field recovery.  out has 3 planes of N: [u, c1, c2].
+// driven TM (driven.jl:40-41): Hx = -w * syb_j (u_ij - u_i,j-1)/dy,
+//                              Hy =  w * sxb_i (u_ij - u_i-1,j)/dx,  w = 1/(i omega mu0')
+// eigen TM (eigen.jl:90-91): forward differences syf_j (u_i,j+1 - u_ij)/dy.
+// driven TE (driven.jl:50-51), w = 1/(i omega):
+//   Ex = w * Teyi_ij * syb_j (u_ij - u_i,j-1)/dy,
+//   Ey = -w * Texi_ij * sxb_i (u_ij - u_i-1,j)/dx,  Texi = 1/(eps0' avg_x eps)
+// eigen TE (eigen.jl:108-109): Ex uses Texi, Ey uses Teyi.
+// eigen_mode: 0 driven, 1 eigen.  The eps0' of TE is folded into w by the host.
+// ---------------------------------------------------------------------------
+__global__ void k_fields(double2* out, const double2* x, const double2* eps, const double2* sxf,
+                         const double2* sxb, const double2* syf, const double2* syb, int Nx,
+                         int Ny, double dx, double dy, int pol, double2 w, int eigen_mode) {
+    long N = (long)Nx * Ny;
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    int i = (int)(k % Nx), j = (int)(k / Nx);
+    long kw = (i == 0) ? k + Nx - 1 : k - 1;
+    long ke = (i + 1 == Nx) ? k - (Nx - 1) : k + 1;
+    long ks = (j == 0) ? k + N - Nx : k - Nx;
+    long kn = (j + 1 == Ny) ? k - (N - Nx) : k + Nx;
+    double2 u = x[k];
+    out[k] = u;
+    if (pol == 1) {
+        double2 dyv, dxv;
+        if (!eigen_mode) {
+            dyv = cscale(cmul(syb[j], csub(u, x[ks])), 1.0 / dy);
+            dxv = cscale(cmul(sxb[i], csub(u, x[kw])), 1.0 / dx);
+        } else {
+            dyv = cscale(cmul(syf[j], csub(x[kn], u)), 1.0 / dy);
+            dxv = cscale(cmul(sxf[i], csub(x[ke], u)), 1.0 / dx);
+        }
+        out[N + k] = cneg(cmul(w, dyv));
+        out[2 * N + k] = cmul(w, dxv);
+    } else {
+        double2 two = make_double2(2.0, 0.0);
+        // 1/avg = 2/(eps_ij + eps_neighbour); eps0' is inside w
+        double2 tx = cdiv(two, cadd(eps[k], eps[kw]));
+        double2 ty = cdiv(two, cadd(eps[k], eps[ks]));
+        double2 dyv = cscale(cmul(syb[j], csub(u, x[ks])), 1.0 / dy);
+        double2 dxv = cscale(cmul(sxb[i], csub(u, x[kw])), 1.0 / dx);
+        double2 tex = eigen_mode ? tx : ty;
+        double2 tey = eigen_mode ? ty : tx;
+        out[N + k] = cmul(w, cmul(tex, dyv));
+        out[2 * N + k] = cneg(cmul(w, cmul(tey, dxv)));
+    }
+}
+
+void launch_fields(hipStream_t s, double2* out, const double2* x, const double2* eps,
+                   const double2* sxf, const double2* sxb, const double2* syf,
+                   const double2* syb, const GridDesc& g, int pol, double2 w,
+                   int eigen_mode) {
+    long N = (long)g.Nx * g.Ny;
+    k_fields<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(out, x, eps, sxf, sxb, syf, syb, g.Nx,
+                                                         g.Ny, g.dx, g.dy, pol, w, eigen_mode);
+}
+
+// grad[k] = Re(coef * lam[k] * e[k])
+__global__ void k_grad(double* grad, const double2* e, const double2* lam, double2 coef,
+                       long n) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k < n) grad[k] = cmul(coef, cmul(lam[k], e[k])).x;
+}
+void launch_grad(hipStream_t s, double* grad, const double2* e, const double2* lam,
+                 double2 coef, long n) {
+    k_grad<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, e, lam, coef, n);
+}
+
+}  // namespace fdfd

@@ -1,0 +1,87 @@
+// Host runtime structures of libfdfd (C++; no types from here cross the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fdfd_common.h"
+
+namespace fdfd {
+
+using cd = std::complex<double>;
+
+enum ProfId { P_TRAIL = 0, P_PANEL = 1, P_SCHUR = 2, P_SWEEP = 3, P_STENCIL = 4, P_ZBUILD = 5,
+              P_COUNT = 6 };
+
+struct Prof {
+    bool enabled = false;
+    struct Pending { hipEvent_t a, b; int id; double flops, bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[P_COUNT] = {0};
+    long long launches[P_COUNT] = {0};
+    double flops[P_COUNT] = {0};
+    double bytes[P_COUNT] = {0};
+};
+
+// Block-tridiagonal factors of one operator (pol, omega) on the handle's grid.
+struct Factor {
+    int id = 0, pol = 1, precision = 0;
+    double omega = 0;
+    int n = 0, m = 0, orient = 0;
+    long si = 1, sj = 1;
+    double scale = 1;       // A' = scale * A
+    double2* coef = nullptr;  // 5 planes of N
+    double2 *sxf = nullptr, *sxb = nullptr, *syf = nullptr, *syb = nullptr;
+    void* Y = nullptr;        // m * n * n complex (double2 or float2)
+    size_t elem = 16;
+    int *rho = nullptr, *rhoinv = nullptr, *kap = nullptr, *kapinv = nullptr;
+    double* pivstat = nullptr;  // 3 per line
+    int k = 0;                  // middle line of the twisted factorization
+    // device sweep tables: [transpose][phase] with phase 0 forward, 1 middle, 2 backward
+    SweepJob* sweep_tab[2] = {nullptr, nullptr};
+    int nfwd = 0, nbwd = 0;     // number of forward / backward steps
+    std::vector<int> fwd_jobs, bwd_jobs;  // jobs per step
+    double t_factor_ms = 0;
+    double pivot_growth = 0;
+    int singular_line = -1;
+};
+
+struct Handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GridDesc g{};
+    bool has_grid = false, has_eps = false;
+    double2* eps = nullptr;
+    std::map<int, Factor*> factors;
+    int next_id = 1;
+    std::string err;
+    int panel_width = 0;  // 0 = default per precision
+    // factorization work (two chains)
+    int work_n = 0;
+    int *used = nullptr, *q = nullptr, *piv = nullptr;
+    void *Vbuf = nullptr, *Vpinv = nullptr, *Zb = nullptr;
+    // sweep / Krylov work
+    long work_N = 0;
+    double2 *r_in = nullptr, *g_mid = nullptr, *x_out = nullptr;  // sweep vectors
+    double2 *V = nullptr, *Zk = nullptr;  // Krylov bases
+    int kdim = 0;
+    double2 *tmp = nullptr, *tmp2 = nullptr, *bvec = nullptr, *xvec = nullptr;
+    double2* dpart = nullptr;   // multidot partials
+    double* rpart = nullptr;    // norm partials
+    double2* hdev = nullptr;    // small device vector of coefficients
+    double2* tpart = nullptr;   // transposed-sweep partials
+    int nblk = 0;
+    Prof prof;
+};
+
+// error helper: record message, return code
+int fail(Handle* h, int code, const std::string& msg);
+
+}  // namespace fdfd
+
+struct fdfd_handle { fdfd::Handle impl; };

@@ -1,0 +1,185 @@
+/*
+ * fdfd.h -- C ABI of libfdfd, the MI355X (gfx950) engine for the 2-D FDFD
+ * driven solve and shift-invert eigenfrequency path of FDFD.jl.
+ *
+ * Plain C: no C++ or HIP types cross this boundary.  Complex numbers are
+ * interleaved (re, im) doubles, identical in memory to Julia ComplexF64,
+ * C `double _Complex` and numpy complex128.  2-D arrays are Nx x Ny with x
+ * fastest (Julia column-major, linear index ix + iy*Nx, 0-based).
+ *
+ * Every function returns an int status: 0 = FDFD_OK, negative = error; the
+ * message of the last error on a handle is returned by fdfd_last_error().
+ * A handle owns one HIP stream and every device buffer it allocates; it is
+ * not thread-safe (distinct handles are independent).  Host buffers passed
+ * in are borrowed for the duration of the call only.  Functions whose names
+ * end in `_d` take device pointers (hipMalloc'd memory on the handle's
+ * device) instead of host pointers and do not copy over PCIe.
+ *
+ * Reference interfaces replaced (FDFD.jl, Julia):
+ *   solve(d::Device, pol)                 src/solver/driven.jl:4-59
+ *   dolinearsolve(A, b, sym)  (lu(A)\b)   src/solver/solver.jl:4-41
+ *   eigenfrequency(d, pol, nev; which)    src/solver/eigen.jl:69-115
+ *   S_create / create_sfactor             src/pml.jl:1-63
+ *   delta / grid_average                  src/grid.jl:128-162
+ *   FieldTM / FieldTE layout              src/data.jl:50-78
+ */
+#ifndef FDFD_H
+#define FDFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdfd_handle fdfd_handle;
+
+typedef struct {
+    double re;
+    double im;
+} fdfd_complex;
+
+/* Polarization: src/types.jl:20 (@enum Polarization TM=1 TE=2). */
+enum { FDFD_TM = 1, FDFD_TE = 2 };
+
+/* Factor precision.  FDFD_FP64: complex128 block factors (the plain path).
+ * FDFD_FP32_IR: complex64 block factors on FP32 MFMA, accuracy restored by
+ * FP64 flexible GMRES on the exact FP64 operator. */
+enum { FDFD_FP64 = 0, FDFD_FP32_IR = 1 };
+
+/* Eigenvalue selection for fdfd_eigenfrequency (Arpack `which`):
+ * LM = largest magnitude of the shift-inverted operator = nearest the shift. */
+enum { FDFD_WHICH_LM = 0, FDFD_WHICH_LR = 1 };
+
+/* Status codes. */
+enum {
+    FDFD_OK = 0,
+    FDFD_ERR_INVALID = -1,   /* bad argument / unsupported size            */
+    FDFD_ERR_OOM = -2,       /* device allocation failed                    */
+    FDFD_ERR_SINGULAR = -3,  /* zero pivot in a line block (stats.line)     */
+    FDFD_ERR_NOCONV = -4,    /* refinement / Arnoldi did not converge       */
+    FDFD_ERR_HIP = -5,       /* HIP runtime error                           */
+    FDFD_ERR_NOTFOUND = -6,  /* unknown factor id                           */
+    FDFD_ERR_STATE = -7      /* call order violated (e.g. no grid set)      */
+};
+
+typedef struct {
+    int iterations;        /* GMRES iterations (all restarts, all RHS)     */
+    int restarts;
+    double residual;       /* final max relative residual ||b-Ax||/||b||    */
+    double t_factor_ms;    /* device time of the block factorization        */
+    double t_solve_ms;     /* device time of the refined solve              */
+    double pivot_growth;   /* max |pivot| / min |pivot| over all line blocks */
+    int singular_line;     /* line index of a zero pivot, -1 if none        */
+    int nconv;             /* converged eigenpairs (eigenfrequency)         */
+} fdfd_stats;
+
+/* ---- library / handle --------------------------------------------------- */
+int fdfd_version(int* major, int* minor, int* patch);
+const char* fdfd_last_error(const fdfd_handle* h);
+/* Which kernel variant a solve used ("gfx950"), for provenance checks. */
+const char* fdfd_arch(void);
+
+int fdfd_create(int device_ordinal, fdfd_handle** out);
+int fdfd_destroy(fdfd_handle* h);
+
+/* Grid(dh, Npml, xrange, yrange; L0): src/grid.jl:26-33.  dx, dy are the
+ * cell sizes span/N (src/grid.jl:68-74) in units of L0; PML m=3.5 and
+ * lnR=-12 are fixed as in src/pml.jl:1,35-38. */
+int fdfd_set_grid(fdfd_handle* h, int Nx, int Ny, double dx, double dy,
+                  int npml_x, int npml_y, double L0);
+
+/* Relative permittivity Device.eps_r (src/device.jl:21), Nx*Ny complex. */
+int fdfd_set_eps(fdfd_handle* h, const fdfd_complex* eps);
+int fdfd_set_eps_d(fdfd_handle* h, const void* d_eps);
+
+/* ---- factorization ------------------------------------------------------ */
+/* Assembles the TM (driven.jl:35) or TE (driven.jl:45) operator at angular
+ * frequency omega (rad/s, normalised units c0/L0 as in device.jl:40) on the
+ * device and computes its block-tridiagonal (twisted block-Thomas) factors.
+ * Each line block is inverted by Gauss-Jordan elimination with partial
+ * pivoting.  Replaces lu(A) of solver.jl:35. */
+int fdfd_factor(fdfd_handle* h, int pol, double omega, int precision,
+                int* factor_id, fdfd_stats* stats);
+int fdfd_release_factor(fdfd_handle* h, int factor_id);
+
+/* ---- solves ------------------------------------------------------------- */
+/* Driven solve for nrhs source arrays src (Nx*Ny*nrhs, Device.src
+ * src/device.jl:22): b = 1im*omega*src (driven.jl:36,46), A x = b solved to
+ * relative residual <= tol, then field recovery (driven.jl:40-41, 50-51).
+ * out is Nx*Ny*3*nrhs: per RHS the three components in FieldTM order
+ * (Ez, Hx, Hy) or FieldTE order (Hz, Ex, Ey), x fastest, component slowest
+ * -- the memory layout of cat(reshape.(...), dims=3) in data.jl:62,77. */
+int fdfd_solve(fdfd_handle* h, int factor_id, int nrhs, const fdfd_complex* src,
+               fdfd_complex* out, double tol, fdfd_stats* stats);
+int fdfd_solve_d(fdfd_handle* h, int factor_id, int nrhs, const void* d_src,
+                 void* d_out, double tol, fdfd_stats* stats);
+
+/* Plain linear solves with the reference-scaled operator A of driven.jl:35/45:
+ * x = A \ b  (transpose=0) or x = A^T \ b (transpose=1).  b, x: Nx*Ny*nrhs. */
+int fdfd_linsolve(fdfd_handle* h, int factor_id, int transpose, int nrhs,
+                  const fdfd_complex* b, fdfd_complex* x, double tol,
+                  fdfd_stats* stats);
+
+/* y = A x or y = A^T x with the exact (periodic, PML-stretched) operator. */
+int fdfd_apply(fdfd_handle* h, int factor_id, int transpose,
+               const fdfd_complex* x, fdfd_complex* y);
+
+/* rel = ||b - A x||_2 / ||b||_2 in FP64 on the device. */
+int fdfd_residual(fdfd_handle* h, int factor_id, const fdfd_complex* x,
+                  const fdfd_complex* b, double* rel);
+
+/* Adjoint gradient with respect to eps_r (TM): for A(eps) e = b and an
+ * adjoint field lam solving A^T lam = dJ/de,
+ *   grad[k] = -Re(weight * lam[k] * omega^2 * eps0' * e[k])
+ * (dA/deps_k = omega^2 eps0' e_k e_k^T from driven.jl:21,35).  For
+ * J = |m^T e|^2 pass weight = 2*conj(m^T e) and lam = A^-T m. */
+int fdfd_adjoint_grad(fdfd_handle* h, int factor_id, const fdfd_complex* e_fwd,
+                      const fdfd_complex* lam, fdfd_complex weight, double* grad);
+
+/* ---- eigenfrequencies --------------------------------------------------- */
+/* Shift-invert Arnoldi (eigen.jl:69-115): eigenvalues lambda of the TM
+ * operator diag(1/eps) (dxf dxb + dyf dyb) nearest sigma = -w0^2 mu0' eps0'
+ * (TE: dxf Tex^-1 dxb + dyf Tey^-1 dyb, sigma = -w0^2 mu0'), returned as
+ * omega = sqrt(-lambda/(mu0' eps0')) (TE: sqrt(-lambda/mu0')), sorted by
+ * |lambda - sigma|.  fields (may be NULL): nev * Nx*Ny*3 eigenfields with
+ * the forward-difference H recovery of eigen.jl:90-91 / 108-109, each
+ * eigenvector normalised to unit 2-norm.  The shifted operator is factored
+ * once and the factors are reused for every Arnoldi step. */
+int fdfd_eigenfrequency(fdfd_handle* h, int pol, double omega0, int nev,
+                        int ncv, int which, int precision, double tol,
+                        int maxiter, fdfd_complex* omegas, fdfd_complex* fields,
+                        fdfd_stats* stats);
+
+/* ---- batched driven solves (many grids x frequencies x sources) --------- */
+/* One job = one grid + eps + nomega frequencies + nsrc sources (each source
+ * is solved at every frequency).  fdfd_batch_run factors all (job, omega)
+ * pairs with the batched block kernels and solves every source; results are
+ * read back with fdfd_batch_result.  Uneven grid sizes may be mixed. */
+typedef struct fdfd_batch fdfd_batch;
+int fdfd_batch_create(fdfd_handle* h, fdfd_batch** out);
+int fdfd_batch_add(fdfd_batch* b, int Nx, int Ny, double dx, double dy,
+                   int npml_x, int npml_y, double L0, int pol,
+                   const fdfd_complex* eps, int nomega, const double* omegas,
+                   int nsrc, const fdfd_complex* srcs, int* job_id);
+int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats);
+/* fields of (job, omega index, source index): Nx*Ny*3 complex. */
+int fdfd_batch_result(fdfd_batch* b, int job_id, int iomega, int isrc,
+                      fdfd_complex* fields);
+int fdfd_batch_destroy(fdfd_batch* b);
+
+/* ---- tuning and profiling ----------------------------------------------- */
+/* Per-kernel device time accumulated with HIP events on the handle's stream
+ * while profiling is enabled.  kernel: 0 = trailing GJE update, 1 = panel,
+ * 2 = Schur formation, 3 = sweep GEMV, 4 = stencil/residual, 5 = Z build. */
+int fdfd_profile_enable(fdfd_handle* h, int enable);
+int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
+                      long long* launches, double* flops, double* bytes);
+/* Panel width override (0 = autotuned / default). */
+int fdfd_set_tuning(fdfd_handle* h, int panel_width, int gemm_tile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDFD_H */

@@ -1,0 +1,515 @@
+"""CPU oracle for the FDFD driven-solve / eigenfrequency path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it restates the
+reference algorithm (FDFD.jl, Julia) in numpy/scipy so that the HIP engine's
+results can be compared against it.  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it; the product path
+(``fdfd.jl_amd``) never does.
+
+Every function cites the reference file:line it follows.  The sparse matrices
+are built in the reference's own order of operations (``kron`` of 1-D periodic
+difference matrices, left-multiplied by diagonal inverse stretch factors,
+products formed as sparse-sparse products) so that the assembled ``A`` and
+``b`` are the reference's ``A`` and ``b`` up to floating-point rounding of the
+same operations.
+
+Parity pinning (what checks each function against something other than itself;
+see tests/test_oracle.py):
+  constants / normalisation ........ literal values, c0*sqrt(eps0*mu0)==1
+  Grid, dx, xc, x2ind .............. hand-computed cases, banker's rounding ties
+  delta (periodic differences) ..... dense brute force; Dxf^T == -Dxb; FFT symbol
+  create_sfactor / S_create ........ closed-form sigma(l) at hand-picked cells
+  grid_average ..................... np.roll restatement + hand case
+  TM/TE operator ................... dense kron brute force; FFT eigenvalues for
+                                     npml=0; Kronecker-sum separability with PML
+  solve (SuperLU + FP64 refinement). residual certificate; FFT closed form for
+                                     npml=0 uniform eps; dense zgesv tiny case
+  field recovery ................... closed-form differences of the solved field
+  eigenfrequency ................... Kronecker-sum closed form (uniform eps, PML)
+  flux_surface_integral ............ lossless energy conservation between planes
+  adjoint gradient ................. central finite differences
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field as dc_field
+from typing import Callable, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+# ---------------------------------------------------------------------------
+# constants: src/types.jl:6-9 ; enums src/types.jl:18-21
+# ---------------------------------------------------------------------------
+EPS0 = 8.85418782e-12
+MU0 = 1.25663706e-6
+C0 = math.sqrt(1 / EPS0 / MU0)
+ETA0 = math.sqrt(MU0 / EPS0)
+DEFAULT_L0 = 1e-6  # src/grid.jl:5
+
+TM = 1  # src/types.jl:20
+TE = 2
+XHAT, YHAT = 1, 2  # src/types.jl:18
+FORWARD, BACKWARD = 1, 2  # src/types.jl:19
+
+
+def normalize_parameters(L0: float = DEFAULT_L0):
+    """src/device.jl:40 -- (eps0*L0, mu0*L0, c0/L0)."""
+    return (EPS0 * L0, MU0 * L0, C0 / L0)
+
+
+# ---------------------------------------------------------------------------
+# Grid: src/grid.jl:7-125
+# ---------------------------------------------------------------------------
+@dataclass
+class Grid:
+    """2-D grid (src/grid.jl:7-33).  ``N = round(L/dh)`` with ties-to-even
+    (Julia ``round`` default, src/grid.jl:28; Python ``round`` is also
+    ties-to-even)."""
+
+    L: tuple
+    L0: float
+    N: tuple
+    Npml: tuple
+    bounds: tuple  # ((x1, y1), (x2, y2))
+
+    @classmethod
+    def create(cls, dh: float, npml: Sequence[int], xrange: Sequence[float],
+               yrange: Sequence[float], L0: float = DEFAULT_L0) -> "Grid":
+        L = (float(xrange[1] - xrange[0]), float(yrange[1] - yrange[0]))
+        N = (int(round(L[0] / dh)), int(round(L[1] / dh)))
+        return cls(L, L0, N, (int(npml[0]), int(npml[1])),
+                   ((float(xrange[0]), float(yrange[0])),
+                    (float(xrange[1]), float(yrange[1]))))
+
+    @property
+    def shape(self):
+        return self.N
+
+    def __len__(self):
+        return self.N[0] * self.N[1]
+
+    # src/grid.jl:68-74 : span / N, not dh
+    @property
+    def dx(self) -> float:
+        return (self.bounds[1][0] - self.bounds[0][0]) / self.N[0]
+
+    @property
+    def dy(self) -> float:
+        return (self.bounds[1][1] - self.bounds[0][1]) / self.N[1]
+
+    def dh(self, w: int) -> float:
+        return self.dx if w == XHAT else self.dy
+
+    # src/grid.jl:76-82 (Julia range arithmetic is TwicePrecision; plain
+    # float64 here -- masks are specified by integer index in the workloads)
+    def xc(self) -> np.ndarray:
+        return self.bounds[0][0] + self.dx * (np.arange(self.N[0]) + 0.5)
+
+    def yc(self) -> np.ndarray:
+        return self.bounds[0][1] + self.dy * (np.arange(self.N[1]) + 0.5)
+
+    # src/grid.jl:112-125
+    def x2ind(self, x: float) -> int:
+        ind = int(round((x - self.bounds[0][0]) / self.L[0] * self.N[0])) + 1
+        return min(max(ind, 1), self.N[0])
+
+    def y2ind(self, y: float) -> int:
+        ind = int(round((y - self.bounds[0][1]) / self.L[1] * self.N[1])) + 1
+        return min(max(ind, 1), self.N[1])
+
+    def coord2ind(self, x: float, y: float):
+        """src/grid.jl:103-109 (1-based indices)."""
+        return (self.x2ind(x), self.y2ind(y))
+
+
+# ---------------------------------------------------------------------------
+# derivative operators: src/grid.jl:128-154
+# ---------------------------------------------------------------------------
+def _d1(n: int, h: float, s: int) -> sp.csc_matrix:
+    """1-D periodic difference; forward: spdiagm(1=>1, 0=>-1, -n+1=>[1])/h
+    (src/grid.jl:138), backward: spdiagm(-1=>-1, 0=>1, n-1=>[-1])/h (:141)."""
+    inv = 1.0 / h
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        if s == FORWARD:
+            rows += [i, i]
+            cols += [i, (i + 1) % n]
+            vals += [-1.0, 1.0]
+        else:
+            rows += [i, i]
+            cols += [i, (i - 1) % n]
+            vals += [1.0, -1.0]
+    if n == 1:  # the +1 and -1 land on the same entry and cancel
+        return sp.csc_matrix((1, 1), dtype=np.float64)
+    m = sp.csc_matrix((np.array(vals) * inv, (rows, cols)), shape=(n, n))
+    m.sum_duplicates()
+    return m
+
+
+def delta(w: int, s: int, g: Grid) -> sp.csc_matrix:
+    """src/grid.jl:128-154: kron(I_Ny, D_x) for x, kron(D_y, I_Nx) for y
+    (column-major, x fastest)."""
+    Nx, Ny = g.N
+    if w == XHAT:
+        return sp.kron(sp.identity(Ny, format="csc"), _d1(Nx, g.dx, s), format="csc")
+    return sp.kron(_d1(Ny, g.dy, s), sp.identity(Nx, format="csc"), format="csc")
+
+
+def grid_average(a: np.ndarray, w: int) -> np.ndarray:
+    """src/grid.jl:157-162: (a + circshift(a, 1 along w))/2, periodic."""
+    if a.ndim == 1:
+        return (a + np.roll(a, 1)) / 2
+    if w == XHAT:
+        return (a + np.roll(a, 1, axis=0)) / 2
+    if w == YHAT:
+        return (a + np.roll(a, 1, axis=1)) / 2
+    return a
+
+
+# ---------------------------------------------------------------------------
+# SC-PML: src/pml.jl:1-63
+# ---------------------------------------------------------------------------
+def create_sfactor(w: int, s: int, g: Grid, omega: float, m: float = 3.5,
+                   lnR: float = -12.0) -> np.ndarray:
+    """src/pml.jl:1-31.  s(l) = 1 - i*sigma(l)/(omega*eps0'),
+    sigma(l) = sigma_max*(l/T)^m, sigma_max = -(m+1)lnR/(2*eta0*T)
+    (eta0 unnormalised, eps0 normalised)."""
+    eps0, _, _ = normalize_parameters(g.L0)
+    dw = g.dh(w)
+    nw_pml = g.Npml[w - 1]
+    Tw = nw_pml * dw
+    Nw = g.N[w - 1]
+    out = np.ones(Nw, dtype=np.complex128)
+    if nw_pml == 0:
+        return out
+    smax = -(m + 1) * lnR / (2 * ETA0 * Tw)
+
+    def S(l):
+        sig = smax * (l / Tw) ** m
+        return 1 - 1j * sig / (omega * eps0)
+
+    for i in range(1, Nw + 1):
+        if s == FORWARD:
+            if i <= nw_pml:
+                out[i - 1] = S(dw * (nw_pml - i + 0.5))
+            elif i > Nw - nw_pml:
+                out[i - 1] = S(dw * (i - (Nw - nw_pml) - 0.5))
+        else:
+            if i <= nw_pml:
+                out[i - 1] = S(dw * (nw_pml - i + 1))
+            elif i > Nw - nw_pml:
+                out[i - 1] = S(dw * (i - (Nw - nw_pml) - 1))
+    return out
+
+
+def sfactor_vectors(g: Grid, omega: float):
+    """The four inverse stretch vectors 1/s of src/pml.jl:33-54 (x-forward,
+    x-backward, y-forward, y-backward), kept 1-D."""
+    return tuple(1.0 / create_sfactor(w, s, g, omega)
+                 for (w, s) in ((XHAT, FORWARD), (XHAT, BACKWARD),
+                                (YHAT, FORWARD), (YHAT, BACKWARD)))
+
+
+def S_create(g: Grid, omega: float):
+    """src/pml.jl:33-63: diagonal matrices of 1/s broadcast to 2-D."""
+    Nx, Ny = g.N
+    sxf, sxb, syf, syb = sfactor_vectors(g, omega)
+    Sxf = np.repeat(sxf[:, None], Ny, axis=1)
+    Sxb = np.repeat(sxb[:, None], Ny, axis=1)
+    Syf = np.repeat(syf[None, :], Nx, axis=0)
+    Syb = np.repeat(syb[None, :], Nx, axis=0)
+    f = lambda a: sp.diags(a.ravel(order="F"), format="csc")
+    return (f(Sxf), f(Sxb), f(Syf), f(Syb))
+
+
+# ---------------------------------------------------------------------------
+# Device: src/device.jl:19-108
+# ---------------------------------------------------------------------------
+@dataclass
+class Device:
+    """src/device.jl:19-35; eps_r and src are complex Nx x Ny (x fastest)."""
+
+    grid: Grid
+    omega: np.ndarray
+    eps_r: np.ndarray = None
+    src: np.ndarray = None
+
+    def __post_init__(self):
+        self.omega = np.atleast_1d(np.asarray(self.omega, dtype=np.float64))
+        if self.eps_r is None:
+            self.eps_r = np.ones(self.grid.N, dtype=np.complex128)
+        if self.src is None:
+            self.src = np.zeros(self.grid.N, dtype=np.complex128)
+
+    def setup_eps_region(self, region: Callable, value):
+        """src/device.jl:63-86: region evaluated at cell centres."""
+        _mask_values(self.eps_r, self.grid, region, value)
+
+    def setup_src_region(self, region: Callable, value):
+        """src/device.jl:92."""
+        _mask_values(self.src, self.grid, region, value)
+
+    def setup_src_point(self, x: float, y: float):
+        """src/device.jl:95-98: writes 1im at coord2ind."""
+        ix, iy = self.grid.coord2ind(x, y)
+        self.src[ix - 1, iy - 1] = 1j
+
+    def setup_src_line(self, x: float, y: float, normal: int):
+        """src/device.jl:101-108."""
+        ix, iy = self.grid.coord2ind(x, y)
+        if normal == XHAT:
+            self.src[ix - 1, :] = 1j
+        elif normal == YHAT:
+            self.src[:, iy - 1] = 1j
+
+
+def _mask_values(pixels, g: Grid, region, value):
+    X, Y = np.meshgrid(g.xc(), g.yc(), indexing="ij")
+    mask = np.vectorize(region)(X, Y).astype(bool)
+    if callable(value):
+        pixels[mask] = np.vectorize(value)(X, Y)[mask]
+    else:
+        pixels[mask] = value
+
+
+# ---------------------------------------------------------------------------
+# operator assembly: src/solver/driven.jl:21-53
+# ---------------------------------------------------------------------------
+def stretched_deltas(g: Grid, omega: float):
+    """src/solver/driven.jl:25-31: S*delta for the four derivatives."""
+    Sxf, Sxb, Syf, Syb = S_create(g, omega)
+    dxb = (Sxb @ delta(XHAT, BACKWARD, g)).tocsc()
+    dxf = (Sxf @ delta(XHAT, FORWARD, g)).tocsc()
+    dyb = (Syb @ delta(YHAT, BACKWARD, g)).tocsc()
+    dyf = (Syf @ delta(YHAT, FORWARD, g)).tocsc()
+    return dxf, dxb, dyf, dyb
+
+
+def assemble(g: Grid, eps_r: np.ndarray, omega: float, pol: int):
+    """TM: A = dxf*mu0^-1*dxb + dyf*mu0^-1*dyb + w^2*T_eps (driven.jl:35).
+    TE: A = dxf*Texi*dxb + dyf*Teyi*dyb + w^2*mu0*I (driven.jl:45)."""
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    dxf, dxb, dyf, dyb = stretched_deltas(g, omega)
+    N = len(g)
+    if pol == TM:
+        Teps = sp.diags(eps0 * eps_r.ravel(order="F"), format="csc")
+        A = (dxf * (mu0 ** -1)) @ dxb + (dyf * (mu0 ** -1)) @ dyb + (omega ** 2) * Teps
+    else:
+        Texi = sp.diags((1 / grid_average(eps0 * eps_r, XHAT)).ravel(order="F"), format="csc")
+        Teyi = sp.diags((1 / grid_average(eps0 * eps_r, YHAT)).ravel(order="F"), format="csc")
+        A = dxf @ Texi @ dxb + dyf @ Teyi @ dyb + (omega ** 2 * mu0) * sp.identity(N, format="csc")
+    return A.tocsc()
+
+
+def rhs(omega: float, src: np.ndarray) -> np.ndarray:
+    """b = 1im*omega*src[:] (driven.jl:36,46)."""
+    return (1j * omega) * src.ravel(order="F").astype(np.complex128)
+
+
+# ---------------------------------------------------------------------------
+# the reference's `\`: src/solver/solver.jl:4-41 (lu(A)\b)
+# ---------------------------------------------------------------------------
+def rel_residual(A, x, b) -> float:
+    r = b - A @ x
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def linear_solve(A, b, refine_tol: float = 1e-14, max_refine: int = 8,
+                 lu=None, trans: str = "N"):
+    """FP64 sparse direct solve (SuperLU, the scipy stand-in for UMFPACK's
+    lu(A)\\b at solver.jl:35) followed by FP64 iterative refinement so the
+    relative residual certifies the solution.  ``trans='T'`` solves A^T x=b."""
+    if lu is None:
+        lu = spla.splu(A.tocsc(), permc_spec="COLAMD")
+    Aop = A if trans == "N" else A.T
+    x = lu.solve(b, trans=trans)
+    for _ in range(max_refine):
+        r = b - Aop @ x
+        if np.linalg.norm(r) <= refine_tol * np.linalg.norm(b):
+            break
+        x = x + lu.solve(r, trans=trans)
+    return x, lu
+
+
+@dataclass
+class FieldTM:
+    """src/data.jl:50-63: data is Nx x Ny x 3 (Ez, Hx, Hy)."""
+    grid: Grid
+    omega: complex
+    data: np.ndarray
+
+
+@dataclass
+class FieldTE:
+    """src/data.jl:65-78: data is Nx x Ny x 3 (Hz, Ex, Ey)."""
+    grid: Grid
+    omega: complex
+    data: np.ndarray
+
+
+def _stack(g: Grid, *vs):
+    sz = g.N
+    return np.stack([v.reshape(sz, order="F") for v in vs], axis=2)
+
+
+def solve(d: Device, pol: int = TM, return_system: bool = False):
+    """src/solver/driven.jl:4-59 (mode sources out of scope)."""
+    g = d.grid
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    out = []
+    systems = []
+    for omega in d.omega:
+        omega = float(omega)
+        A = assemble(g, d.eps_r, omega, pol)
+        b = rhs(omega, d.src)
+        x, lu = linear_solve(A, b)
+        dxf, dxb, dyf, dyb = stretched_deltas(g, omega)
+        if pol == TM:
+            ez = x
+            hx = (-1 / 1j / omega / mu0) * (dyb @ ez)   # driven.jl:40
+            hy = (1 / 1j / omega / mu0) * (dxb @ ez)    # driven.jl:41
+            out.append(FieldTM(g, complex(omega), _stack(g, ez, hx, hy)))
+        else:
+            Texi = sp.diags((1 / grid_average(eps0 * d.eps_r, XHAT)).ravel(order="F"))
+            Teyi = sp.diags((1 / grid_average(eps0 * d.eps_r, YHAT)).ravel(order="F"))
+            hz = x
+            ex = (1 / 1j / omega) * (Teyi @ (dyb @ hz))     # driven.jl:50
+            ey = (1 / 1j / omega) * (Texi @ (-(dxb @ hz)))  # driven.jl:51
+            out.append(FieldTE(g, complex(omega), _stack(g, hz, ex, ey)))
+        systems.append((A, b, lu))
+    res = out[0] if len(out) == 1 else out
+    return (res, systems) if return_system else res
+
+
+# ---------------------------------------------------------------------------
+# eigenfrequency: src/solver/eigen.jl:69-115
+# ---------------------------------------------------------------------------
+def eigen_operator(g: Grid, eps_r: np.ndarray, omega0: float, pol: int):
+    """TM: A = diag(1/eps_r) (dxf dxb + dyf dyb), sigma = -w0^2 mu0 eps0
+    (eigen.jl:83-86).  TE: A = dxf Tex^-1 dxb + dyf Tey^-1 dyb,
+    sigma = -w0^2 mu0 (eigen.jl:100-104)."""
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    dxf, dxb, dyf, dyb = stretched_deltas(g, omega0)
+    if pol == TM:
+        Tinv = sp.diags(1 / eps_r.ravel(order="F"), format="csc")
+        A = Tinv @ dxf @ dxb + Tinv @ dyf @ dyb
+        sigma = -omega0 ** 2 * mu0 * eps0
+    else:
+        Texi = sp.diags((1 / grid_average(eps0 * eps_r, XHAT)).ravel(order="F"))
+        Teyi = sp.diags((1 / grid_average(eps0 * eps_r, YHAT)).ravel(order="F"))
+        A = dxf @ Texi @ dxb + dyf @ Teyi @ dyb
+        sigma = -omega0 ** 2 * mu0
+    return A.tocsc(), sigma
+
+
+def eigenfrequency(d: Device, pol: int, nev: int, ncv: int | None = None,
+                   tol: float = 0.0):
+    """Shift-invert Arnoldi via ARPACK (scipy.sparse.linalg.eigs with sigma),
+    the same ARPACK family Arpack.jl wraps (eigen.jl:86,104).  Eigenvalues are
+    returned sorted by distance to sigma; fields use the forward stretched
+    differences (eigen.jl:90-91, 108-109)."""
+    g = d.grid
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    omega0 = float(d.omega[0])
+    A, sigma = eigen_operator(g, d.eps_r, omega0, pol)
+    if ncv is None:
+        ncv = max(20, 2 * nev + 1)
+    lam, vec = spla.eigs(A, k=nev, sigma=sigma, which="LM", ncv=ncv, tol=tol)
+    order = np.argsort(np.abs(lam - sigma))
+    lam, vec = lam[order], vec[:, order]
+    dxf, dxb, dyf, dyb = stretched_deltas(g, omega0)
+    fields = []
+    if pol == TM:
+        w = np.sqrt(-lam / mu0 / eps0)
+        for i in range(nev):
+            ez = vec[:, i]
+            hx = (-1 / 1j / w[i] / mu0) * (dyf @ ez)
+            hy = (1 / 1j / w[i] / mu0) * (dxf @ ez)
+            fields.append(FieldTM(g, w[i], _stack(g, ez, hx, hy)))
+    else:
+        w = np.sqrt(-lam / mu0)
+        Texi = sp.diags((1 / grid_average(eps0 * d.eps_r, XHAT)).ravel(order="F"))
+        Teyi = sp.diags((1 / grid_average(eps0 * d.eps_r, YHAT)).ravel(order="F"))
+        for i in range(nev):
+            hz = vec[:, i]
+            ex = (1 / 1j / w[i]) * (Texi @ (dyb @ hz))
+            ey = (1 / 1j / w[i]) * (Teyi @ (-(dxb @ hz)))
+            fields.append(FieldTE(g, w[i], _stack(g, hz, ex, ey)))
+    return w, fields, lam, sigma
+
+
+# ---------------------------------------------------------------------------
+# flux: src/flux.jl:37-47 (TM, x-normal)
+# ---------------------------------------------------------------------------
+def flux_surface_integral_tm_x(f: FieldTM, xcen: float, ycen: float, width: float) -> float:
+    """-1/2 sum Re(avg_x(Ez) conj(Hy)) dy over y in [yc-w, yc+w] at the column
+    nearest xcen (atvalue tolerance dx/2), flux.jl:39-47."""
+    g = f.grid
+    xs, ys = g.xc(), g.yc()
+    cand = np.nonzero(np.abs(xs - xcen) <= g.dx / 2)[0]
+    if cand.size == 0:
+        raise ValueError("no column within dx/2")
+    xi = int(cand[0])
+    yi = np.nonzero((ys >= ycen - width) & (ys <= ycen + width))[0]
+    Ez = f.data[xi:xi + 2, yi, 0]
+    # grid_average over the 2-row view, first row: circshift((1,0)) of a
+    # 2-row array puts row 2 into row 1, so the average is (Ez[x]+Ez[x+1])/2
+    Ezx = (Ez[0, :] + Ez[1, :]) / 2
+    Hy = f.data[xi, yi, 2]
+    return float(np.sum(-0.5 * np.real(Ezx * np.conj(Hy))) * g.dy)
+
+
+# ---------------------------------------------------------------------------
+# workload generators (integer-index geometry; x fastest; 0-based arrays)
+# ---------------------------------------------------------------------------
+def omega_of_wavelength(lam_um: float, L0: float = DEFAULT_L0) -> float:
+    _, _, c0 = normalize_parameters(L0)
+    return 2 * math.pi * c0 / lam_um
+
+
+def config_c1():
+    """200x120 cells, dl=0.02, 15-cell PML, eps=12.25 guide on iy 54:68
+    (1-based), point source 1im at (51,61), lambda=1.55."""
+    g = Grid.create(0.02, (15, 15), (0.0, 4.0), (-1.2, 1.2))
+    d = Device(g, omega_of_wavelength(1.55))
+    d.eps_r[:, 53:68] = 12.25
+    d.src[50, 60] = 1j
+    return d
+
+
+def random_pixel_device(Nx: int, Ny: int, npml: int, lam_um: float, seed,
+                        design: int | None = None, pixel: int = 8,
+                        guide: int = 15, src_len: int = 25, src_ix: int = 51,
+                        dl: float = 0.02):
+    """Random-pixel coupler: centred design region of pixel x pixel blocks
+    with eps in {1, 12.25} (Bernoulli 0.5), two 15-cell guides through the
+    PML on the centre row, and a 25-cell line source (value 1.0) at
+    ix=src_ix.  At 1024x1024 the design is ix,iy in 213:812 (1-based)."""
+    g = Grid.create(dl, (npml, npml), (-Nx * dl / 2, Nx * dl / 2), (-Ny * dl / 2, Ny * dl / 2))
+    assert g.N == (Nx, Ny), g.N
+    d = Device(g, omega_of_wavelength(lam_um))
+    if design is None:
+        design = int(round(600 / 1024 * min(Nx, Ny) / pixel)) * pixel
+    npx = design // pixel
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pix = rng.random((npx, npx)) < 0.5
+    x0 = (Nx - design) // 2
+    y0 = (Ny - design) // 2
+    eps = np.ones((Nx, Ny))
+    blk = np.kron(np.where(pix, 12.25, 1.0), np.ones((pixel, pixel)))
+    eps[x0:x0 + design, y0:y0 + design] = blk
+    # 1024: iy 506:520 (1-based) -> 0-based 505..519
+    yg0 = Ny // 2 - guide // 2
+    eps[:x0, yg0:yg0 + guide] = 12.25
+    eps[x0 + design:, yg0:yg0 + guide] = 12.25
+    d.eps_r[:] = eps
+    # 1024: iy 501:525 (1-based) -> 0-based 500..524
+    ys0 = Ny // 2 - src_len // 2
+    d.src[src_ix - 1, ys0:ys0 + src_len] = 1.0
+    return d
+
+
+def config_c2(n: int = 1024, npml: int = 20, seed: int = 2):
+    return random_pixel_device(n, n, npml, 1.55, seed)

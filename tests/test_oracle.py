@@ -1,0 +1,329 @@
+"""Pins of the CPU oracle (oracle/fdfd_oracle.py) against facts that do not
+come from the oracle's own code: literal constants, hand-computed grid
+cases, independent loop-built operators, FFT / Kronecker-sum closed forms,
+the PML's design attenuation, a discrete Maxwell identity and finite
+differences.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import fdfd_oracle as O
+
+EPS0n, MU0n, C0n = O.normalize_parameters()
+
+
+# --- constants and normalisation (src/types.jl:6-9, src/device.jl:40) ------
+def test_constants():
+    assert O.EPS0 == 8.85418782e-12 and O.MU0 == 1.25663706e-6
+    assert abs(O.C0 * math.sqrt(O.EPS0 * O.MU0) - 1) < 1e-15
+    assert abs(O.ETA0 - 376.730313) < 1e-5
+    assert EPS0n == 8.85418782e-18 and MU0n == pytest.approx(1.25663706e-12, rel=1e-15)
+    assert C0n == pytest.approx(2.9979245813099606e14, rel=1e-14)
+
+
+# --- Grid (src/grid.jl:26-33, 68-82, 112-125) --------------------------------
+def test_grid_c1():
+    g = O.Grid.create(0.02, (15, 15), (0.0, 4.0), (-1.2, 1.2))
+    assert g.N == (200, 120)
+    assert g.dx == pytest.approx(0.02) and g.dy == pytest.approx(0.02)
+    assert g.coord2ind(1.0, 0.0) == (51, 61)
+    assert g.xc()[0] == pytest.approx(0.01) and g.yc()[-1] == pytest.approx(1.19)
+
+
+def test_grid_rounding_ties_to_even():
+    # N = round(L/dh): 2.5 -> 2, 3.5 -> 4 (Julia round is ties-to-even)
+    assert O.Grid.create(1.0, (0, 0), (0, 2.5), (0, 3.5)).N == (2, 4)
+    g = O.Grid.create(1.0, (0, 0), (0, 4.0), (0, 4.0))
+    # (x - x1)/L*N = 1.5 -> round = 2 -> index 3 ; 2.5 -> 2 -> index 3
+    assert g.x2ind(1.5) == 3 and g.x2ind(2.5) == 3 and g.x2ind(0.5) == 1
+    assert g.x2ind(-10) == 1 and g.x2ind(99) == 4  # clamped
+
+
+# --- periodic differences (src/grid.jl:128-154) ------------------------------
+def _dense_1d(n, h, fwd):
+    D = np.zeros((n, n))
+    for i in range(n):
+        if fwd:
+            D[i, i] -= 1 / h
+            D[i, (i + 1) % n] += 1 / h
+        else:
+            D[i, i] += 1 / h
+            D[i, (i - 1) % n] -= 1 / h
+    return D
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7])
+def test_delta_1d_and_transpose(n):
+    g = O.Grid.create(0.5, (0, 0), (0, 0.5 * n), (0, 0.5 * 3))
+    dxf = O.delta(O.XHAT, O.FORWARD, g).toarray()
+    dxb = O.delta(O.XHAT, O.BACKWARD, g).toarray()
+    assert np.allclose(dxf[:n, :n], _dense_1d(n, 0.5, True))
+    assert np.allclose(dxf, -dxb.T)  # Dxf^T = -Dxb for the periodic operators
+
+
+def test_delta_2d_layout():
+    """x fastest: y-difference couples k and k+Nx, periodic in y."""
+    Nx, Ny = 4, 3
+    g = O.Grid.create(1.0, (0, 0), (0, Nx), (0, Ny))
+    dyf = O.delta(O.YHAT, O.FORWARD, g).toarray()
+    f = np.arange(Nx * Ny, dtype=float) ** 2
+    F = f.reshape((Nx, Ny), order="F")
+    ref = (np.roll(F, -1, axis=1) - F).ravel(order="F")
+    assert np.allclose(dyf @ f, ref)
+
+
+def test_grid_average():
+    a = np.array([[1.0, 2.0], [3.0, 5.0], [7.0, 11.0]])
+    assert np.allclose(O.grid_average(a, O.XHAT), [[4.0, 6.5], [2.0, 3.5], [5.0, 8.0]])
+    assert np.allclose(O.grid_average(a, O.YHAT), [[1.5, 1.5], [4.0, 4.0], [9.0, 9.0]])
+
+
+# --- SC-PML (src/pml.jl) -----------------------------------------------------
+def test_sfactor_design_attenuation():
+    """The conductivity integrates to the design reflection: for a graded
+    profile, 2 eta0 * int_0^T sigma dl = -lnR = 12.  The forward factors sit
+    at cell centres, so the midpoint rule gives the integral."""
+    npml, dl = 40, 0.01
+    g = O.Grid.create(dl, (npml, npml), (0, 2.0), (0, 2.0))
+    om = O.omega_of_wavelength(1.55)
+    s = O.create_sfactor(O.XHAT, O.FORWARD, g, om)
+    sigma = -s.imag * om * EPS0n
+    integral = np.sum(sigma[:npml]) * dl
+    assert 2 * O.ETA0 * integral == pytest.approx(12.0, rel=2e-3)
+    assert np.allclose(s[npml:-npml], 1.0)
+    assert np.allclose(s, s[::-1])                      # symmetric profile
+    assert np.all(np.diff(sigma[:npml]) < 0)           # grows toward the wall
+    sb = O.create_sfactor(O.XHAT, O.BACKWARD, g, om)
+    assert sb[-npml] == 1.0 and sb[0].imag < s[0].imag  # edge sample is deeper
+
+
+def test_sfactor_no_pml():
+    g = O.Grid.create(0.02, (0, 0), (0, 1.0), (0, 1.0))
+    for w in (O.XHAT, O.YHAT):
+        for sd in (O.FORWARD, O.BACKWARD):
+            assert np.all(O.create_sfactor(w, sd, g, 1e15) == 1.0)
+
+
+# --- operator: FFT closed form (npml = 0) ------------------------------------
+@pytest.mark.parametrize("pol", [O.TM, O.TE])
+def test_operator_fft_spectrum(pol):
+    Nx, Ny, dx = 10, 7, 0.03
+    g = O.Grid.create(dx, (0, 0), (0, Nx * dx), (0, Ny * dx))
+    om = O.omega_of_wavelength(1.3)
+    eps = 2.5
+    d = O.Device(g, om)
+    d.eps_r[:] = eps
+    A = O.assemble(g, d.eps_r, om, pol)
+    X, Y = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    for p, q in [(0, 0), (1, 2), (3, 6), (5, 1)]:
+        mode = np.exp(2j * np.pi * (p * X / Nx + q * Y / Ny)).ravel(order="F")
+        lap = -4 * (np.sin(np.pi * p / Nx) ** 2 / g.dx ** 2 + np.sin(np.pi * q / Ny) ** 2 / g.dy ** 2)
+        if pol == O.TM:
+            lam = lap / MU0n + om ** 2 * EPS0n * eps
+        else:
+            lam = lap / (EPS0n * eps) + om ** 2 * MU0n
+        assert np.allclose(A @ mode, lam * mode, rtol=1e-12, atol=1e-12 * abs(lam))
+
+
+def test_solve_fft_closed_form():
+    Nx, Ny, dx = 16, 12, 0.025
+    g = O.Grid.create(dx, (0, 0), (0, Nx * dx), (0, Ny * dx))
+    om = O.omega_of_wavelength(1.55)
+    d = O.Device(g, om)
+    d.eps_r[:] = 3.0
+    rng = np.random.default_rng(0)
+    d.src[:] = rng.standard_normal((Nx, Ny)) + 1j * rng.standard_normal((Nx, Ny))
+    f = O.solve(d, O.TM)
+    b = (1j * om) * d.src
+    p = np.fft.fftfreq(Nx) * Nx
+    q = np.fft.fftfreq(Ny) * Ny
+    P, Q = np.meshgrid(p, q, indexing="ij")
+    lam = (-4 * (np.sin(np.pi * P / Nx) ** 2 / g.dx ** 2 + np.sin(np.pi * Q / Ny) ** 2 / g.dy ** 2)
+           / MU0n + om ** 2 * EPS0n * 3.0)
+    ez = np.fft.ifft2(np.fft.fft2(b) / lam)
+    assert np.linalg.norm(f.data[:, :, 0] - ez) / np.linalg.norm(ez) < 1e-12
+
+
+# --- operator with PML: Kronecker sum of independently built 1-D operators --
+def _op1d(n, h, sf, sb):
+    """s_f[i] (s_b[i+1](f[i+1]-f[i]) - s_b[i](f[i]-f[i-1])) / h^2, periodic."""
+    M = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        ip, im = (i + 1) % n, (i - 1) % n
+        M[i, ip] += sf[i] * sb[ip] / h ** 2
+        M[i, i] -= sf[i] * (sb[ip] + sb[i]) / h ** 2
+        M[i, im] += sf[i] * sb[i] / h ** 2
+    return M
+
+
+def test_tm_operator_kronecker_sum_with_pml():
+    Nx, Ny, dx = 11, 9, 0.05
+    g = O.Grid.create(dx, (3, 2), (0, Nx * dx), (0, Ny * dx))
+    om = O.omega_of_wavelength(1.55)
+    eps = 4.0
+    d = O.Device(g, om)
+    d.eps_r[:] = eps
+    A = O.assemble(g, d.eps_r, om, O.TM).toarray()
+    sxf, sxb, syf, syb = O.sfactor_vectors(g, om)
+    Ax = _op1d(Nx, g.dx, sxf, sxb) / MU0n
+    Ay = _op1d(Ny, g.dy, syf, syb) / MU0n
+    K = np.kron(np.eye(Ny), Ax) + np.kron(Ay, np.eye(Nx)) + om ** 2 * EPS0n * eps * np.eye(Nx * Ny)
+    assert np.abs(A - K).max() <= 1e-12 * np.abs(K).max()
+    # spectrum is the sum of the 1-D spectra
+    ev = np.sort_complex(np.linalg.eigvals(A))
+    ex = np.sort_complex((np.linalg.eigvals(Ax)[:, None] + np.linalg.eigvals(Ay)[None, :]).ravel()
+                         + om ** 2 * EPS0n * eps)
+    assert np.allclose(ev, ex, rtol=1e-9, atol=1e-9 * np.abs(ex).max())
+
+
+def test_te_operator_loop_restatement():
+    """TE with non-uniform eps against a per-cell loop of driven.jl:45."""
+    Nx, Ny, dx = 6, 5, 0.05
+    g = O.Grid.create(dx, (2, 1), (0, Nx * dx), (0, Ny * dx))
+    om = O.omega_of_wavelength(1.4)
+    rng = np.random.default_rng(3)
+    eps = 1 + 5 * rng.random((Nx, Ny)) + 0.1j * rng.random((Nx, Ny))
+    A = O.assemble(g, eps, om, O.TE).toarray()
+    sxf, sxb, syf, syb = O.sfactor_vectors(g, om)
+    tx = lambda i, j: 1 / (EPS0n * (eps[i % Nx, j] + eps[(i - 1) % Nx, j]) / 2)
+    ty = lambda i, j: 1 / (EPS0n * (eps[i, j % Ny] + eps[i, (j - 1) % Ny]) / 2)
+    K = np.zeros_like(A)
+    idx = lambda i, j: (i % Nx) + (j % Ny) * Nx
+    for i in range(Nx):
+        for j in range(Ny):
+            k = idx(i, j)
+            ip = (i + 1) % Nx
+            K[k, idx(i + 1, j)] += sxf[i] * tx(i + 1, j) * sxb[ip] / dx ** 2
+            K[k, k] -= sxf[i] * (tx(i + 1, j) * sxb[ip] + tx(i, j) * sxb[i]) / dx ** 2
+            K[k, idx(i - 1, j)] += sxf[i] * tx(i, j) * sxb[i] / dx ** 2
+            jp = (j + 1) % Ny
+            K[k, idx(i, j + 1)] += syf[j] * ty(i, j + 1) * syb[jp] / dx ** 2
+            K[k, k] -= syf[j] * (ty(i, j + 1) * syb[jp] + ty(i, j) * syb[j]) / dx ** 2
+            K[k, idx(i, j - 1)] += syf[j] * ty(i, j) * syb[j] / dx ** 2
+            K[k, k] += om ** 2 * MU0n
+    assert np.abs(A - K).max() <= 1e-12 * np.abs(K).max()
+
+
+# --- solve, fields, flux -----------------------------------------------------
+def test_solve_residual_and_dense_agreement():
+    g = O.Grid.create(0.02, (4, 4), (0, 0.4), (0, 0.36))
+    d = O.Device(g, O.omega_of_wavelength(1.55))
+    rng = np.random.default_rng(5)
+    d.eps_r[:] = 1 + 11.25 * (rng.random(g.N) < 0.5)
+    d.src[10, 9] = 1j
+    f, systems = O.solve(d, O.TM, return_system=True)
+    A, b, _ = systems[0]
+    ez = f.data[:, :, 0].ravel(order="F")
+    assert O.rel_residual(A, ez, b) < 1e-14
+    xd = np.linalg.solve(A.toarray(), b)
+    assert np.linalg.norm(ez - xd) / np.linalg.norm(xd) < 1e-11
+
+
+def test_tm_fields_satisfy_ampere():
+    """Recovered H satisfies dxf Hy - dyf Hx - i w eps0 eps Ez = src, which
+    follows from A ez = i w src and the definitions of Hx, Hy; checks the
+    H recovery (driven.jl:40-41) through the physics, not the formula."""
+    d = O.config_c1()
+    f = O.solve(d, O.TM)
+    g = d.grid
+    om = float(d.omega[0])
+    dxf, dxb, dyf, dyb = O.stretched_deltas(g, om)
+    ez, hx, hy = (f.data[:, :, c].ravel(order="F") for c in range(3))
+    lhs = dxf @ hy - dyf @ hx - 1j * om * EPS0n * d.eps_r.ravel(order="F") * ez
+    src = d.src.ravel(order="F")
+    assert np.linalg.norm(lhs - src) / np.linalg.norm(src) < 1e-9
+
+
+def test_flux_indexing_hand_case():
+    g = O.Grid.create(1.0, (0, 0), (0, 4.0), (0, 6.0))
+    data = np.zeros((4, 6, 3), dtype=complex)
+    data[:, :, 0] = 2.0   # Ez
+    data[:, :, 2] = 0.5j  # Hy
+    data[2, :, 0] = 4.0   # column 3 (x = 2.5), averaged with column 2
+    f = O.FieldTM(g, 1.0, data)
+    # centre x=1.5 -> column index 1 (0-based); rows with |y - 3| <= 1: y = 2.5, 3.5
+    val = O.flux_surface_integral_tm_x(f, 1.5, 3.0, 1.0)
+    ez_avg = (2.0 + 4.0) / 2
+    assert val == pytest.approx(2 * (-0.5 * np.real(ez_avg * np.conj(0.5j))) * 1.0)
+
+
+def test_flux_conservation_lossless_slab():
+    """Npml_y = 0 (periodic, lossless in y): the power through two vertical
+    planes on the same side of the source agrees up to the half-cell
+    staggering of the reference's flux formula."""
+    g = O.Grid.create(0.02, (20, 0), (0, 4.0), (0, 1.0))
+    d = O.Device(g, O.omega_of_wavelength(1.55))
+    d.eps_r[:, 20:31] = 12.25
+    d.src[50, :] = 1.0
+    f = O.solve(d, O.TM)
+    p1 = O.flux_surface_integral_tm_x(f, 2.01, 0.5, 0.6)
+    p2 = O.flux_surface_integral_tm_x(f, 3.01, 0.5, 0.6)
+    assert p1 > 0 and abs(p1 - p2) / abs(p1) < 2e-2
+
+
+# --- eigenfrequency (src/solver/eigen.jl) ------------------------------------
+def test_eigenfrequency_fft_closed_form():
+    Nx, Ny, dx = 12, 10, 0.05
+    g = O.Grid.create(dx, (0, 0), (0, Nx * dx), (0, Ny * dx))
+    om0 = 2 * np.pi * C0n / 1.2
+    d = O.Device(g, om0)
+    eps = 2.0
+    d.eps_r[:] = eps
+    w, fields, lam, sigma = O.eigenfrequency(d, O.TM, 4)
+    P, Q = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    lam_all = (-4 * (np.sin(np.pi * P / Nx) ** 2 / dx ** 2 + np.sin(np.pi * Q / Ny) ** 2 / dx ** 2)
+               / eps).ravel()
+    near = lam_all[np.argsort(np.abs(lam_all - sigma))]
+    # degenerate clusters: compare the sorted distances
+    assert np.allclose(np.sort(np.abs(lam - sigma)), np.sort(np.abs(near[:4] - sigma)), rtol=1e-9)
+    # omega = sqrt(-lambda/(mu0 eps0)) (eigen.jl:87)
+    assert np.allclose(w ** 2, -lam / MU0n / EPS0n, rtol=1e-12,
+                       atol=1e-12 * np.abs(lam).max() / MU0n / EPS0n)
+
+
+def test_eigenfrequency_kronecker_with_pml():
+    Nx, Ny, dx = 14, 12, 0.05
+    g = O.Grid.create(dx, (3, 3), (0, Nx * dx), (0, Ny * dx))
+    om0 = 2 * np.pi * C0n / 1.0
+    d = O.Device(g, om0)
+    d.eps_r[:] = 1.0
+    w, fields, lam, sigma = O.eigenfrequency(d, O.TM, 5)
+    sxf, sxb, syf, syb = O.sfactor_vectors(g, om0)
+    a = np.linalg.eigvals(_op1d(Nx, dx, sxf, sxb))
+    b = np.linalg.eigvals(_op1d(Ny, dx, syf, syb))
+    allv = (a[:, None] + b[None, :]).ravel()
+    near = allv[np.argsort(np.abs(allv - sigma))][:5]
+    assert np.allclose(np.sort(np.abs(lam - sigma)), np.sort(np.abs(near - sigma)), rtol=1e-8)
+
+
+# --- adjoint gradient (new; checked by central finite differences) ----------
+def test_adjoint_gradient_matches_fd():
+    g = O.Grid.create(0.02, (4, 4), (0, 0.5), (0, 0.4))
+    om = O.omega_of_wavelength(1.55)
+    rng = np.random.default_rng(11)
+    eps = 1 + 11.25 * rng.random(g.N)
+    src = np.zeros(g.N, dtype=complex)
+    src[6, 10] = 1j
+    mvec = np.zeros(g.N, dtype=complex)
+    mvec[18, 5:15] = np.exp(-((np.arange(5, 15) - 10) / 3.0) ** 2)
+    m = mvec.ravel(order="F")
+
+    def J(e_r):
+        A = O.assemble(g, e_r, om, O.TM)
+        e, _ = O.linear_solve(A, O.rhs(om, src))
+        s = m @ e
+        return abs(s) ** 2, A, e, s
+
+    J0, A, e, s = J(eps)
+    lam, _ = O.linear_solve(A, m, trans="T")
+    grad = -np.real(2 * np.conj(s) * lam * om ** 2 * EPS0n * e)
+    h = 1e-4
+    for k in [(6, 12), (12, 8), (18, 10), (3, 3)]:
+        kk = k[0] + k[1] * g.N[0]
+        ep, em = eps.copy(), eps.copy()
+        ep[k] += h
+        em[k] -= h
+        fd = (J(ep)[0] - J(em)[0]) / (2 * h)
+        assert abs(grad[kk] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(grad).max())
