@@ -1,17 +1,217 @@
-// Batched driven solves (many grids x frequencies x sources).
-#include "runtime.h"
-#include "../../include/fdfd.h"
+// Batched driven solves: many grids x frequencies x sources.
+//
+// The reference solves one device at a time and refactors per frequency
+// (src/solver/driven.jl:11-55), each source needing its own solve() call.
+// Here all frequencies of one grid that fit in HBM are factored together:
+// every factorization kernel takes a job table, so B frequencies become 2B
+// independent elimination chains in the same launches (B times the work per
+// launch for the same number of launches).  Each (frequency, source) pair is
+// then solved with FP64 FGMRES on its factor; fields stay in device memory
+// until fdfd_batch_result copies one set out.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
-struct fdfd_batch { fdfd_handle* h; };
+#include "../../include/fdfd.h"
+#include "runtime.h"
+
+namespace fdfd {
+int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, double tol, int* iters,
+             double* resid);
+int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);
+void rt_free_factor(Factor* F);
+}  // namespace fdfd
+
+using namespace fdfd;
+
+struct BatchJob {
+    GridDesc g{};
+    int pol = FDFD_TM;
+    std::vector<double2> eps;
+    std::vector<double> omegas;
+    int nsrc = 0;
+    std::vector<double2> srcs;
+    double2* result = nullptr;  // device: nomega * nsrc * 3N
+};
+
+struct fdfd_batch {
+    fdfd_handle* hh = nullptr;
+    std::vector<BatchJob> jobs;
+    bool ran = false;
+};
+
+static void free_results(fdfd_batch* b) {
+    for (auto& j : b->jobs)
+        if (j.result) {
+            (void)hipFree(j.result);
+            j.result = nullptr;
+        }
+}
+
 extern "C" {
-int fdfd_batch_create(fdfd_handle* h, fdfd_batch** out) {
-    return fdfd::fail(h ? &h->impl : nullptr, FDFD_ERR_INVALID, "batch: not built");
+
+int fdfd_batch_create(fdfd_handle* hh, fdfd_batch** out) {
+    if (!hh || !out) return FDFD_ERR_INVALID;
+    auto* b = new fdfd_batch();
+    b->hh = hh;
+    *out = b;
+    return FDFD_OK;
 }
-int fdfd_batch_add(fdfd_batch*, int, int, double, double, int, int, double, int,
-                   const fdfd_complex*, int, const double*, int, const fdfd_complex*, int*) {
-    return FDFD_ERR_INVALID;
+
+int fdfd_batch_add(fdfd_batch* b, int Nx, int Ny, double dx, double dy, int npml_x, int npml_y,
+                   double L0, int pol, const fdfd_complex* eps, int nomega, const double* omegas,
+                   int nsrc, const fdfd_complex* srcs, int* job_id) {
+    if (!b || !eps || !omegas || !srcs || nomega < 1 || nsrc < 1) return FDFD_ERR_INVALID;
+    Handle* h = &b->hh->impl;
+    if (Nx < 1 || Ny < 1 || !(dx > 0) || !(dy > 0) || npml_x < 0 || npml_y < 0 || !(L0 > 0))
+        return fail(h, FDFD_ERR_INVALID, "batch job: invalid grid parameters");
+    if (std::min(Nx, Ny) > MAX_LINE)
+        return fail(h, FDFD_ERR_INVALID, "batch job: min(Nx, Ny) must be <= 1024");
+    if (pol != FDFD_TM && pol != FDFD_TE) return fail(h, FDFD_ERR_INVALID, "batch job: bad pol");
+    for (int i = 0; i < nomega; ++i)
+        if (!(omegas[i] > 0)) return fail(h, FDFD_ERR_INVALID, "batch job: omega must be > 0");
+    BatchJob j;
+    j.g = GridDesc{Nx, Ny, dx, dy, npml_x, npml_y, L0};
+    j.pol = pol;
+    const long N = (long)Nx * Ny;
+    j.eps.assign((const double2*)eps, (const double2*)eps + N);
+    j.omegas.assign(omegas, omegas + nomega);
+    j.nsrc = nsrc;
+    j.srcs.assign((const double2*)srcs, (const double2*)srcs + N * nsrc);
+    b->jobs.push_back(std::move(j));
+    if (job_id) *job_id = (int)b->jobs.size() - 1;
+    b->ran = false;
+    return FDFD_OK;
 }
-int fdfd_batch_run(fdfd_batch*, int, double, fdfd_stats*) { return FDFD_ERR_INVALID; }
-int fdfd_batch_result(fdfd_batch*, int, int, int, fdfd_complex*) { return FDFD_ERR_INVALID; }
-int fdfd_batch_destroy(fdfd_batch*) { return FDFD_ERR_INVALID; }
+
+int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) {
+    if (!b) return FDFD_ERR_INVALID;
+    fdfd_handle* hh = b->hh;
+    Handle* h = &hh->impl;
+    if (precision != FDFD_FP64 && precision != FDFD_FP32_IR)
+        return fail(h, FDFD_ERR_INVALID, "precision must be FDFD_FP64 or FDFD_FP32_IR");
+    if (!(tol > 0)) return fail(h, FDFD_ERR_INVALID, "tol must be > 0");
+    free_results(b);
+    int iters = 0, pivot_fallback = 0;
+    double worst = 0, t_factor = 0, t_solve = 0, growth = 0;
+    int rc = FDFD_OK;
+    for (auto& job : b->jobs) {
+        const GridDesc& g = job.g;
+        const long N = (long)g.Nx * g.Ny;
+        if ((rc = fdfd_set_grid(hh, g.Nx, g.Ny, g.dx, g.dy, g.npx, g.npy, g.L0))) break;
+        if ((rc = fdfd_set_eps(hh, (const fdfd_complex*)job.eps.data()))) break;
+        const size_t nres = (size_t)job.omegas.size() * job.nsrc * 3 * N;
+        if (hipMalloc(&job.result, sizeof(double2) * nres) != hipSuccess) {
+            job.result = nullptr;
+            rc = fail(h, FDFD_ERR_OOM, "batch: result buffer allocation failed");
+            break;
+        }
+        double2* dsrc = nullptr;
+        if (hipMalloc(&dsrc, sizeof(double2) * N * job.nsrc) != hipSuccess ||
+            hipMemcpy(dsrc, job.srcs.data(), sizeof(double2) * N * job.nsrc,
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            if (dsrc) (void)hipFree(dsrc);
+            rc = fail(h, FDFD_ERR_OOM, "batch: source upload failed");
+            break;
+        }
+        // frequencies per group: what the free HBM holds (factors + operator)
+        const int n = std::min(g.Nx, g.Ny), m = std::max(g.Nx, g.Ny);
+        const double elem = (precision == FDFD_FP64) ? 16.0 : 8.0;
+        const double per = (double)m * n * n * elem + 96.0 * N + 16.0 * m * n;
+        size_t freeb = 0, totalb = 0;
+        (void)hipMemGetInfo(&freeb, &totalb);
+        int B = (int)std::max(1.0, std::floor(0.8 * (double)freeb / per - 1.0));
+        B = std::min<int>({B, (int)job.omegas.size(), 64});
+        for (size_t o0 = 0; o0 < job.omegas.size() && rc == FDFD_OK; o0 += B) {
+            const int nb = (int)std::min<size_t>(B, job.omegas.size() - o0);
+            std::vector<Factor*> Fs;
+            for (int q = 0; q < nb; ++q) {
+                Factor* F = new Factor();
+                F->pol = job.pol;
+                F->omega = job.omegas[o0 + q];
+                F->precision = precision;
+                Fs.push_back(F);
+            }
+            rc = rt_build_factors(h, Fs);
+            if (rc == FDFD_OK) {
+                t_factor += Fs[0]->t_factor_ms;
+                hipEvent_t e0, e1;
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                (void)hipEventRecord(e0, h->stream);
+                const double eps0n = EPS0 * g.L0, mu0n = MU0 * g.L0;
+                for (int q = 0; q < nb && rc == FDFD_OK; ++q) {
+                    Factor* F = Fs[q];
+                    growth = std::max(growth, F->pivot_growth);
+                    pivot_fallback |= F->pivot_fallback;
+                    for (int sidx = 0; sidx < job.nsrc && rc == FDFD_OK; ++sidx) {
+                        // b' = scale * (i omega) * src   (driven.jl:36,46)
+                        launch_scale_copy(h->stream, h->bvec, dsrc + (long)sidx * N,
+                                          make_double2(0.0, F->scale * F->omega), N);
+                        launch_zero(h->stream, h->xvec, N);
+                        double res = 0;
+                        rc = rt_gmres(h, F, 0, h->bvec, h->xvec, tol, &iters, &res);
+                        worst = std::max(worst, res);
+                        if (rc) break;
+                        const double c = (F->pol == FDFD_TM) ? mu0n : eps0n;
+                        double2 w = make_double2(0.0, -1.0 / (F->omega * c));
+                        double2* out = job.result + ((o0 + q) * job.nsrc + sidx) * 3 * N;
+                        launch_fields(h->stream, out, h->xvec, h->eps, F->sxf, F->sxb, F->syf,
+                                      F->syb, g, F->pol, w, 0);
+                    }
+                }
+                (void)hipEventRecord(e1, h->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                t_solve += ms;
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            }
+            for (Factor* F : Fs) rt_free_factor(F);
+        }
+        (void)hipFree(dsrc);
+        if (rc) break;
+    }
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->iterations = iters;
+        stats->residual = worst;
+        stats->t_factor_ms = t_factor;
+        stats->t_solve_ms = t_solve;
+        stats->pivot_growth = growth;
+        stats->singular_line = -1;
+        stats->pivot_fallback = pivot_fallback;
+    }
+    if (rc) {
+        free_results(b);
+        return rc;
+    }
+    b->ran = true;
+    return FDFD_OK;
 }
+
+int fdfd_batch_result(fdfd_batch* b, int job_id, int iomega, int isrc, fdfd_complex* fields) {
+    if (!b || !fields) return FDFD_ERR_INVALID;
+    Handle* h = &b->hh->impl;
+    if (!b->ran) return fail(h, FDFD_ERR_STATE, "fdfd_batch_run has not completed");
+    if (job_id < 0 || job_id >= (int)b->jobs.size()) return fail(h, FDFD_ERR_NOTFOUND, "bad job id");
+    const BatchJob& j = b->jobs[job_id];
+    if (iomega < 0 || iomega >= (int)j.omegas.size() || isrc < 0 || isrc >= j.nsrc)
+        return fail(h, FDFD_ERR_INVALID, "frequency or source index out of range");
+    const long N = (long)j.g.Nx * j.g.Ny;
+    const double2* src = j.result + ((long)iomega * j.nsrc + isrc) * 3 * N;
+    if (hipMemcpy(fields, src, sizeof(double2) * 3 * N, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, FDFD_ERR_HIP, "copy of batch result failed");
+    return FDFD_OK;
+}
+
+int fdfd_batch_destroy(fdfd_batch* b) {
+    if (!b) return FDFD_ERR_INVALID;
+    free_results(b);
+    delete b;
+    return FDFD_OK;
+}
+
+}  // extern "C"

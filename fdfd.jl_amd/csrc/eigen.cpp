@@ -65,7 +65,8 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
     if (rc) { rt_free_factor(F); return rc; }
     const double sigma = (pol == FDFD_TM) ? -omega0 * omega0 * mu0n * eps0n : -omega0 * omega0 * mu0n;
     const double opscale = (pol == FDFD_TM) ? g.dx * g.dx : eps0n * g.dx * g.dx;
-    const double inner_tol = std::min(1e-12, tol * 1e-2);
+    // accuracy of each shift-invert application (FP64 FGMRES on the factors)
+    const double inner_tol = std::max(1e-13, std::min(1e-12, tol * 1e-1));
 
     double2* Vb = nullptr;  // ncv + 1 Arnoldi vectors
     double2* Wb = nullptr;  // restart workspace

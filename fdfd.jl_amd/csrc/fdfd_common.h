@@ -100,9 +100,11 @@ struct LineJob {
     int* kap;         // n: column map, X[a,b] = Y[rho(a), kap(b)]
     int* kapinv;      // n: inverse of kap
     void* Vbuf;       // n x PMAX panel copy (original panel values)
-    void* Vpinv;      // PMAX x PMAX inverse of the pivot block
+    void* LU;         // 2 x PMAX x PMAX: unit-lower L then upper U, V[piv, :] = L U
     int* piv;         // PMAX: pivot rows of the current panel
-    void* Z;          // PMAX x n: Vpinv * [A(piv, :) | I]
+    void* Z;          // PMAX x n: (L U)^-1 * [A(piv, :) | I]
+    int* win;         // 64: candidate-row window (first unpivoted rows by index)
+    int* verr;        // set to 1 when a row outside the window beats a chosen pivot
     double* pivstat;  // [0] = min |pivot|, [1] = max |pivot|, [2] = zero-pivot flag
     int n;
 };
@@ -122,6 +124,7 @@ struct SchurJob {
     long su1, su2;            // stride of the u vectors (neighbour line)
     int* q;                   // out: row map of S'
     int* used; int* iprow;    // reset for the new line
+    int* win;                 // candidate window reset to rows 0..WIN-1
     int n;
 };
 
@@ -172,7 +175,11 @@ void launch_schur(hipStream_t s, const SchurJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_panel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
 template <typename R>
-void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
+void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
+                   int fresh);
+template <typename R>
+void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
+                   int fresh, int verify);
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
 void launch_maps(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
@@ -182,7 +189,9 @@ void launch_sweep(hipStream_t s, const SweepJob* jobs_d, int njobs, int n, int t
 
 // largest line-block size the panel kernel supports (one row per thread)
 constexpr int MAX_LINE = 1024;
-// widest Gauss-Jordan panel; row stride of the panel copy and of Vpinv
-constexpr int PMAX = 16;
+// widest Gauss-Jordan panel; row stride of the panel copy and of L, U
+constexpr int PMAX = 32;
+// rows in the pivot-candidate window of the fast panel (one wave)
+constexpr int WIN = 64;
 
 }  // namespace fdfd

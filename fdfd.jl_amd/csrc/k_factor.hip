@@ -18,6 +18,7 @@
 #include "fdfd_common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fdfd {
 
@@ -25,6 +26,14 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 template <typename R> __device__ inline R shfl_x(R v, int o) { return __shfl_xor(v, o, 64); }
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Schur formation: one block per row r of S'.
@@ -41,6 +50,7 @@ __global__ __launch_bounds__(256) void k_schur(const SchurJob* jobs, int n) {
         J.q[r] = a;
         J.used[r] = 0;
         J.iprow[r] = -1;
+        if (r < WIN) J.win[r] = r;
     }
     if (J.Y1) {
         const C* y1 = (const C*)J.Y1 + (long)r * n;
@@ -76,10 +86,163 @@ template void launch_schur<double>(hipStream_t, const SchurJob*, int, int);
 template void launch_schur<float>(hipStream_t, const SchurJob*, int, int);
 
 // ---------------------------------------------------------------------------
-// Panel: pivot selection (GEPP on unpivoted rows) + inverse of the pivot block.
-// blockDim = 1024, thread t owns row t (n <= 1024).  The panel width NB is a
-// compile-time constant (16, 8, 4, 2 or 1) so the row segment W[] lives in
-// registers; inner loops have constant bounds and are predicated on cc.
+// Broadcast of a lane's value to the whole wave (lane index wave-uniform).
+// ---------------------------------------------------------------------------
+__device__ inline float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double bcast(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 64-bit DPP move within a 16-lane row (all 64 lanes active)
+template <int CTRL> __device__ inline unsigned long long dpp64(unsigned long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffffULL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ inline unsigned long long readlane64(unsigned long long v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(v & 0xffffffffULL), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// wave-wide maximum of an unsigned 64-bit key: quad xor-1 / xor-2 and row
+// rotations by 4 and 8 give every lane its row maximum, then the four rows
+// are combined through scalar reads.  Result is wave-uniform.
+__device__ inline unsigned long long wave_max_u64(unsigned long long k) {
+    unsigned long long o;
+    o = dpp64<0xB1>(k);  k = o > k ? o : k;   // quad_perm [1,0,3,2]
+    o = dpp64<0x4E>(k);  k = o > k ? o : k;   // quad_perm [2,3,0,1]
+    o = dpp64<0x124>(k); k = o > k ? o : k;   // row_ror:4
+    o = dpp64<0x128>(k); k = o > k ? o : k;   // row_ror:8
+    unsigned long long a = readlane64(k, 0), b = readlane64(k, 16);
+    unsigned long long c = readlane64(k, 32), d = readlane64(k, 48);
+    a = b > a ? b : a;
+    c = d > c ? d : c;
+    return c > a ? c : a;
+}
+// monotone key of a non-negative magnitude
+__device__ inline unsigned long long mag_key(double m) { return (unsigned long long)__double_as_longlong(m); }
+__device__ inline unsigned long long mag_key(float m) { return (unsigned long long)(unsigned)__float_as_int(m); }
+
+// store the L/U rows of panel step cc and the pivot bookkeeping
+template <typename C> __device__ inline void put_lu_row(C* L, C* U, int cc, int c, C v) {
+    using R = decltype(C{}.x);
+    L[cc * PMAX + c] = mk<C>((c < cc) ? v.x : (c == cc ? R(1) : R(0)), (c < cc) ? v.y : R(0));
+    U[cc * PMAX + c] = mk<C>((c >= cc) ? v.x : R(0), (c >= cc) ? v.y : R(0));
+}
+
+// ---------------------------------------------------------------------------
+// Fast panel (one wave): GEPP on the candidate window, the first WIN
+// unpivoted rows by index.  Lane l owns window row win[l].  The pivot
+// sequence equals GEPP's on all unpivoted rows whenever no row outside the
+// window would have won a step; k_zbuild checks exactly that (verr).
+// Outputs: pivots, L and U of V[piv, :], updated window list.
+// ---------------------------------------------------------------------------
+template <typename R, int NB>
+__global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh) {
+    using C = cplx<R>;
+    const LineJob J = jobs[blockIdx.x];
+    const int n = J.n;
+    const int lane = threadIdx.x;
+    const int nwin = min(WIN, n - k0);
+    __shared__ int prs[NB];
+    __shared__ R pmags[NB];
+    const int row = (lane < nwin) ? J.win[lane] : -1;
+    bool act = row >= 0;
+    bool chosen = false;
+    const C* A = (const C*)J.Y;
+    C* L = (C*)J.LU;
+    C* U = L + PMAX * PMAX;
+    C W[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        W[c] = (act && c < nb) ? A[(long)row * n + k0 + c] : mk<C>(R(0), R(0));
+    static_for<0, NB>([&](auto ic) {
+        constexpr int cc = decltype(ic)::value;
+        if (cc < nb) {
+            // argmax of |.|_1 over the active window rows; lanes hold the
+            // window in ascending row order, so the lowest lane among exact
+            // ties is the smallest row index (GEPP's tie rule)
+            const R mag = cabs1(W[cc]);
+            const unsigned long long key = act ? mag_key(mag) : 0ULL;
+            const unsigned long long kmax = wave_max_u64(key);
+            unsigned long long hit = __ballot(act && key == kmax);
+            if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
+            const int pl = (int)__ffsll((long long)hit) - 1;  // wave-uniform
+            const R bm = act ? mag : R(-1);
+            if (lane == pl) {
+                prs[cc] = row;
+                pmags[cc] = bm;
+                act = false;
+                chosen = true;
+#pragma unroll
+                for (int c = 0; c < NB; ++c) put_lu_row(L, U, cc, c, W[c]);
+            }
+            // pivot-row entries are broadcast one at a time (uniform control
+            // flow); the pivot lane's registers are not modified in this step
+            const C ucc = mk<C>(bcast(W[cc].x, pl), bcast(W[cc].y, pl));
+            const R pivmag = cabs1(ucc);
+            const C m = csel(act && pivmag > R(0), cmul(W[cc], crcp(ucc)), mk<C>(R(0), R(0)));
+            if (act) W[cc] = m;
+#pragma unroll
+            for (int c = cc + 1; c < NB; ++c) {
+                const C uc = mk<C>(bcast(W[c].x, pl), bcast(W[c].y, pl));
+                if (act) W[c] = csub(W[c], cmul(m, uc));
+            }
+        }
+    });
+    // identity padding of L and U beyond nb
+    if (lane >= nb && lane < PMAX) {
+        for (int c = 0; c < PMAX; ++c) {
+            L[lane * PMAX + c] = mk<C>(c == lane ? R(1) : R(0), R(0));
+            U[lane * PMAX + c] = mk<C>(c == lane ? R(1) : R(0), R(0));
+        }
+    }
+    __syncthreads();
+    if (lane < nb) {
+        const int p = prs[lane];
+        J.piv[lane] = p;
+        J.prow[k0 + lane] = p;
+        J.iprow[p] = k0 + lane;
+        J.used[p] = 1;
+    }
+    if (lane == 0) {
+        double mn = J.pivstat[0], mx = J.pivstat[1];
+        for (int c = 0; c < nb; ++c) {
+            double v = (double)pmags[c];
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+            if (!(v > 0)) J.pivstat[2] = 1.0;
+        }
+        J.pivstat[0] = mn;
+        J.pivstat[1] = mx;
+    }
+    // new window: surviving rows, then the next nb fresh rows
+    const bool keep = (row >= 0) && !chosen;
+    const unsigned long long km = __ballot(keep);
+    const int below = __popcll(km & ((1ull << lane) - 1ull));
+    const int nkeep = __popcll(km);
+    if (keep) J.win[below] = row;
+    if (lane < nb && fresh + lane < n) J.win[nkeep + lane] = fresh + lane;
+}
+
+template <typename R>
+void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
+                   int fresh) {
+    (void)n;
+    k_wpanel<R, PMAX><<<njobs, 64, 0, s>>>(jobs_d, k0, nb, fresh);
+}
+template void launch_wpanel<double>(hipStream_t, const LineJob*, int, int, int, int, int);
+template void launch_wpanel<float>(hipStream_t, const LineJob*, int, int, int, int, int);
+
+// ---------------------------------------------------------------------------
+// Full panel (fallback, exact GEPP over every unpivoted row): one workgroup of
+// 1024 threads, thread t owns row t.  NB is a compile-time width (16 .. 1) so
+// the row segment stays in registers.  Outputs as k_wpanel plus the panel copy.
 // ---------------------------------------------------------------------------
 template <typename R, int NB>
 __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
@@ -90,24 +253,18 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
     __shared__ R cmag[2][16];
     __shared__ int crow[2][16];
     __shared__ C cval[2][16][NB];
-    __shared__ C Ls[NB][NB];
-    __shared__ C Us[NB][NB];
-    __shared__ C Li[NB][NB];
-    __shared__ C Ui[NB][NB];
     __shared__ int prs[NB];
     __shared__ R pmags[NB];
-    C* A = (C*)J.Y;
-    C* Vb = (C*)J.Vbuf;
+    const C* A = (const C*)J.Y;
+    C* Lg = (C*)J.LU;
+    C* Ug = Lg + PMAX * PMAX;
     const int r = t;
     const bool valid = r < n;
     bool act = valid && (J.used[r] == 0);
     const C zero = mk<C>(R(0), R(0));
     C W[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        W[c] = valid ? A[(long)r * n + k0 + c] : mk<C>(R(0), R(0));
-        if (valid) Vb[(long)r * PMAX + c] = W[c];
-    }
+    for (int c = 0; c < NB; ++c) W[c] = valid ? A[(long)r * n + k0 + c] : mk<C>(R(0), R(0));
     int buf = 0;
 #pragma unroll
     for (int cc = 0; cc < NB; ++cc) {
@@ -141,11 +298,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
             prs[cc] = pr;
             pmags[cc] = pm;
         }
-        if (t < NB) {
-            C v = cval[buf][pw][t];
-            Ls[cc][t] = mk<C>((t < cc) ? v.x : (t == cc ? R(1) : R(0)), (t < cc) ? v.y : R(0));
-            Us[cc][t] = mk<C>((t >= cc) ? v.x : R(0), (t >= cc) ? v.y : R(0));
-        }
+        if (t < PMAX) put_lu_row(Lg, Ug, cc, t, (t < NB) ? cval[buf][pw][t < NB ? t : 0] : zero);
         if (r == pr) act = false;
         if (act) {
             const C ucc = cval[buf][pw][cc];
@@ -158,36 +311,11 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    // Triangular inverses: column c handled by the NB lanes t = c*NB + i
-    // (NB divides 64, so a column never straddles two waves).
-    if (t < NB * NB) {
-        const int c = t / NB, i = t % NB;
-        const int gbase = (lane / NB) * NB;
-        // Li = L^-1 (unit lower), forward substitution
-        C x = mk<C>(i == c ? R(1) : R(0), R(0));
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            R xr = __shfl(x.x, gbase + k, 64), xi = __shfl(x.y, gbase + k, 64);
-            if (i > k) x = csub(x, cmul(Ls[i][k], mk<C>(xr, xi)));
+    if (t >= NB && t < PMAX) {
+        for (int c = 0; c < PMAX; ++c) {
+            Lg[t * PMAX + c] = mk<C>(c == t ? R(1) : R(0), R(0));
+            Ug[t * PMAX + c] = mk<C>(c == t ? R(1) : R(0), R(0));
         }
-        Li[i][c] = x;
-        // Ui = U^-1 (upper), right-looking back substitution
-        C y = mk<C>(i == c ? R(1) : R(0), R(0));
-#pragma unroll
-        for (int k = NB - 1; k >= 0; --k) {
-            if (i == k) y = cmul(y, crcp(Us[k][k]));
-            R yr = __shfl(y.x, gbase + k, 64), yi = __shfl(y.y, gbase + k, 64);
-            if (i < k) y = csub(y, cmul(Us[i][k], mk<C>(yr, yi)));
-        }
-        Ui[i][c] = y;
-    }
-    __syncthreads();
-    // Vpinv = U^-1 L^-1
-    for (int e = t; e < NB * NB; e += blockDim.x) {
-        const int i = e / NB, j = e % NB;
-        C acc = zero;
-        for (int k = (i > j ? i : j); k < NB; ++k) acc = cadd(acc, cmul(Ui[i][k], Li[k][j]));
-        ((C*)J.Vpinv)[i * PMAX + j] = acc;
     }
     if (t < NB) {
         const int p = prs[t];
@@ -200,8 +328,8 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         double mn = J.pivstat[0], mx = J.pivstat[1];
         for (int c = 0; c < NB; ++c) {
             double v = (double)pmags[c];
-            if (v < mn) mn = v;
-            if (v > mx) mx = v;
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
             if (!(v > 0)) J.pivstat[2] = 1.0;
         }
         J.pivstat[0] = mn;
@@ -224,53 +352,98 @@ template void launch_panel<double>(hipStream_t, const LineJob*, int, int, int, i
 template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// Z = Vpinv * T, T[k][j] = delta(k, j-k0) for j in K, else A[piv[k]][j].
-// One block per 64 columns.
+// Z build, panel copy and pivot check.
+//   blocks [0, nzb):  Z[:, col] = U^-1 L^-1 T[:, col], T[k][col] = A[piv[k]][col]
+//                     (delta(k, col-k0) for col in K).  A column is solved by a
+//                     group of PMAX lanes with shuffle substitution; a block
+//                     of 256 threads handles 8 columns.
+//   blocks [nzb, ..): copy V = A[:, K] to Vbuf (every row) and, if verify,
+//                     check rows r >= fresh: the value GEPP would compare at
+//                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
+//                     y_c = a_c / U[c][c], must not exceed |U[c][c]|.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int nb) {
-    constexpr int NB = PMAX;
+__device__ inline cplx<R> bcast_c(cplx<R> v, int l) {
+    return mk<cplx<R>>(bcast(v.x, l), bcast(v.y, l));
+}
+
+// blockDim = 512 (8 waves); each wave handles ZB columns (Z part) or ZB rows
+// (copy / check part), lane i < nb holding entry i of the small vector.
+constexpr int ZB = 2;
+template <typename R>
+__global__ __launch_bounds__(512) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
+                                                int verify, int nzb) {
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ C Vs[NB][NB];
-    __shared__ C Ts[NB][64];
-    const int t = threadIdx.x;
-    const int j0 = blockIdx.x * 64;
-    const C* A = (const C*)J.Y;
-    for (int e = t; e < NB * NB; e += 256) {
-        const int i = e / NB, j = e % NB;
-        Vs[i][j] = (i < nb && j < nb) ? ((const C*)J.Vpinv)[i * PMAX + j] : mk<C>(R(0), R(0));
-    }
-    for (int e = t; e < NB * 64; e += 256) {
-        int k = e / 64, jj = e % 64, j = j0 + jj;
-        C v = mk<C>(R(0), R(0));
-        if (k < nb && j < n) {
-            if (j >= k0 && j < k0 + nb) v = mk<C>(R(j - k0 == k ? 1 : 0), R(0));
-            else v = A[(long)J.piv[k] * n + j];
-        }
-        Ts[k][jj] = v;
+    __shared__ C Ls[PMAX][PMAX + 1];
+    __shared__ C Us[PMAX][PMAX + 1];
+    __shared__ C Ud[PMAX];  // 1 / U[k][k]
+    const C* Lg = (const C*)J.LU;
+    const C* Ug = Lg + PMAX * PMAX;
+    for (int e = threadIdx.x; e < PMAX * PMAX; e += blockDim.x) {
+        Ls[e / PMAX][e % PMAX] = Lg[e];
+        Us[e / PMAX][e % PMAX] = Ug[e];
     }
     __syncthreads();
-    C* Z = (C*)J.Z;
-    for (int e = t; e < NB * 64; e += 256) {
-        int c = e / 64, jj = e % 64, j = j0 + jj;
-        if (c < nb && j < n) {
-            C acc = mk<C>(R(0), R(0));
-#pragma unroll
-            for (int k = 0; k < NB; ++k) acc = cadd(acc, cmul(Vs[c][k], Ts[k][jj]));
-            Z[(long)c * n + j] = acc;
+    if (threadIdx.x < PMAX) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
+    __syncthreads();
+    const C* A = (const C*)J.Y;
+    const int i = threadIdx.x & 63;  // lane = row of the small system
+    const int wv = threadIdx.x >> 6;
+    const bool il = i < nb;
+    if ((int)blockIdx.x < nzb) {
+        for (int q = 0; q < ZB; ++q) {
+            const int col = (blockIdx.x * 8 + wv) * ZB + q;
+            if (col >= n) break;
+            C x = mk<C>(R(0), R(0));
+            if (il) {
+                if (col >= k0 && col < k0 + nb) x = mk<C>(col - k0 == i ? R(1) : R(0), R(0));
+                else x = A[(long)J.piv[i] * n + col];
+            }
+            for (int k = 0; k < nb; ++k) {  // L y = t (unit lower)
+                const C xk = bcast_c<R>(x, k);
+                if (il && i > k) x = csub(x, cmul(Ls[i][k], xk));
+            }
+            for (int k = nb - 1; k >= 0; --k) {  // U z = y
+                if (i == k) x = cmul(x, Ud[k]);
+                const C xk = bcast_c<R>(x, k);
+                if (il && i < k) x = csub(x, cmul(Us[i][k], xk));
+            }
+            if (il) ((C*)J.Z)[(long)i * n + col] = x;
         }
+    } else {
+        C* Vb = (C*)J.Vbuf;
+        int bad = 0;
+        for (int q = 0; q < ZB; ++q) {
+            const int r = (((int)blockIdx.x - nzb) * 8 + wv) * ZB + q;
+            if (r >= n) break;
+            C v = il ? A[(long)r * n + k0 + i] : mk<C>(R(0), R(0));
+            if (i < PMAX) Vb[(long)r * PMAX + i] = v;
+            if (verify && r >= fresh) {
+                // the value GEPP compares at step k: a_k = V[r][k] - sum y_c U[c][k]
+                for (int k = 0; k < nb; ++k) {
+                    const C ak = bcast_c<R>(v, k);  // wave-uniform
+                    if (cabs1(ak) > cabs1(Us[k][k])) bad = 1;
+                    const C yk = cmul(ak, Ud[k]);
+                    if (il && i > k) v = csub(v, cmul(yk, Us[k][i]));
+                }
+            }
+        }
+        if (bad && i == 0) *J.verr = 1;
     }
 }
 
 template <typename R>
-void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb) {
-    dim3 grid((n + 63) / 64, njobs);
-    k_zbuild<R><<<grid, 256, 0, s>>>(jobs_d, k0, nb);
+void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
+                   int fresh, int verify) {
+    const int per_block = 8 * ZB;
+    const int nzb = (n + per_block - 1) / per_block;
+    const int nvb = (n + per_block - 1) / per_block;
+    k_zbuild<R><<<dim3(nzb + nvb, njobs), 512, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb);
 }
-template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int);
-template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int);
+template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
+template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Trailing update on MFMA.  64x64 output tile per block, 4 waves of 32x32,
@@ -278,6 +451,8 @@ template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, i
 //   re += Vr Zr - Vi Zi,  im += Vr Zi + Vi Zr.
 // Operand layout of v_mfma_*_16x16x4: A[i = lane%16][k = lane/16],
 // B[k = lane/16][j = lane%16]; result D[mfma_t::row(lane, v)][lane%16].
+// The old tile values are loaded before the MFMA loop so their latency
+// overlaps the operand staging.
 // ---------------------------------------------------------------------------
 template <typename R> struct mfma_t;
 // Output row of result element v held by a lane: the f32 form keeps four
@@ -299,28 +474,46 @@ template <> struct mfma_t<float> {
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, int nb) {
-    constexpr int NB = PMAX;
     using C = cplx<R>;
     using M = mfma_t<R>;
     using acc_t = typename M::acc;
+    constexpr int NB = PMAX;
     const LineJob J = jobs[blockIdx.z];
     const int n = J.n;
     const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
     __shared__ R Vr[64][NB + 1], Vi[64][NB + 1];
     __shared__ R Zr[NB][64 + 1], Zi[NB][64 + 1];
+    C* A = (C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
     const C* Zg = (const C*)J.Z;
+    // old tile values (zero for the panel columns) and the pivot-row slots
+    C old[2][4][2];
+    int slot[2][4];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + wr * 32 + bi * 16 + M::row(lane, v);
+            slot[bi][v] = (i < n) ? J.iprow[i] - k0 : -1;
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int j = j0 + wc * 32 + bj * 16 + lr;
+                const bool ld = i < n && j < n && !(j >= k0 && j < k0 + nb);
+                old[bi][v][bj] = ld ? A[(long)i * n + j] : mk<C>(R(0), R(0));
+            }
+        }
     for (int e = t; e < 64 * NB; e += 256) {
-        int ii = e / NB, k = e % NB, i = i0 + ii;
-        C v = (i < n && k < nb) ? Vb[(long)i * PMAX + k] : mk<C>(R(0), R(0));
+        const int ii = e / NB, k = e % NB, i = i0 + ii;
+        const C v = (i < n && k < nb) ? Vb[(long)i * PMAX + k] : mk<C>(R(0), R(0));
         Vr[ii][k] = v.x;
         Vi[ii][k] = v.y;
     }
     for (int e = t; e < 64 * NB; e += 256) {
-        int k = e / 64, jj = e % 64, j = j0 + jj;
-        C v = (j < n && k < nb) ? Zg[(long)k * n + j] : mk<C>(R(0), R(0));
+        const int k = e / 64, jj = e % 64, j = j0 + jj;
+        const C v = (j < n && k < nb) ? Zg[(long)k * n + j] : mk<C>(R(0), R(0));
         Zr[k][jj] = v.x;
         Zi[k][jj] = v.y;
     }
@@ -333,9 +526,8 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
             are[bi][bj] = acc_t{0, 0, 0, 0};
             aim[bi][bj] = acc_t{0, 0, 0, 0};
         }
-    const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int kk = 0; kk < NB; kk += 4) {
+    const int nbr = (nb + 3) & ~3;
+    for (int kk = 0; kk < nbr; kk += 4) {
         R ar[2], ai[2], br[2], bi_[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -354,28 +546,22 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
                 aim[bi][bj] = M::run(ai[bi], br[bj], aim[bi][bj]);
             }
     }
-    C* A = (C*)J.Y;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int ii = wr * 32 + bi * 16 + M::row(lane, v);
-            const int i = i0 + ii;
+            const int i = i0 + wr * 32 + bi * 16 + M::row(lane, v);
             if (i >= n) continue;
-            const int slot = J.iprow[i] - k0;
-            const bool pivrow = (slot >= 0 && slot < nb);
+            const int sl = slot[bi][v];
+            const bool pivrow = (sl >= 0 && sl < nb);
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) {
                 const int jj = wc * 32 + bj * 16 + lr;
                 const int j = j0 + jj;
                 if (j >= n) continue;
                 C out;
-                if (pivrow) {
-                    out = Zg[(long)slot * n + j];
-                } else {
-                    C old = (j >= k0 && j < k0 + nb) ? mk<C>(R(0), R(0)) : A[(long)i * n + j];
-                    out = mk<C>(old.x - are[bi][bj][v], old.y - aim[bi][bj][v]);
-                }
+                if (pivrow) out = Zg[(long)sl * n + j];
+                else out = mk<C>(old[bi][v][bj].x - are[bi][bj][v], old[bi][v][bj].y - aim[bi][bj][v]);
                 A[(long)i * n + j] = out;
             }
         }
@@ -441,22 +627,49 @@ __global__ __launch_bounds__(256) void k_sweep(const SweepJob* jobs, int n) {
     const int r0 = blockIdx.x * rows_per_block;
     const int r1 = min(n, r0 + rows_per_block);
     const C* Y = (const C*)J.Y;
-    for (int r = r0 + wv; r < r1; r += 4) {
-        const C* yr = Y + (long)r * n;
-        R sr = 0, sim = 0;
-        for (int c = lane; c < n; c += 64) {
-            C y = yr[c], x = w[c];
-            sr += y.x * x.x - y.y * x.y;
-            sim += y.x * x.y + y.y * x.x;
+    // two rows per wave at a time, four columns per lane per trip: eight
+    // independent loads in flight per lane
+    for (int r = r0 + 2 * wv; r < r1; r += 8) {
+        const bool two = (r + 1) < r1;
+        const C* ya = Y + (long)r * n;
+        const C* yb = Y + (long)(two ? r + 1 : r) * n;
+        R ar = 0, ai = 0, br = 0, bi = 0;
+        int c = lane;
+        for (; c + 192 < n; c += 256) {
+            C a0 = ya[c], a1 = ya[c + 64], a2 = ya[c + 128], a3 = ya[c + 192];
+            C b0 = yb[c], b1 = yb[c + 64], b2 = yb[c + 128], b3 = yb[c + 192];
+            C x0 = w[c], x1 = w[c + 64], x2 = w[c + 128], x3 = w[c + 192];
+            ar += a0.x * x0.x - a0.y * x0.y + a1.x * x1.x - a1.y * x1.y + a2.x * x2.x -
+                  a2.y * x2.y + a3.x * x3.x - a3.y * x3.y;
+            ai += a0.x * x0.y + a0.y * x0.x + a1.x * x1.y + a1.y * x1.x + a2.x * x2.y +
+                  a2.y * x2.x + a3.x * x3.y + a3.y * x3.x;
+            br += b0.x * x0.x - b0.y * x0.y + b1.x * x1.x - b1.y * x1.y + b2.x * x2.x -
+                  b2.y * x2.y + b3.x * x3.x - b3.y * x3.y;
+            bi += b0.x * x0.y + b0.y * x0.x + b1.x * x1.y + b1.y * x1.x + b2.x * x2.y +
+                  b2.y * x2.x + b3.x * x3.y + b3.y * x3.x;
+        }
+        for (; c < n; c += 64) {
+            C a = ya[c], b = yb[c], x = w[c];
+            ar += a.x * x.x - a.y * x.y;
+            ai += a.x * x.y + a.y * x.x;
+            br += b.x * x.x - b.y * x.y;
+            bi += b.x * x.y + b.y * x.x;
         }
         for (int o = 32; o > 0; o >>= 1) {
-            sr += shfl_x(sr, o);
-            sim += shfl_x(sim, o);
+            ar += shfl_x(ar, o);
+            ai += shfl_x(ai, o);
+            br += shfl_x(br, o);
+            bi += shfl_x(bi, o);
         }
         if (lane == 0) {
-            const long ea = (long)J.rhoinv[r] * J.si;
-            double2 z = make_double2((double)sr, (double)sim);
+            long ea = (long)J.rhoinv[r] * J.si;
+            double2 z = make_double2((double)ar, (double)ai);
             J.out[ea] = (J.mode == 1) ? csub(J.g[ea], z) : z;
+            if (two) {
+                ea = (long)J.rhoinv[r + 1] * J.si;
+                z = make_double2((double)br, (double)bi);
+                J.out[ea] = (J.mode == 1) ? csub(J.g[ea], z) : z;
+            }
         }
     }
 }

@@ -83,19 +83,35 @@ static void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-static int ensure_factor_work(Handle* h, int n) {
-    if (h->work_n >= n) return FDFD_OK;
-    dfree(h->used); dfree(h->q); dfree(h->piv); dfree(h->Vbuf); dfree(h->Vpinv); dfree(h->Zb);
-    h->used = h->q = h->piv = nullptr;
-    h->Vbuf = h->Vpinv = h->Zb = nullptr;
+static void clear_graphs(Handle* h) {
+    for (auto& g : h->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        dfree(g.lj_d);
+        dfree(g.sj_d);
+    }
+    h->graphs.clear();
+}
+
+static int ensure_factor_work(Handle* h, int n, int nch) {
+    if (h->work_n >= n && h->work_ch >= nch) return FDFD_OK;
+    n = std::max(n, h->work_n);
+    nch = std::max(nch, h->work_ch);
+    clear_graphs(h);
+    dfree(h->used); dfree(h->q); dfree(h->piv); dfree(h->Vbuf); dfree(h->LU); dfree(h->Zb);
+    dfree(h->win); dfree(h->verr);
+    h->used = h->q = h->piv = h->win = h->verr = nullptr;
+    h->Vbuf = h->LU = h->Zb = nullptr;
     const int NBmax = PMAX;
-    HIPCHK(h, hipMalloc(&h->used, sizeof(int) * 2 * n));
-    HIPCHK(h, hipMalloc(&h->q, sizeof(int) * 2 * n));
-    HIPCHK(h, hipMalloc(&h->piv, sizeof(int) * 2 * NBmax));
-    HIPCHK(h, hipMalloc(&h->Vbuf, sizeof(double2) * 2 * (size_t)n * NBmax));
-    HIPCHK(h, hipMalloc(&h->Vpinv, sizeof(double2) * 2 * NBmax * NBmax));
-    HIPCHK(h, hipMalloc(&h->Zb, sizeof(double2) * 2 * (size_t)NBmax * n));
+    HIPCHK(h, hipMalloc(&h->used, sizeof(int) * nch * n));
+    HIPCHK(h, hipMalloc(&h->q, sizeof(int) * nch * n));
+    HIPCHK(h, hipMalloc(&h->piv, sizeof(int) * nch * NBmax));
+    HIPCHK(h, hipMalloc(&h->win, sizeof(int) * nch * WIN));
+    HIPCHK(h, hipMalloc(&h->verr, sizeof(int)));
+    HIPCHK(h, hipMalloc(&h->Vbuf, sizeof(double2) * nch * (size_t)n * NBmax));
+    HIPCHK(h, hipMalloc(&h->LU, sizeof(double2) * nch * 2 * NBmax * NBmax));
+    HIPCHK(h, hipMalloc(&h->Zb, sizeof(double2) * nch * (size_t)NBmax * n));
     h->work_n = n;
+    h->work_ch = nch;
     return FDFD_OK;
 }
 
@@ -133,6 +149,8 @@ static void free_factor(Factor* F) {
     dfree(F->coef); dfree(F->sxf); dfree(F->sxb); dfree(F->syf); dfree(F->syb);
     dfree(F->Y); dfree(F->rho); dfree(F->rhoinv); dfree(F->kap); dfree(F->kapinv);
     dfree(F->pivstat); dfree(F->sweep_tab[0]); dfree(F->sweep_tab[1]);
+    for (auto& e : F->sweep_exec)
+        if (e) (void)hipGraphExecDestroy(e);
     delete F;
 }
 
@@ -279,23 +297,51 @@ static int build_sweep_tables(Handle* h, Factor* F) {
 
 // out = M^-1 in (transpose: M^-T in), M = block-tridiagonal part of A' without
 // the periodic corner blocks between the first and last line.
-static int prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out) {
-    const long N = (long)h->g.Nx * h->g.Ny;
-    const int n = F->n, m = F->m;
-    HIPCHK(h, hipMemcpyAsync(h->r_in, in, sizeof(double2) * N, hipMemcpyDeviceToDevice,
-                             h->stream));
+static void launch_sweeps(Handle* h, Factor* F, int trans, bool prof) {
+    const int n = F->n;
     const SweepJob* tab = F->sweep_tab[trans];
     const int nsteps = F->nfwd;
     const double bpl = (double)n * n * F->elem;  // factor bytes per line
     auto run = [&](const SweepJob* t, int nj) {
-        ProfScope ps(h, P_SWEEP, 8.0 * n * n * nj, bpl * nj);
-        if (F->precision == FDFD_FP64) launch_sweep<double>(h->stream, t, nj, n, trans, h->tpart);
-        else launch_sweep<float>(h->stream, t, nj, n, trans, h->tpart);
+        if (prof) {
+            ProfScope ps(h, P_SWEEP, 8.0 * n * n * nj, bpl * nj);
+            if (F->precision == FDFD_FP64) launch_sweep<double>(h->stream, t, nj, n, trans, h->tpart);
+            else launch_sweep<float>(h->stream, t, nj, n, trans, h->tpart);
+        } else {
+            if (F->precision == FDFD_FP64) launch_sweep<double>(h->stream, t, nj, n, trans, h->tpart);
+            else launch_sweep<float>(h->stream, t, nj, n, trans, h->tpart);
+        }
     };
     for (int s = 0; s < nsteps; ++s) run(tab + 2 * s, F->fwd_jobs[s]);
     run(tab + 2 * nsteps, 1);
     for (int s = 0; s < nsteps; ++s) run(tab + 2 * (nsteps + 1 + s), F->bwd_jobs[s]);
-    (void)m;
+}
+
+// out = M^-1 in (transpose: M^-T in), M = block-tridiagonal part of A' without
+// the periodic corner blocks between the first and last line.  The sweep
+// launch sequence of a factor is captured once in a HIP graph and replayed
+// (direct launches while per-kernel profiling is on).
+static int prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out) {
+    const long N = (long)h->g.Nx * h->g.Ny;
+    HIPCHK(h, hipMemcpyAsync(h->r_in, in, sizeof(double2) * N, hipMemcpyDeviceToDevice,
+                             h->stream));
+    if (h->prof.enabled || !h->use_graphs) {
+        launch_sweeps(h, F, trans, h->prof.enabled);
+    } else {
+        if (!F->sweep_exec[trans]) {
+            hipGraph_t gr;
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            launch_sweeps(h, F, trans, false);
+            HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+            hipError_t e = hipGraphInstantiate(&F->sweep_exec[trans], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                F->sweep_exec[trans] = nullptr;
+                return fail(h, FDFD_ERR_HIP, std::string("sweep graph: ") + hipGetErrorString(e));
+            }
+        }
+        HIPCHK(h, hipGraphLaunch(F->sweep_exec[trans], h->stream));
+    }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(out, h->x_out, sizeof(double2) * N, hipMemcpyDeviceToDevice,
                              h->stream));
@@ -309,45 +355,52 @@ static int prec_apply(Handle* h, Factor* F, int trans, const double2* in, double
 //   middle line k:                    S_k = D_k - L_k X_{k-1} U_{k-1} - U_k X_{k+1} L_{k+1}
 // X_j = S_j^-1 by in-place Gauss-Jordan with partial pivoting.
 // ---------------------------------------------------------------------------
-static int build_factor(Handle* h, Factor* F) {
+static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     const GridDesc& g = h->g;
     const long N = (long)g.Nx * g.Ny;
-    if (g.Nx <= g.Ny) { F->orient = 0; F->n = g.Nx; F->m = g.Ny; F->si = 1; F->sj = g.Nx; }
-    else { F->orient = 1; F->n = g.Ny; F->m = g.Nx; F->si = g.Nx; F->sj = 1; }
-    const int n = F->n, m = F->m;
+    const int B = (int)Fs.size();
+    if (B < 1) return fail(h, FDFD_ERR_INVALID, "no factor to build");
+    const int prec = Fs[0]->precision;
+    for (Factor* F : Fs) {
+        if (F->precision != prec)
+            return fail(h, FDFD_ERR_INVALID, "factors built together share one precision");
+        if (g.Nx <= g.Ny) { F->orient = 0; F->n = g.Nx; F->m = g.Ny; F->si = 1; F->sj = g.Nx; }
+        else { F->orient = 1; F->n = g.Ny; F->m = g.Nx; F->si = g.Nx; F->sj = 1; }
+        F->elem = (F->precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
+        F->k = F->m / 2;
+    }
+    const Factor* F0 = Fs[0];
+    const int n = F0->n, m = F0->m, k = F0->k;
+    const size_t elem = F0->elem;
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
-    F->elem = (F->precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
-    F->k = m / 2;
-    const int k = F->k;
-    int rc = assemble(h, F);
-    if (rc) return rc;
-    if ((rc = ensure_factor_work(h, n))) return rc;
+    int rc;
+    if ((rc = ensure_factor_work(h, n, 2 * B))) return rc;
     if ((rc = ensure_solve_work(h, N))) return rc;
-    HIPCHK(h, hipMalloc(&F->Y, (size_t)m * n * n * F->elem));
-    HIPCHK(h, hipMalloc(&F->rho, sizeof(int) * (size_t)m * n));
-    HIPCHK(h, hipMalloc(&F->rhoinv, sizeof(int) * (size_t)m * n));
-    HIPCHK(h, hipMalloc(&F->kap, sizeof(int) * (size_t)m * n));
-    HIPCHK(h, hipMalloc(&F->kapinv, sizeof(int) * (size_t)m * n));
-    HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
-    {
-        std::vector<double> ps(3 * (size_t)m);
-        for (int j = 0; j < m; ++j) {
-            ps[3 * j] = std::numeric_limits<double>::infinity();
-            ps[3 * j + 1] = 0;
-            ps[3 * j + 2] = 0;
-        }
-        HIPCHK(h, hipMemcpy(F->pivstat, ps.data(), sizeof(double) * ps.size(),
-                            hipMemcpyHostToDevice));
+    for (Factor* F : Fs) {
+        if ((rc = assemble(h, F))) return rc;
+        HIPCHK(h, hipMalloc(&F->Y, (size_t)m * n * n * elem));
+        HIPCHK(h, hipMalloc(&F->rho, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->rhoinv, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->kap, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->kapinv, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
     }
-    const int NB = h->panel_width ? h->panel_width : PMAX;
+    std::vector<double> ps0(3 * (size_t)m);
+    for (int j = 0; j < m; ++j) {
+        ps0[3 * j] = std::numeric_limits<double>::infinity();
+        ps0[3 * j + 1] = 0;
+        ps0[3 * j + 2] = 0;
+    }
+    const int NB = h->panel_width ? h->panel_width : PMAX;  // fast-panel width
     const int nsteps = std::max(k, m - 1 - k);
-    Planes P = planes(F, N);
-    const long sj = F->sj;
-    std::vector<LineJob> lj((size_t)2 * (nsteps + 1));
-    std::vector<SchurJob> sjb((size_t)2 * (nsteps + 1));
+    const int JS = 2 * B;  // job slots per step
+    const long sj = F0->sj;
+    std::vector<LineJob> lj((size_t)JS * (nsteps + 1));
+    std::vector<SchurJob> sjb((size_t)JS * (nsteps + 1));
     std::vector<int> njobs(nsteps + 1, 0);
-    auto line_job = [&](int j, int c) {
+    // chain c (2 per factor) owns slices of the handle's work buffers
+    auto line_job = [&](const Factor* F, int j, int c) {
         LineJob J{};
         J.Y = yline(F, j);
         J.used = h->used + (long)c * n;
@@ -356,15 +409,17 @@ static int build_factor(Handle* h, Factor* F) {
         J.q = h->q + (long)c * n;
         J.kap = F->kap + (long)j * n;
         J.kapinv = F->kapinv + (long)j * n;
-        J.Vbuf = (char*)h->Vbuf + (size_t)c * n * PMAX * F->elem;
-        J.Vpinv = (char*)h->Vpinv + (size_t)c * PMAX * PMAX * F->elem;
+        J.Vbuf = (char*)h->Vbuf + (size_t)c * n * PMAX * elem;
+        J.LU = (char*)h->LU + (size_t)c * 2 * PMAX * PMAX * elem;
         J.piv = h->piv + c * PMAX;
-        J.Z = (char*)h->Zb + (size_t)c * PMAX * n * F->elem;
+        J.Z = (char*)h->Zb + (size_t)c * PMAX * n * elem;
+        J.win = h->win + c * WIN;
+        J.verr = h->verr;
         J.pivstat = F->pivstat + 3 * (long)j;
         J.n = n;
         return J;
     };
-    auto schur_base = [&](int j, int c) {
+    auto schur_base = [&](const Factor* F, const Planes& P, int j, int c) {
         SchurJob S{};
         S.S = yline(F, j);
         S.dsub = P.sub + j * sj; S.ddiag = P.diag + j * sj; S.dsup = P.sup + j * sj;
@@ -372,116 +427,184 @@ static int build_factor(Handle* h, Factor* F) {
         S.q = h->q + (long)c * n;
         S.used = h->used + (long)c * n;
         S.iprow = F->rhoinv + (long)j * n;
+        S.win = h->win + c * WIN;
         S.n = n;
         return S;
     };
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = 0; s <= nsteps; ++s) {
         int nj = 0;
-        if (s < k) {
-            int j = s;
-            lj[2 * s + nj] = line_job(j, nj);
-            SchurJob S = schur_base(j, nj);
-            if (j > 0) {
-                S.Y1 = yline(F, j - 1);
-                S.rhoinv1 = F->rhoinv + (long)(j - 1) * n;
-                S.kap1 = F->kap + (long)(j - 1) * n;
-                S.l1 = P.L + j * sj;
-                S.u1 = P.U + (j - 1) * sj;
+        for (int f = 0; f < B; ++f) {
+            const Factor* F = Fs[f];
+            const Planes P = planes(F, N);
+            if (s == nsteps) {  // middle line k
+                const int c = 2 * f;
+                lj[(size_t)JS * s + nj] = line_job(F, k, c);
+                SchurJob S = schur_base(F, P, k, c);
+                if (k >= 1) {
+                    S.Y1 = yline(F, k - 1);
+                    S.rhoinv1 = F->rhoinv + (long)(k - 1) * n;
+                    S.kap1 = F->kap + (long)(k - 1) * n;
+                    S.l1 = P.L + k * sj;
+                    S.u1 = P.U + (k - 1) * sj;
+                }
+                if (k + 1 <= m - 1) {
+                    S.Y2 = yline(F, k + 1);
+                    S.rho2 = F->rho + (long)(k + 1) * n;
+                    S.kap2 = F->kap + (long)(k + 1) * n;
+                    S.l2 = P.U + k * sj;
+                    S.u2 = P.L + (k + 1) * sj;
+                }
+                sjb[(size_t)JS * s + nj] = S;
+                ++nj;
+                continue;
             }
-            sjb[2 * s + nj] = S;
-            ++nj;
-        }
-        if (s < m - 1 - k) {
-            int j = m - 1 - s;
-            lj[2 * s + nj] = line_job(j, nj);
-            SchurJob S = schur_base(j, nj);
-            if (j < m - 1) {
-                S.Y1 = yline(F, j + 1);
-                S.rhoinv1 = F->rhoinv + (long)(j + 1) * n;
-                S.kap1 = F->kap + (long)(j + 1) * n;
-                S.l1 = P.U + j * sj;
-                S.u1 = P.L + (j + 1) * sj;
+            if (s < k) {  // chain A, line j = s:  S_j = D_j - L_j X_{j-1} U_{j-1}
+                const int j = s, c = 2 * f;
+                lj[(size_t)JS * s + nj] = line_job(F, j, c);
+                SchurJob S = schur_base(F, P, j, c);
+                if (j > 0) {
+                    S.Y1 = yline(F, j - 1);
+                    S.rhoinv1 = F->rhoinv + (long)(j - 1) * n;
+                    S.kap1 = F->kap + (long)(j - 1) * n;
+                    S.l1 = P.L + j * sj;
+                    S.u1 = P.U + (j - 1) * sj;
+                }
+                sjb[(size_t)JS * s + nj] = S;
+                ++nj;
             }
-            sjb[2 * s + nj] = S;
-            ++nj;
+            if (s < m - 1 - k) {  // chain B, line j = m-1-s:  S_j = D_j - U_j X_{j+1} L_{j+1}
+                const int j = m - 1 - s, c = 2 * f + 1;
+                lj[(size_t)JS * s + nj] = line_job(F, j, c);
+                SchurJob S = schur_base(F, P, j, c);
+                if (j < m - 1) {
+                    S.Y1 = yline(F, j + 1);
+                    S.rhoinv1 = F->rhoinv + (long)(j + 1) * n;
+                    S.kap1 = F->kap + (long)(j + 1) * n;
+                    S.l1 = P.U + j * sj;
+                    S.u1 = P.L + (j + 1) * sj;
+                }
+                sjb[(size_t)JS * s + nj] = S;
+                ++nj;
+            }
         }
         njobs[s] = nj;
     }
-    {
-        lj[2 * nsteps] = line_job(k, 0);
-        SchurJob S = schur_base(k, 0);
-        if (k >= 1) {
-            S.Y1 = yline(F, k - 1);
-            S.rhoinv1 = F->rhoinv + (long)(k - 1) * n;
-            S.kap1 = F->kap + (long)(k - 1) * n;
-            S.l1 = P.L + k * sj;
-            S.u1 = P.U + (k - 1) * sj;
+    const bool fp64 = (prec == FDFD_FP64);
+    // one pass of the factorization as a launch sequence
+    auto launch_pass = [&](const LineJob* lj_d, const SchurJob* sj_d, bool full, bool prof) {
+        auto scope = [&](int id, double fl, double by) {
+            return prof ? new ProfScope(h, id, fl, by) : nullptr;
+        };
+        for (int s = 0; s <= nsteps; ++s) {
+            const int nj = njobs[s];
+            const LineJob* J = lj_d + (size_t)JS * s;
+            {
+                ProfScope* ps = scope(P_SCHUR, 0, 2.0 * n * n * elem * nj);
+                if (fp64) launch_schur<double>(h->stream, sj_d + (size_t)JS * s, nj, n);
+                else launch_schur<float>(h->stream, sj_d + (size_t)JS * s, nj, n);
+                delete ps;
+            }
+            for (int k0 = 0; k0 < n;) {
+                int nb;
+                const int fresh = std::min(n, WIN + k0);
+                {
+                    ProfScope* ps = scope(P_PANEL, 0, 0);
+                    if (!full) {
+                        nb = std::min(NB, n - k0);
+                        if (fp64) launch_wpanel<double>(h->stream, J, nj, n, k0, nb, fresh);
+                        else launch_wpanel<float>(h->stream, J, nj, n, k0, nb, fresh);
+                    } else {
+                        nb = 16;  // full-search panel width: powers of two
+                        while (nb > n - k0) nb >>= 1;
+                        if (fp64) launch_panel<double>(h->stream, J, nj, n, k0, nb);
+                        else launch_panel<float>(h->stream, J, nj, n, k0, nb);
+                    }
+                    delete ps;
+                }
+                {
+                    ProfScope* ps = scope(P_ZBUILD, 0, 0);
+                    if (fp64) launch_zbuild<double>(h->stream, J, nj, n, k0, nb, fresh, !full);
+                    else launch_zbuild<float>(h->stream, J, nj, n, k0, nb, fresh, !full);
+                    delete ps;
+                }
+                {
+                    ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * nb * nj,
+                                          2.0 * n * (double)n * elem * nj);
+                    if (fp64) launch_trailing<double>(h->stream, J, nj, n, k0, nb);
+                    else launch_trailing<float>(h->stream, J, nj, n, k0, nb);
+                    delete ps;
+                }
+                k0 += nb;
+            }
+            launch_maps(h->stream, J, nj, n);
         }
-        if (k + 1 <= m - 1) {
-            S.Y2 = yline(F, k + 1);
-            S.rho2 = F->rho + (long)(k + 1) * n;
-            S.kap2 = F->kap + (long)(k + 1) * n;
-            S.l2 = P.U + k * sj;
-            S.u2 = P.L + (k + 1) * sj;
-        }
-        sjb[2 * nsteps] = S;
-        njobs[nsteps] = 1;
-    }
-    LineJob* lj_d = nullptr;
-    SchurJob* sj_d = nullptr;
-    HIPCHK(h, hipMalloc(&lj_d, sizeof(LineJob) * lj.size()));
-    HIPCHK(h, hipMalloc(&sj_d, sizeof(SchurJob) * sjb.size()));
-    HIPCHK(h, hipMemcpy(lj_d, lj.data(), sizeof(LineJob) * lj.size(), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(sj_d, sjb.data(), sizeof(SchurJob) * sjb.size(), hipMemcpyHostToDevice));
+    };
 
     hipEvent_t e0, e1;
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
-    HIPCHK(h, hipEventRecord(e0, h->stream));
-    const bool fp64 = (F->precision == FDFD_FP64);
-    for (int s = 0; s <= nsteps; ++s) {
-        const int nj = njobs[s];
-        const LineJob* J = lj_d + 2 * s;
-        {
-            ProfScope ps(h, P_SCHUR, 0, 2.0 * n * n * F->elem * nj);
-            if (fp64) launch_schur<double>(h->stream, sj_d + 2 * s, nj, n);
-            else launch_schur<float>(h->stream, sj_d + 2 * s, nj, n);
+    // Pass 0: windowed pivot search with verification.  Pass 1 (only if a
+    // row outside a window would have won a GEPP step): full-column search.
+    int used_full = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool full = (pass == 1) || h->full_pivot_search;
+        used_full = full ? 1 : 0;
+        // job tables live at fixed addresses per shape so that a captured
+        // launch sequence can be replayed for every factorization of it
+        FactorGraph* G = nullptr;
+        for (auto& gr : h->graphs)
+            if (gr.n == n && gr.m == m && gr.prec == prec && gr.nb == NB && gr.full == full &&
+                gr.nfac == B)
+                G = &gr;
+        if (!G) {
+            h->graphs.push_back(FactorGraph{});
+            G = &h->graphs.back();
+            G->n = n; G->m = m; G->prec = prec; G->nb = NB; G->full = full; G->nfac = B;
+            HIPCHK(h, hipMalloc(&G->lj_d, sizeof(LineJob) * lj.size()));
+            HIPCHK(h, hipMalloc(&G->sj_d, sizeof(SchurJob) * sjb.size()));
         }
-        for (int k0 = 0; k0 < n;) {
-            // panel width: NB, the remainder split into powers of two
-            int nb = NB;
-            while (nb > n - k0) nb >>= 1;
-            {
-                ProfScope ps(h, P_PANEL, 0, 0);
-                if (fp64) launch_panel<double>(h->stream, J, nj, n, k0, nb);
-                else launch_panel<float>(h->stream, J, nj, n, k0, nb);
+        HIPCHK(h, hipMemcpyAsync(G->lj_d, lj.data(), sizeof(LineJob) * lj.size(),
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(G->sj_d, sjb.data(), sizeof(SchurJob) * sjb.size(),
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemsetAsync(h->verr, 0, sizeof(int), h->stream));
+        for (Factor* F : Fs)
+            HIPCHK(h, hipMemcpyAsync(F->pivstat, ps0.data(), sizeof(double) * ps0.size(),
+                                     hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));  // host tables are stack-local
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        if (h->prof.enabled || !h->use_graphs) {
+            launch_pass(G->lj_d, G->sj_d, full, h->prof.enabled);
+        } else {
+            if (!G->exec) {
+                hipGraph_t gr;
+                HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                launch_pass(G->lj_d, G->sj_d, full, false);
+                HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+                hipError_t e = hipGraphInstantiate(&G->exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e != hipSuccess) {
+                    G->exec = nullptr;
+                    return fail(h, FDFD_ERR_HIP, std::string("factor graph: ") + hipGetErrorString(e));
+                }
             }
-            {
-                ProfScope ps(h, P_ZBUILD, 0, 0);
-                if (fp64) launch_zbuild<double>(h->stream, J, nj, n, k0, nb);
-                else launch_zbuild<float>(h->stream, J, nj, n, k0, nb);
-            }
-            {
-                ProfScope ps(h, P_TRAIL, 8.0 * n * (double)n * nb * nj,
-                             2.0 * n * (double)n * F->elem * nj);
-                if (fp64) launch_trailing<double>(h->stream, J, nj, n, k0, nb);
-                else launch_trailing<float>(h->stream, J, nj, n, k0, nb);
-            }
-            k0 += nb;
+            HIPCHK(h, hipGraphLaunch(G->exec, h->stream));
         }
-        launch_maps(h->stream, J, nj, n);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        int verr = 0;
+        HIPCHK(h, hipMemcpy(&verr, h->verr, sizeof(int), hipMemcpyDeviceToHost));
+        if (full || !verr) break;
     }
-    HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipEventRecord(e1, h->stream));
-    HIPCHK(h, hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-    F->t_factor_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(lj_d);
-    (void)hipFree(sj_d);
-    {
+    int first_singular = -1;
+    for (Factor* F : Fs) {
+        F->t_factor_ms = ms;
+        F->pivot_fallback = used_full;
         std::vector<double> ps(3 * (size_t)m);
         HIPCHK(h, hipMemcpy(ps.data(), F->pivstat, sizeof(double) * ps.size(),
                             hipMemcpyDeviceToHost));
@@ -493,12 +616,16 @@ static int build_factor(Handle* h, Factor* F) {
             if (ps[3 * j + 2] != 0 && F->singular_line < 0) F->singular_line = j;
         }
         F->pivot_growth = (mn > 0) ? mx / mn : std::numeric_limits<double>::infinity();
+        if (F->singular_line >= 0 && first_singular < 0) first_singular = F->singular_line;
     }
-    if (F->singular_line >= 0)
-        return fail(h, FDFD_ERR_SINGULAR,
-                    "zero pivot in line block " + std::to_string(F->singular_line));
-    return build_sweep_tables(h, F);
+    if (first_singular >= 0)
+        return fail(h, FDFD_ERR_SINGULAR, "zero pivot in line block " + std::to_string(first_singular));
+    for (Factor* F : Fs)
+        if ((rc = build_sweep_tables(h, F))) return rc;
+    return FDFD_OK;
 }
+
+static int build_factor(Handle* h, Factor* F) { return build_factors(h, std::vector<Factor*>{F}); }
 
 // ---------------------------------------------------------------------------
 // FP64 Krylov building blocks
@@ -632,9 +759,10 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
         if ((rc = dev_update(h, x, Zk, N, jdone, y.data(), 1.0))) return rc;
     }
     st.resid = resid;
-    return fail(h, FDFD_ERR_NOCONV,
-                "refinement did not reach the tolerance (relative residual " +
-                    std::to_string(resid) + ")");
+    char buf[160];
+    snprintf(buf, sizeof(buf),
+             "refinement did not reach the tolerance %.1e (relative residual %.3e)", tol, resid);
+    return fail(h, FDFD_ERR_NOCONV, buf);
 }
 
 }  // namespace fdfd
@@ -673,6 +801,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     if (hipSetDevice(device_ordinal) != hipSuccess) return FDFD_ERR_HIP;
     auto* hh = new fdfd_handle();
     hh->impl.device = device_ordinal;
+    const char* eg = std::getenv("FDFD_GRAPHS");
+    hh->impl.use_graphs = !(eg && eg[0] == '0');
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;
@@ -687,7 +817,9 @@ int fdfd_destroy(fdfd_handle* hh) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     for (auto& kv : h->factors) free_factor(kv.second);
-    void* bufs[] = {h->eps, h->used, h->q, h->piv, h->Vbuf, h->Vpinv, h->Zb, h->r_in, h->g_mid,
+    clear_graphs(h);
+    void* bufs[] = {h->eps, h->used, h->q, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
+                    h->r_in, h->g_mid,
                     h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec, h->dpart,
                     h->rpart, h->hdev, h->tpart};
     for (void* b : bufs) dfree(b);
@@ -764,6 +896,7 @@ int fdfd_factor(fdfd_handle* hh, int pol, double omega, int precision, int* fact
         stats->t_factor_ms = F->t_factor_ms;
         stats->pivot_growth = F->pivot_growth;
         stats->singular_line = F->singular_line;
+        stats->pivot_fallback = F->pivot_fallback;
     }
     if (rc) {
         free_factor(F);
@@ -994,13 +1127,13 @@ int fdfd_profile_read(fdfd_handle* hh, int kernel, double* total_ms, long long* 
     return FDFD_OK;
 }
 
-int fdfd_set_tuning(fdfd_handle* hh, int panel_width, int gemm_tile) {
+int fdfd_set_tuning(fdfd_handle* hh, int panel_width, int force_full_search) {
     Handle* h = H(hh);
     if (!h) return FDFD_ERR_INVALID;
-    if (panel_width != 0 && panel_width != 1 && panel_width != 2 && panel_width != 4 &&
-        panel_width != 8 && panel_width != 16)
-        return fail(h, FDFD_ERR_INVALID, "panel width must be 0 or a power of two <= 16");
-    (void)gemm_tile;
+    if (panel_width < 0 || panel_width > PMAX)
+        return fail(h, FDFD_ERR_INVALID, "panel width must be in [0, 32] (0 = default)");
+    // force the full-column pivot search on every panel
+    h->full_pivot_search = force_full_search != 0;
     h->panel_width = panel_width;
     return FDFD_OK;
 }
@@ -1018,6 +1151,7 @@ int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, doub
     return rc;
 }
 int rt_build_factor(Handle* h, Factor* F) { return build_factor(h, F); }
+int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) { return build_factors(h, Fs); }
 void rt_free_factor(Factor* F) { free_factor(F); }
 int rt_norm(Handle* h, const double2* x, long N, double* out) { return dev_norm(h, x, N, out); }
 int rt_multidot(Handle* h, const double2* V, long N, int k, const double2* w, cd* out) {

@@ -44,11 +44,22 @@ struct Factor {
     int k = 0;                  // middle line of the twisted factorization
     // device sweep tables: [transpose][phase] with phase 0 forward, 1 middle, 2 backward
     SweepJob* sweep_tab[2] = {nullptr, nullptr};
+    hipGraphExec_t sweep_exec[2] = {nullptr, nullptr};
     int nfwd = 0, nbwd = 0;     // number of forward / backward steps
     std::vector<int> fwd_jobs, bwd_jobs;  // jobs per step
     double t_factor_ms = 0;
     double pivot_growth = 0;
     int singular_line = -1;
+    int pivot_fallback = 0;   // 1 if the full-column pivot search was used
+};
+
+// captured launch sequence of one factorization pass for a shape
+struct FactorGraph {
+    int n = 0, m = 0, prec = 0, nb = 0, nfac = 1;
+    bool full = false;
+    hipGraphExec_t exec = nullptr;
+    LineJob* lj_d = nullptr;
+    SchurJob* sj_d = nullptr;
 };
 
 struct Handle {
@@ -62,9 +73,12 @@ struct Handle {
     std::string err;
     int panel_width = 0;  // 0 = default per precision
     // factorization work (two chains)
-    int work_n = 0;
-    int *used = nullptr, *q = nullptr, *piv = nullptr;
-    void *Vbuf = nullptr, *Vpinv = nullptr, *Zb = nullptr;
+    int work_n = 0, work_ch = 0;
+    int *used = nullptr, *q = nullptr, *piv = nullptr, *win = nullptr, *verr = nullptr;
+    void *Vbuf = nullptr, *LU = nullptr, *Zb = nullptr;
+    bool full_pivot_search = false;
+    bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
+    std::vector<FactorGraph> graphs;
     // sweep / Krylov work
     long work_N = 0;
     double2 *r_in = nullptr, *g_mid = nullptr, *x_out = nullptr;  // sweep vectors

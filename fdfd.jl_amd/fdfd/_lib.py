@@ -30,6 +30,7 @@ class fdfd_stats(C.Structure):
         ("iterations", C.c_int), ("restarts", C.c_int), ("residual", C.c_double),
         ("t_factor_ms", C.c_double), ("t_solve_ms", C.c_double),
         ("pivot_growth", C.c_double), ("singular_line", C.c_int), ("nconv", C.c_int),
+        ("pivot_fallback", C.c_int),
     ]
 
     def as_dict(self):

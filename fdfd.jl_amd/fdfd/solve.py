@@ -140,6 +140,47 @@ class Engine:
             fl = fl.reshape(nev, 3, g.N[1], g.N[0]).transpose(0, 3, 2, 1)
         return om, fl, st.as_dict()
 
+    def batch_solve(self, jobs, precision="mixed", tol: float = 1e-10, keep: bool = True):
+        """Batched driven solves.  jobs: list of (grid, pol, eps_r, omegas,
+        srcs[Nx, Ny, nsrc]).  Returns (results, stats) where results[j] is an
+        array [nomega, nsrc, Nx, Ny, 3] (or None when keep=False).
+        Resets the engine's grid (and releases its factors)."""
+        b = C.c_void_p()
+        self._chk(self.lib.fdfd_batch_create(self.h, C.byref(b)))
+        try:
+            ids = []
+            for (g, pol, eps, omegas, srcs) in jobs:
+                e = _flat(eps)
+                om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64))
+                srcs = np.asarray(srcs, dtype=np.complex128)
+                if srcs.ndim == 2:
+                    srcs = srcs[:, :, None]
+                ss = np.ascontiguousarray(np.stack([_flat(srcs[:, :, r])
+                                                    for r in range(srcs.shape[2])]))
+                jid = C.c_int()
+                self._chk(self.lib.fdfd_batch_add(
+                    b, g.N[0], g.N[1], g.dx, g.dy, g.Npml[0], g.Npml[1], g.L0, int(pol),
+                    _ptr(e), om.size, _ptr(om), ss.shape[0], _ptr(ss), C.byref(jid)))
+                ids.append((jid.value, g, om.size, ss.shape[0]))
+            st = fdfd_stats()
+            self._chk(self.lib.fdfd_batch_run(b, PRECISIONS[precision], tol, C.byref(st)))
+            out = []
+            for jid, g, nom, ns in ids:
+                if not keep:
+                    out.append(None)
+                    continue
+                res = np.empty((nom, ns, g.N[0], g.N[1], 3), dtype=np.complex128)
+                buf = np.empty(3 * len(g), dtype=np.complex128)
+                for o in range(nom):
+                    for r in range(ns):
+                        self._chk(self.lib.fdfd_batch_result(b, jid, o, r, _ptr(buf)))
+                        res[o, r] = buf.reshape(3, g.N[1], g.N[0]).transpose(2, 1, 0)
+                out.append(res)
+            self.grid = None
+            return out, st.as_dict()
+        finally:
+            self.lib.fdfd_batch_destroy(b)
+
     def profile(self, enable: bool = True):
         self._chk(self.lib.fdfd_profile_enable(self.h, int(enable)))
 
@@ -149,8 +190,8 @@ class Engine:
                                              C.byref(fl), C.byref(by)))
         return {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value}
 
-    def set_tuning(self, panel_width: int = 0):
-        self._chk(self.lib.fdfd_set_tuning(self.h, panel_width, 0))
+    def set_tuning(self, panel_width: int = 0, force_full_search: bool = False):
+        self._chk(self.lib.fdfd_set_tuning(self.h, panel_width, int(force_full_search)))
 
 
 _default_engine = None

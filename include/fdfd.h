@@ -73,6 +73,9 @@ typedef struct {
     double pivot_growth;   /* max |pivot| / min |pivot| over all line blocks */
     int singular_line;     /* line index of a zero pivot, -1 if none        */
     int nconv;             /* converged eigenpairs (eigenfrequency)         */
+    int pivot_fallback;    /* 1: a pivot outside the candidate window won a
+                              step and the factorization was redone with the
+                              full-column pivot search                       */
 } fdfd_stats;
 
 /* ---- library / handle --------------------------------------------------- */
@@ -176,8 +179,9 @@ int fdfd_batch_destroy(fdfd_batch* b);
 int fdfd_profile_enable(fdfd_handle* h, int enable);
 int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
                       long long* launches, double* flops, double* bytes);
-/* Panel width override (0 = autotuned / default). */
-int fdfd_set_tuning(fdfd_handle* h, int panel_width, int gemm_tile);
+/* Panel width override (0 = default 32).  force_full_search != 0 makes every
+ * panel search the whole column for its pivots (no candidate window). */
+int fdfd_set_tuning(fdfd_handle* h, int panel_width, int force_full_search);
 
 #ifdef __cplusplus
 }

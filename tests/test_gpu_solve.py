@@ -106,3 +106,32 @@ def test_config_c1_fields(engine, precision, pol):
               f"its={st['iterations']} growth={fst['pivot_growth']:.2e}", flush=True)
         assert err <= FIELD_TOL
     assert st["residual"] <= 1e-13
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_window_pivot_search_matches_full_search(engine, precision):
+    """The windowed pivot search (verified against every unpivoted row) and
+    the full-column search give the same solution; on this workload the
+    verification passes, so no fallback refactorization happens."""
+    d = O.random_pixel_device(160, 144, 12, 1.55, 5)
+    _engine_grid(engine, d)
+    om = float(d.omega[0])
+    A = O.assemble(d.grid, d.eps_r, om, O.TM)
+    b = O.rhs(om, d.src)
+    xs = {}
+    for full in (False, True):
+        engine.set_tuning(0, force_full_search=full)
+        try:
+            fid, fst = engine.factor(O.TM, om, precision)
+            x, st = engine.linsolve(fid, b, tol=1e-13)
+            engine.release(fid)
+        finally:
+            engine.set_tuning(0, force_full_search=False)
+        assert fst["pivot_fallback"] == (1 if full else 0)
+        xs[full] = x
+        print(f"{precision} full={full}: growth={fst['pivot_growth']:.3e} its={st['iterations']}",
+              flush=True)
+    err = np.linalg.norm(xs[True] - xs[False]) / np.linalg.norm(xs[True])
+    assert err <= FIELD_TOL
+    res = np.linalg.norm(b - A @ xs[False]) / np.linalg.norm(b)
+    assert res <= 1e-12
