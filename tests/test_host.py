@@ -1,0 +1,76 @@
+"""Host-side mirror of the reference problem description (CPU only):
+Grid / index maps, Device painting, FieldTM indexing, flux post-processing."""
+import numpy as np
+import pytest
+
+import fdfd_oracle as O
+from fdfd import core, flux
+
+
+@pytest.mark.parametrize("dh,npml,xr,yr", [
+    (0.02, (15, 15), (0.0, 4.0), (-1.2, 1.2)),
+    (0.02, (20, 20), (-10.24, 10.24), (-10.24, 10.24)),
+    (1.0, (0, 0), (0.0, 2.5), (0.0, 3.5)),
+    (0.05, (3, 5), (-1.0, 1.35), (0.2, 1.0)),
+])
+def test_grid_matches_oracle(dh, npml, xr, yr):
+    g = core.Grid(dh, npml, xr, yr)
+    o = O.Grid.create(dh, npml, xr, yr)
+    assert g.N == o.N and g.Npml == o.Npml
+    assert g.dx == o.dx and g.dy == o.dy
+    assert np.array_equal(g.xc(), o.xc()) and np.array_equal(g.yc(), o.yc())
+    for x in np.linspace(xr[0] - 0.3, xr[1] + 0.3, 37):
+        assert g.x2ind(x) == o.x2ind(x)
+    for y in np.linspace(yr[0] - 0.3, yr[1] + 0.3, 29):
+        assert g.y2ind(y) == o.y2ind(y)
+
+
+def test_setup_src_point_and_line():
+    g = core.Grid(0.02, (15, 15), (0.0, 4.0), (-1.2, 1.2))
+    d = core.Device(g, 1e15)
+    core.setup_src(d, core.Point(1.0, 0.0))
+    assert d.src[50, 60] == 1j and np.count_nonzero(d.src) == 1  # device.jl:97 writes 1im
+    d2 = core.Device(g, 1e15)
+    core.setup_src(d2, core.Point(1.0, 0.0), normal=core.XHAT)
+    assert np.all(d2.src[50, :] == 1j) and np.count_nonzero(d2.src) == g.N[1]
+
+
+def test_setup_eps_region_and_callable():
+    g = core.Grid(0.1, (0, 0), (0.0, 1.0), (0.0, 1.0))
+    d = core.Device(g, 1e15)
+    core.setup_eps(d, lambda x, y: x < 0.5, 4.0)
+    assert np.all(d.eps_r[:5, :] == 4.0) and np.all(d.eps_r[5:, :] == 1.0)
+    core.setup_eps(d, lambda x, y: y > 0.8, lambda x, y: 2.0 + x)
+    assert d.eps_r[3, 9] == pytest.approx(2.35)
+
+
+def test_field_indexing_layout():
+    g = core.Grid(1.0, (0, 0), (0, 3.0), (0, 2.0))
+    data = np.arange(3 * 2 * 3).reshape(3, 2, 3).astype(complex)
+    f = core.FieldTM(g, 2.0, data)
+    assert np.array_equal(f["Hx"], data[:, :, 1])
+    assert f.components == ("Ez", "Hx", "Hy")
+
+
+def test_flux_matches_oracle_restatement():
+    rng = np.random.default_rng(2)
+    g = core.Grid(0.02, (0, 0), (0, 1.0), (0, 0.8))
+    og = O.Grid.create(0.02, (0, 0), (0, 1.0), (0, 0.8))
+    data = rng.standard_normal((50, 40, 3)) + 1j * rng.standard_normal((50, 40, 3))
+    f = core.FieldTM(g, 1.0, data)
+    fo = O.FieldTM(og, 1.0, data)
+    for xc, yc, w in [(0.31, 0.4, 0.2), (0.55, 0.21, 0.1)]:
+        assert flux.flux_surface_integral(f, core.Point(xc, yc), w, core.XHAT) == pytest.approx(
+            O.flux_surface_integral_tm_x(fo, xc, yc, w), rel=1e-13)
+    with pytest.raises(NotImplementedError):
+        flux.flux_surface_integral(f, core.Point(0.31, 0.4), 0.2, core.YHAT)
+
+
+def test_poynting_tm_constant_fields():
+    g = core.Grid(1.0, (0, 0), (0, 4.0), (0, 4.0))
+    data = np.zeros((4, 4, 3), dtype=complex)
+    data[:, :, 0] = 2.0
+    data[:, :, 1] = 1j
+    data[:, :, 2] = 3.0
+    sx, sy = flux.poynting(core.FieldTM(g, 1.0, data))
+    assert np.allclose(sx, -0.5 * 2.0 * 3.0) and np.allclose(sy, 0.0)
