@@ -59,17 +59,18 @@ static void prof_resolve(Prof& p) {
     p.pending.clear();
 }
 struct ProfScope {
-    Handle* h; int id; double flops, bytes; hipEvent_t a{}, b{};
-    ProfScope(Handle* h_, int id_, double f, double by) : h(h_), id(id_), flops(f), bytes(by) {
+    Handle* h; int id; double flops, bytes; hipStream_t st; hipEvent_t a{}, b{};
+    ProfScope(Handle* h_, int id_, double f, double by, hipStream_t s = nullptr)
+        : h(h_), id(id_), flops(f), bytes(by), st(s ? s : h_->stream) {
         if (h->prof.enabled) {
             a = ev_get(h->prof);
-            (void)hipEventRecord(a, h->stream);
+            (void)hipEventRecord(a, st);
         }
     }
     ~ProfScope() {
         if (h->prof.enabled) {
             b = ev_get(h->prof);
-            (void)hipEventRecord(b, h->stream);
+            (void)hipEventRecord(b, st);
             h->prof.pending.push_back({a, b, id, flops, bytes});
             if (h->prof.pending.size() > 4096) prof_resolve(h->prof);
         }
@@ -219,6 +220,8 @@ static int build_sweep_tables(Handle* h, Factor* F) {
     F->nbwd = nsteps;
     F->fwd_jobs.assign(nsteps, 0);
     F->bwd_jobs.assign(nsteps, 0);
+    F->fwd_has_a.assign(nsteps, 0);
+    F->bwd_has_a.assign(nsteps, 0);
     for (int tr = 0; tr < 2; ++tr) {
         std::vector<SweepJob> tab((size_t)2 * (2 * nsteps + 1));
         for (int s = 0; s < nsteps; ++s) {
@@ -234,6 +237,7 @@ static int build_sweep_tables(Handle* h, Factor* F) {
                 }
                 J.out = h->g_mid + j * sj;
                 tab[2 * s + nj++] = J;
+                F->fwd_has_a[s] = 1;
             }
             if (s < m - 1 - k) {  // forward B, line j = m-1-s
                 int j = m - 1 - s;
@@ -275,6 +279,7 @@ static int build_sweep_tables(Handle* h, Factor* F) {
                 J.g = h->g_mid + ja * sj;
                 J.out = h->x_out + ja * sj;
                 tab[2 * (nsteps + 1 + s) + nj++] = J;
+                F->bwd_has_a[s] = 1;
             }
             int jb = k + 1 + s;
             if (jb <= m - 1) {  // backward B
@@ -302,19 +307,39 @@ static void launch_sweeps(Handle* h, Factor* F, int trans, bool prof) {
     const SweepJob* tab = F->sweep_tab[trans];
     const int nsteps = F->nfwd;
     const double bpl = (double)n * n * F->elem;  // factor bytes per line
-    auto run = [&](const SweepJob* t, int nj) {
-        if (prof) {
-            ProfScope ps(h, P_SWEEP, 8.0 * n * n * nj, bpl * nj);
-            if (F->precision == FDFD_FP64) launch_sweep<double>(h->stream, t, nj, n, trans, h->tpart);
-            else launch_sweep<float>(h->stream, t, nj, n, trans, h->tpart);
-        } else {
-            if (F->precision == FDFD_FP64) launch_sweep<double>(h->stream, t, nj, n, trans, h->tpart);
-            else launch_sweep<float>(h->stream, t, nj, n, trans, h->tpart);
-        }
+    auto run = [&](hipStream_t st, const SweepJob* t, double2* part) {
+        ProfScope* ps = prof ? new ProfScope(h, P_SWEEP, 8.0 * n * n, bpl, st) : nullptr;
+        if (F->precision == FDFD_FP64) launch_sweep<double>(st, t, 1, n, trans, part);
+        else launch_sweep<float>(st, t, 1, n, trans, part);
+        delete ps;
     };
-    for (int s = 0; s < nsteps; ++s) run(tab + 2 * s, F->fwd_jobs[s]);
-    run(tab + 2 * nsteps, 1);
-    for (int s = 0; s < nsteps; ++s) run(tab + 2 * (nsteps + 1 + s), F->bwd_jobs[s]);
+    // transposed sweeps keep their partial sums in one buffer slot per stream
+    double2* partB = h->tpart + (size_t)16 * MAX_LINE;
+    auto fork = [&]() {
+        (void)hipEventRecord(h->ev_fork, h->stream);
+        (void)hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+    };
+    auto join = [&]() {
+        (void)hipEventRecord(h->ev_join, h->stream2);
+        (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
+    };
+    // chain A (listed first in a step) on the handle's stream, chain B on the
+    // second stream; the middle line needs both forward sweeps
+    fork();
+    for (int s = 0; s < nsteps; ++s) {
+        const SweepJob* t = tab + 2 * s;
+        if (F->fwd_has_a[s]) run(h->stream, t, h->tpart);
+        if (F->fwd_jobs[s] > F->fwd_has_a[s]) run(h->stream2, t + F->fwd_has_a[s], partB);
+    }
+    join();
+    run(h->stream, tab + 2 * nsteps, h->tpart);
+    fork();
+    for (int s = 0; s < nsteps; ++s) {
+        const SweepJob* t = tab + 2 * (nsteps + 1 + s);
+        if (F->bwd_has_a[s]) run(h->stream, t, h->tpart);
+        if (F->bwd_jobs[s] > F->bwd_has_a[s]) run(h->stream2, t + F->bwd_has_a[s], partB);
+    }
+    join();
 }
 
 // out = M^-1 in (transpose: M^-T in), M = block-tridiagonal part of A' without
@@ -431,77 +456,85 @@ static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         S.n = n;
         return S;
     };
+    // per step: the chain-A jobs of all factors, then the chain-B jobs
+    std::vector<int> nA(nsteps + 1, 0), nB(nsteps + 1, 0);
     for (int s = 0; s <= nsteps; ++s) {
         int nj = 0;
-        for (int f = 0; f < B; ++f) {
-            const Factor* F = Fs[f];
-            const Planes P = planes(F, N);
-            if (s == nsteps) {  // middle line k
-                const int c = 2 * f;
-                lj[(size_t)JS * s + nj] = line_job(F, k, c);
-                SchurJob S = schur_base(F, P, k, c);
-                if (k >= 1) {
-                    S.Y1 = yline(F, k - 1);
-                    S.rhoinv1 = F->rhoinv + (long)(k - 1) * n;
-                    S.kap1 = F->kap + (long)(k - 1) * n;
-                    S.l1 = P.L + k * sj;
-                    S.u1 = P.U + (k - 1) * sj;
+        for (int chain = 0; chain < 2; ++chain) {
+            for (int f = 0; f < B; ++f) {
+                const Factor* F = Fs[f];
+                const Planes P = planes(F, N);
+                if (s == nsteps) {  // middle line k (listed once)
+                    if (chain == 1) continue;
+                    const int c = 2 * f;
+                    lj[(size_t)JS * s + nj] = line_job(F, k, c);
+                    SchurJob S = schur_base(F, P, k, c);
+                    if (k >= 1) {
+                        S.Y1 = yline(F, k - 1);
+                        S.rhoinv1 = F->rhoinv + (long)(k - 1) * n;
+                        S.kap1 = F->kap + (long)(k - 1) * n;
+                        S.l1 = P.L + k * sj;
+                        S.u1 = P.U + (k - 1) * sj;
+                    }
+                    if (k + 1 <= m - 1) {
+                        S.Y2 = yline(F, k + 1);
+                        S.rho2 = F->rho + (long)(k + 1) * n;
+                        S.kap2 = F->kap + (long)(k + 1) * n;
+                        S.l2 = P.U + k * sj;
+                        S.u2 = P.L + (k + 1) * sj;
+                    }
+                    sjb[(size_t)JS * s + nj] = S;
+                    ++nj;
+                    ++nA[s];
+                    continue;
                 }
-                if (k + 1 <= m - 1) {
-                    S.Y2 = yline(F, k + 1);
-                    S.rho2 = F->rho + (long)(k + 1) * n;
-                    S.kap2 = F->kap + (long)(k + 1) * n;
-                    S.l2 = P.U + k * sj;
-                    S.u2 = P.L + (k + 1) * sj;
+                if (chain == 0 && s < k) {  // chain A, line j = s
+                    const int j = s, c = 2 * f;
+                    lj[(size_t)JS * s + nj] = line_job(F, j, c);
+                    SchurJob S = schur_base(F, P, j, c);
+                    if (j > 0) {  // S_j = D_j - L_j X_{j-1} U_{j-1}
+                        S.Y1 = yline(F, j - 1);
+                        S.rhoinv1 = F->rhoinv + (long)(j - 1) * n;
+                        S.kap1 = F->kap + (long)(j - 1) * n;
+                        S.l1 = P.L + j * sj;
+                        S.u1 = P.U + (j - 1) * sj;
+                    }
+                    sjb[(size_t)JS * s + nj] = S;
+                    ++nj;
+                    ++nA[s];
                 }
-                sjb[(size_t)JS * s + nj] = S;
-                ++nj;
-                continue;
-            }
-            if (s < k) {  // chain A, line j = s:  S_j = D_j - L_j X_{j-1} U_{j-1}
-                const int j = s, c = 2 * f;
-                lj[(size_t)JS * s + nj] = line_job(F, j, c);
-                SchurJob S = schur_base(F, P, j, c);
-                if (j > 0) {
-                    S.Y1 = yline(F, j - 1);
-                    S.rhoinv1 = F->rhoinv + (long)(j - 1) * n;
-                    S.kap1 = F->kap + (long)(j - 1) * n;
-                    S.l1 = P.L + j * sj;
-                    S.u1 = P.U + (j - 1) * sj;
+                if (chain == 1 && s < m - 1 - k) {  // chain B, line j = m-1-s
+                    const int j = m - 1 - s, c = 2 * f + 1;
+                    lj[(size_t)JS * s + nj] = line_job(F, j, c);
+                    SchurJob S = schur_base(F, P, j, c);
+                    if (j < m - 1) {  // S_j = D_j - U_j X_{j+1} L_{j+1}
+                        S.Y1 = yline(F, j + 1);
+                        S.rhoinv1 = F->rhoinv + (long)(j + 1) * n;
+                        S.kap1 = F->kap + (long)(j + 1) * n;
+                        S.l1 = P.U + j * sj;
+                        S.u1 = P.L + (j + 1) * sj;
+                    }
+                    sjb[(size_t)JS * s + nj] = S;
+                    ++nj;
+                    ++nB[s];
                 }
-                sjb[(size_t)JS * s + nj] = S;
-                ++nj;
-            }
-            if (s < m - 1 - k) {  // chain B, line j = m-1-s:  S_j = D_j - U_j X_{j+1} L_{j+1}
-                const int j = m - 1 - s, c = 2 * f + 1;
-                lj[(size_t)JS * s + nj] = line_job(F, j, c);
-                SchurJob S = schur_base(F, P, j, c);
-                if (j < m - 1) {
-                    S.Y1 = yline(F, j + 1);
-                    S.rhoinv1 = F->rhoinv + (long)(j + 1) * n;
-                    S.kap1 = F->kap + (long)(j + 1) * n;
-                    S.l1 = P.U + j * sj;
-                    S.u1 = P.L + (j + 1) * sj;
-                }
-                sjb[(size_t)JS * s + nj] = S;
-                ++nj;
             }
         }
         njobs[s] = nj;
     }
     const bool fp64 = (prec == FDFD_FP64);
     // one pass of the factorization as a launch sequence
+    // chain A runs on the handle's stream and chain B on the second stream
+    // (they are independent until the middle line); joined before the middle
     auto launch_pass = [&](const LineJob* lj_d, const SchurJob* sj_d, bool full, bool prof) {
-        auto scope = [&](int id, double fl, double by) {
-            return prof ? new ProfScope(h, id, fl, by) : nullptr;
-        };
-        for (int s = 0; s <= nsteps; ++s) {
-            const int nj = njobs[s];
-            const LineJob* J = lj_d + (size_t)JS * s;
+        auto line = [&](hipStream_t st, const LineJob* J, const SchurJob* SJ, int nj) {
+            auto scope = [&](int id, double fl, double by) {
+                return prof ? new ProfScope(h, id, fl, by, st) : nullptr;
+            };
             {
                 ProfScope* ps = scope(P_SCHUR, 0, 2.0 * n * n * elem * nj);
-                if (fp64) launch_schur<double>(h->stream, sj_d + (size_t)JS * s, nj, n);
-                else launch_schur<float>(h->stream, sj_d + (size_t)JS * s, nj, n);
+                if (fp64) launch_schur<double>(st, SJ, nj, n);
+                else launch_schur<float>(st, SJ, nj, n);
                 delete ps;
             }
             for (int k0 = 0; k0 < n;) {
@@ -511,33 +544,44 @@ static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     ProfScope* ps = scope(P_PANEL, 0, 0);
                     if (!full) {
                         nb = std::min(NB, n - k0);
-                        if (fp64) launch_wpanel<double>(h->stream, J, nj, n, k0, nb, fresh);
-                        else launch_wpanel<float>(h->stream, J, nj, n, k0, nb, fresh);
+                        if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb, fresh);
+                        else launch_wpanel<float>(st, J, nj, n, k0, nb, fresh);
                     } else {
                         nb = 16;  // full-search panel width: powers of two
                         while (nb > n - k0) nb >>= 1;
-                        if (fp64) launch_panel<double>(h->stream, J, nj, n, k0, nb);
-                        else launch_panel<float>(h->stream, J, nj, n, k0, nb);
+                        if (fp64) launch_panel<double>(st, J, nj, n, k0, nb);
+                        else launch_panel<float>(st, J, nj, n, k0, nb);
                     }
                     delete ps;
                 }
                 {
                     ProfScope* ps = scope(P_ZBUILD, 0, 0);
-                    if (fp64) launch_zbuild<double>(h->stream, J, nj, n, k0, nb, fresh, !full);
-                    else launch_zbuild<float>(h->stream, J, nj, n, k0, nb, fresh, !full);
+                    if (fp64) launch_zbuild<double>(st, J, nj, n, k0, nb, fresh, !full);
+                    else launch_zbuild<float>(st, J, nj, n, k0, nb, fresh, !full);
                     delete ps;
                 }
                 {
                     ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * nb * nj,
                                           2.0 * n * (double)n * elem * nj);
-                    if (fp64) launch_trailing<double>(h->stream, J, nj, n, k0, nb);
-                    else launch_trailing<float>(h->stream, J, nj, n, k0, nb);
+                    if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb);
+                    else launch_trailing<float>(st, J, nj, n, k0, nb);
                     delete ps;
                 }
                 k0 += nb;
             }
-            launch_maps(h->stream, J, nj, n);
+            launch_maps(st, J, nj, n);
+        };
+        (void)hipEventRecord(h->ev_fork, h->stream);
+        (void)hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+        for (int s = 0; s < nsteps; ++s) {
+            const LineJob* J = lj_d + (size_t)JS * s;
+            const SchurJob* SJ = sj_d + (size_t)JS * s;
+            if (nA[s]) line(h->stream, J, SJ, nA[s]);
+            if (nB[s]) line(h->stream2, J + nA[s], SJ + nA[s], nB[s]);
         }
+        (void)hipEventRecord(h->ev_join, h->stream2);
+        (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
+        line(h->stream, lj_d + (size_t)JS * nsteps, sj_d + (size_t)JS * nsteps, njobs[nsteps]);
     };
 
     hipEvent_t e0, e1;
@@ -803,7 +847,10 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.device = device_ordinal;
     const char* eg = std::getenv("FDFD_GRAPHS");
     hh->impl.use_graphs = !(eg && eg[0] == '0');
-    if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&hh->impl.stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&hh->impl.ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&hh->impl.ev_join, hipEventDisableTiming) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;
     }
@@ -825,6 +872,9 @@ int fdfd_destroy(fdfd_handle* hh) {
     for (void* b : bufs) dfree(b);
     prof_resolve(h->prof);
     for (auto e : h->prof.pool) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(h->ev_fork);
+    (void)hipEventDestroy(h->ev_join);
+    (void)hipStreamDestroy(h->stream2);
     (void)hipStreamDestroy(h->stream);
     delete hh;
     return FDFD_OK;

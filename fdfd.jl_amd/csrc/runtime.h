@@ -47,6 +47,7 @@ struct Factor {
     hipGraphExec_t sweep_exec[2] = {nullptr, nullptr};
     int nfwd = 0, nbwd = 0;     // number of forward / backward steps
     std::vector<int> fwd_jobs, bwd_jobs;  // jobs per step
+    std::vector<int> fwd_has_a, bwd_has_a;  // step has a chain-A job (listed first)
     double t_factor_ms = 0;
     double pivot_growth = 0;
     int singular_line = -1;
@@ -65,6 +66,8 @@ struct FactorGraph {
 struct Handle {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;          // second elimination chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     GridDesc g{};
     bool has_grid = false, has_eps = false;
     double2* eps = nullptr;
