@@ -109,8 +109,8 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         }
         double2* dsrc = nullptr;
         if (hipMalloc(&dsrc, sizeof(double2) * N * job.nsrc) != hipSuccess ||
-            hipMemcpy(dsrc, job.srcs.data(), sizeof(double2) * N * job.nsrc,
-                      hipMemcpyHostToDevice) != hipSuccess) {
+            stream_copy(h, dsrc, job.srcs.data(), sizeof(double2) * N * job.nsrc,
+                        hipMemcpyHostToDevice) != FDFD_OK) {
             if (dsrc) (void)hipFree(dsrc);
             rc = fail(h, FDFD_ERR_OOM, "batch: source upload failed");
             break;
@@ -202,9 +202,7 @@ int fdfd_batch_result(fdfd_batch* b, int job_id, int iomega, int isrc, fdfd_comp
         return fail(h, FDFD_ERR_INVALID, "frequency or source index out of range");
     const long N = (long)j.g.Nx * j.g.Ny;
     const double2* src = j.result + ((long)iomega * j.nsrc + isrc) * 3 * N;
-    if (hipMemcpy(fields, src, sizeof(double2) * 3 * N, hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(h, FDFD_ERR_HIP, "copy of batch result failed");
-    return FDFD_OK;
+    return stream_copy(h, fields, src, sizeof(double2) * 3 * N, hipMemcpyDeviceToHost);
 }
 
 int fdfd_batch_destroy(fdfd_batch* b) {

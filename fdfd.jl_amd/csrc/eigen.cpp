@@ -101,9 +101,9 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
             double b = (double)(s >> 11) / 9007199254740992.0 - 0.5;
             v0[i] = make_double2(a, b);
         }
-        if (hipMemcpy(vec(0), v0.data(), sizeof(double2) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        if ((rc = stream_copy(h, vec(0), v0.data(), sizeof(double2) * N, hipMemcpyHostToDevice))) {
             cleanup();
-            return fail(h, FDFD_ERR_HIP, "copy of start vector failed");
+            return rc;
         }
         double nv;
         if ((rc = rt_norm(h, vec(0), N, &nv))) { cleanup(); return rc; }
@@ -224,10 +224,10 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
             for (int i = 0; i < m; ++i) col[i] = Y[(size_t)i * m + sel[q]];
             launch_zero(h->stream, x, N);
             if ((rc = rt_update(h, x, Vb, N, m, col.data(), 1.0))) { (void)hipFree(outd); cleanup(); return rc; }
-            if (hipMemcpy(hx.data(), x, sizeof(double2) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+            if ((rc = stream_copy(h, hx.data(), x, sizeof(double2) * N, hipMemcpyDeviceToHost))) {
                 (void)hipFree(outd);
                 cleanup();
-                return fail(h, FDFD_ERR_HIP, "copy failed");
+                return rc;
             }
             double nx = 0, best = -1;
             cd ph = 1;
@@ -242,11 +242,11 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
             cd w = 1.0 / (cd(0, 1) * om[q] * c);
             launch_fields(h->stream, outd, x, h->eps, F->sxf, F->sxb, F->syf, F->syb, g, pol,
                           make_double2(w.real(), w.imag()), 1);
-            if (hipMemcpy(fields + (size_t)q * 3 * N, outd, sizeof(double2) * 3 * N,
-                          hipMemcpyDeviceToHost) != hipSuccess) {
+            if ((rc = stream_copy(h, fields + (size_t)q * 3 * N, outd, sizeof(double2) * 3 * N,
+                                  hipMemcpyDeviceToHost))) {
                 (void)hipFree(outd);
                 cleanup();
-                return fail(h, FDFD_ERR_HIP, "copy of fields failed");
+                return rc;
             }
         }
         (void)hipFree(outd);

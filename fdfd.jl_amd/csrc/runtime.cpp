@@ -31,6 +31,18 @@ int fail(Handle* h, int code, const std::string& msg) {
                         std::string(#call) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
+int stream_copy(Handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    HIPCHK(h, hipMemcpyAsync(dst, src, bytes, kind, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FDFD_OK;
+}
+
+#define COPY(h, dst, src, bytes, kind)                         \
+    do {                                                       \
+        int rc_ = stream_copy((h), (dst), (src), (bytes), (kind)); \
+        if (rc_) return rc_;                                   \
+    } while (0)
+
 // ---------------------------------------------------------------------------
 // profiling: HIP events around launches on the handle's stream
 // ---------------------------------------------------------------------------
@@ -294,8 +306,8 @@ static int build_sweep_tables(Handle* h, Factor* F) {
             F->bwd_jobs[s] = nj;
         }
         HIPCHK(h, hipMalloc(&F->sweep_tab[tr], sizeof(SweepJob) * tab.size()));
-        HIPCHK(h, hipMemcpy(F->sweep_tab[tr], tab.data(), sizeof(SweepJob) * tab.size(),
-                            hipMemcpyHostToDevice));
+        COPY(h, F->sweep_tab[tr], tab.data(), sizeof(SweepJob) * tab.size(),
+             hipMemcpyHostToDevice);
     }
     return FDFD_OK;
 }
@@ -638,7 +650,7 @@ static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         HIPCHK(h, hipEventRecord(e1, h->stream));
         HIPCHK(h, hipEventSynchronize(e1));
         int verr = 0;
-        HIPCHK(h, hipMemcpy(&verr, h->verr, sizeof(int), hipMemcpyDeviceToHost));
+        COPY(h, &verr, h->verr, sizeof(int), hipMemcpyDeviceToHost);
         if (full || !verr) break;
     }
     float ms = 0;
@@ -650,8 +662,7 @@ static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         F->t_factor_ms = ms;
         F->pivot_fallback = used_full;
         std::vector<double> ps(3 * (size_t)m);
-        HIPCHK(h, hipMemcpy(ps.data(), F->pivstat, sizeof(double) * ps.size(),
-                            hipMemcpyDeviceToHost));
+        COPY(h, ps.data(), F->pivstat, sizeof(double) * ps.size(), hipMemcpyDeviceToHost);
         double mn = std::numeric_limits<double>::infinity(), mx = 0;
         F->singular_line = -1;
         for (int j = 0; j < m; ++j) {
@@ -920,7 +931,7 @@ int fdfd_set_eps(fdfd_handle* hh, const fdfd_complex* eps) {
     if (!h->has_grid) return fail(h, FDFD_ERR_STATE, "fdfd_set_grid must be called first");
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
-    HIPCHK(h, hipMemcpy(h->eps, eps, sizeof(double2) * N, hipMemcpyHostToDevice));
+    COPY(h, h->eps, eps, sizeof(double2) * N, hipMemcpyHostToDevice);
     h->has_eps = true;
     return FDFD_OK;
 }
@@ -1043,12 +1054,9 @@ int fdfd_solve(fdfd_handle* hh, int factor_id, int nrhs, const fdfd_complex* src
         return fail(h, FDFD_ERR_OOM, "output allocation failed");
     }
     int rc = FDFD_OK;
-    if (hipMemcpy(ds, src, sizeof(double2) * N * nrhs, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(h, FDFD_ERR_HIP, "copy of sources failed");
+    rc = stream_copy(h, ds, src, sizeof(double2) * N * nrhs, hipMemcpyHostToDevice);
     if (!rc) rc = fdfd_solve_d(hh, factor_id, nrhs, ds, dout, tol, stats);
-    if (!rc && hipMemcpy(out, dout, sizeof(double2) * N * 3 * nrhs, hipMemcpyDeviceToHost) !=
-                   hipSuccess)
-        rc = fail(h, FDFD_ERR_HIP, "copy of fields failed");
+    if (!rc) rc = stream_copy(h, out, dout, sizeof(double2) * N * 3 * nrhs, hipMemcpyDeviceToHost);
     dfree(ds);
     dfree(dout);
     return rc;
@@ -1066,7 +1074,7 @@ int fdfd_linsolve(fdfd_handle* hh, int factor_id, int transpose, int nrhs, const
     SolveStats st;
     double worst = 0;
     for (int r = 0; r < nrhs; ++r) {
-        HIPCHK(h, hipMemcpy(h->tmp, b + (long)r * N, sizeof(double2) * N, hipMemcpyHostToDevice));
+        COPY(h, h->tmp, b + (long)r * N, sizeof(double2) * N, hipMemcpyHostToDevice);
         launch_scale_copy(h->stream, h->bvec, h->tmp, make_double2(F->scale, 0), N);
         launch_zero(h->stream, h->xvec, N);
         SolveStats s1;
@@ -1097,7 +1105,7 @@ int fdfd_apply(fdfd_handle* hh, int factor_id, int transpose, const fdfd_complex
     if (!F) return fail(h, FDFD_ERR_NOTFOUND, "unknown factor id");
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
-    HIPCHK(h, hipMemcpy(h->tmp, x, sizeof(double2) * N, hipMemcpyHostToDevice));
+    COPY(h, h->tmp, x, sizeof(double2) * N, hipMemcpyHostToDevice);
     launch_apply(h->stream, h->tmp2, h->tmp, F->coef, h->g.Nx, h->g.Ny, transpose ? 1 : 0,
                  nullptr);
     launch_scale_copy(h->stream, h->tmp2, h->tmp2, make_double2(1.0 / F->scale, 0), N);
@@ -1114,8 +1122,8 @@ int fdfd_residual(fdfd_handle* hh, int factor_id, const fdfd_complex* x, const f
     if (!F) return fail(h, FDFD_ERR_NOTFOUND, "unknown factor id");
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
-    HIPCHK(h, hipMemcpy(h->tmp, x, sizeof(double2) * N, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->tmp2, b, sizeof(double2) * N, hipMemcpyHostToDevice));
+    COPY(h, h->tmp, x, sizeof(double2) * N, hipMemcpyHostToDevice);
+    COPY(h, h->tmp2, b, sizeof(double2) * N, hipMemcpyHostToDevice);
     launch_scale_copy(h->stream, h->tmp2, h->tmp2, make_double2(F->scale, 0), N);
     double bn = 0, rn = 0;
     int rc = dev_norm(h, h->tmp2, N, &bn);
@@ -1137,8 +1145,8 @@ int fdfd_adjoint_grad(fdfd_handle* hh, int factor_id, const fdfd_complex* e_fwd,
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
     const double eps0n = EPS0 * h->g.L0;
-    HIPCHK(h, hipMemcpy(h->tmp, e_fwd, sizeof(double2) * N, hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->tmp2, lam, sizeof(double2) * N, hipMemcpyHostToDevice));
+    COPY(h, h->tmp, e_fwd, sizeof(double2) * N, hipMemcpyHostToDevice);
+    COPY(h, h->tmp2, lam, sizeof(double2) * N, hipMemcpyHostToDevice);
     double* dg = (double*)h->bvec;
     const double s = -F->omega * F->omega * eps0n;
     launch_grad(h->stream, dg, h->tmp, h->tmp2, make_double2(s * weight.re, s * weight.im), N);

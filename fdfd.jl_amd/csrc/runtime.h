@@ -99,6 +99,12 @@ struct Handle {
 // error helper: record message, return code
 int fail(Handle* h, int code, const std::string& msg);
 
+// Copy ordered on the handle's stream, complete on return.  The handle's
+// streams are non-blocking, so a plain hipMemcpy (null stream) would not wait
+// for kernels still queued on them; every host<->device transfer of the
+// library goes through here instead.
+int stream_copy(Handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+
 }  // namespace fdfd
 
 struct fdfd_handle { fdfd::Handle impl; };
