@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12}
-KERNELS = ["trailing", "panel", "schur", "sweep", "stencil", "zbuild"]
+KERNELS = ["trailing", "panel", "form", "solve", "stencil", "zbuild", "gemm"]
 
 
 class Hip:

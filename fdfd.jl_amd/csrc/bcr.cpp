@@ -1,0 +1,396 @@
+// Block cyclic reduction of the line-block operator (host side).
+//
+// Lines j = 0..m-1 (n cells each) form a ring: line j couples to j-1 and j+1
+// through diagonal blocks, and the y-periodic corner blocks close the ring
+// (src/grid.jl:138-150 wraps the differences).  A reduction level removes
+// every other line of the current ring (all of them at once):
+//   X_j = D_j^-1                                   (Gauss-Jordan, batched)
+//   D_s -= C_sj X_j C_js          for both surviving neighbours s of j
+//   C'(s, u) = [C(s, u)] - C_sj X_j C_ju   (new coupling s -> u across j)
+// until one line is left.  The critical path is log2(m) batched inversions
+// instead of the m sequential ones of a block-Thomas chain, and the corner
+// blocks are kept exactly, so the factorization is a direct solver of A'.
+//
+// The plan (which blocks, which jobs, in which order) depends on m only; it
+// is built symbolically and then instantiated for each factor as device job
+// tables.  Block ids 0..m-1 are the lines' own blocks (the inverse of line j
+// stays in block j); coupling and product blocks are drawn from a pool.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "../../include/fdfd.h"
+#include "bcr_plan.h"
+#include "runtime.h"
+
+namespace fdfd {
+
+using namespace bcr;
+
+int rt_assemble(Handle* h, Factor* F);
+
+size_t rt_factor_bytes(int n, int m, int precision) {
+    const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
+    const Plan P = make_plan(m);
+    return (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
+}
+
+// ---------------------------------------------------------------------------
+// device tables for a set of factors built together (same grid and shape)
+// ---------------------------------------------------------------------------
+struct Tables {
+    FormJob* form = nullptr;
+    LineJob* lj = nullptr;
+    GemmJob* gemm = nullptr;
+    ~Tables() {
+        if (form) (void)hipFree(form);
+        if (lj) (void)hipFree(lj);
+        if (gemm) (void)hipFree(gemm);
+    }
+};
+
+static const double2* plane_ptr(const Factor* F, long N, int line, int pl) {
+    // plane 0: coupling to the previous line, plane 1: to the next line
+    const double2* c = F->coef;
+    int k;
+    if (F->orient == 0) k = pl ? CN : CS;
+    else k = pl ? CE : CW;
+    return c + (long)k * N + (long)line * F->sj;
+}
+
+static void own_planes(const Factor* F, long N, int line, const double2** sub,
+                       const double2** diag, const double2** sup) {
+    const double2* c = F->coef;
+    const long off = (long)line * F->sj;
+    if (F->orient == 0) {
+        *sub = c + (long)CW * N + off;
+        *sup = c + (long)CE * N + off;
+    } else {
+        *sub = c + (long)CS * N + off;
+        *sup = c + (long)CN * N + off;
+    }
+    *diag = c + (long)CC * N + off;
+}
+
+static void* blk(const Factor* F, int id) {
+    return (char*)F->blocks + (size_t)id * F->n * F->n * F->elem;
+}
+
+int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
+    const GridDesc& g = h->g;
+    const long N = (long)g.Nx * g.Ny;
+    const int B = (int)Fs.size();
+    if (B < 1) return fail(h, FDFD_ERR_INVALID, "no factor to build");
+    const int prec = Fs[0]->precision;
+    for (Factor* F : Fs) {
+        if (F->precision != prec)
+            return fail(h, FDFD_ERR_INVALID, "factors built together share one precision");
+        if (g.Nx <= g.Ny) { F->orient = 0; F->n = g.Nx; F->m = g.Ny; F->si = 1; F->sj = g.Nx; }
+        else { F->orient = 1; F->n = g.Ny; F->m = g.Nx; F->si = g.Nx; F->sj = 1; }
+        F->elem = (prec == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
+    }
+    const int n = Fs[0]->n, m = Fs[0]->m;
+    const size_t elem = Fs[0]->elem;
+    if (n > MAX_LINE)
+        return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
+    const Plan P = make_plan(m);
+    int rc;
+    if ((rc = ensure_factor_work(h, n, std::max(1, P.max_gj * B)))) return rc;
+    if ((rc = ensure_solve_work(h, N))) return rc;
+    for (Factor* F : Fs) {
+        if ((rc = rt_assemble(h, F))) return rc;
+        F->nblocks = P.nblocks;
+        const size_t bytes = (size_t)P.nblocks * n * n * elem;
+        if (hipMalloc(&F->blocks, bytes) != hipSuccess) {
+            F->blocks = nullptr;
+            return fail(h, FDFD_ERR_OOM, "factor storage (" + std::to_string(bytes >> 20) +
+                                             " MiB) does not fit in device memory");
+        }
+        HIPCHK(h, hipMalloc(&F->prow, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->iprow, sizeof(int) * (size_t)m * n));
+        HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
+    }
+    // --- factorization job tables: per step, the jobs of factor 0, 1, ...
+    std::vector<FormJob> forms;
+    std::vector<LineJob> ljs;
+    std::vector<GemmJob> gemms;
+    for (const Step& st : P.fsteps) {
+        for (int f = 0; f < B; ++f) {
+            const Factor* F = Fs[f];
+            auto dvec = [&](const SDiag& d, const double2** out2) {
+                out2[0] = out2[1] = nullptr;
+                for (int q = 0; q < 2; ++q)
+                    if (d.line >= 0 && d.pl[q] >= 0) out2[q] = plane_ptr(F, N, d.line, d.pl[q]);
+            };
+            for (int i = 0; i < st.count; ++i) {
+                if (st.kind == ST_FORM) {
+                    const SForm& s = P.forms[st.first + i];
+                    FormJob J{};
+                    J.out = blk(F, s.out);
+                    if (s.own >= 0) own_planes(F, N, s.own, &J.sub, &J.diag, &J.sup);
+                    dvec(s.dd, J.dd);
+                    J.nt = s.nt;
+                    for (int t = 0; t < s.nt; ++t) {
+                        J.Y[t] = blk(F, s.jl[t]);
+                        J.prow[t] = F->prow + (long)s.jl[t] * n;
+                        J.iprow[t] = F->iprow + (long)s.jl[t] * n;
+                        dvec(s.rs[t], J.rs[t]);
+                        dvec(s.cs[t], J.cs[t]);
+                    }
+                    J.si = F->si;
+                    forms.push_back(J);
+                } else if (st.kind == ST_GJ) {
+                    const int j = P.gj[st.first + i];
+                    const int c = f * st.count + i;  // work channel
+                    LineJob J{};
+                    J.Y = blk(F, j);
+                    J.used = h->used + (long)c * n;
+                    J.prow = F->prow + (long)j * n;
+                    J.iprow = F->iprow + (long)j * n;
+                    J.Vbuf = (char*)h->Vbuf + (size_t)c * n * PMAX * elem;
+                    J.LU = (char*)h->LU + (size_t)c * 2 * PMAX * PMAX * elem;
+                    J.piv = h->piv + c * PMAX;
+                    J.Z = (char*)h->Zb + (size_t)c * PMAX * n * elem;
+                    J.win = h->win + c * WIN;
+                    J.verr = h->verr;
+                    J.pivstat = F->pivstat + 3 * (long)j;
+                    J.n = n;
+                    ljs.push_back(J);
+                } else {
+                    const SGemm& s = P.gemms[st.first + i];
+                    GemmJob J{};
+                    J.out = blk(F, s.out);
+                    J.nt = s.nt;
+                    J.acc = s.acc;
+                    J.sub = s.sub;
+                    for (int t = 0; t < s.nt; ++t) {
+                        J.A[t] = blk(F, s.A[t]);
+                        J.B[t] = blk(F, s.B[t]);
+                        J.bmap[t] = s.bmap[t] >= 0 ? F->prow + (long)s.bmap[t] * n : nullptr;
+                    }
+                    gemms.push_back(J);
+                }
+            }
+        }
+    }
+    Tables T;
+    auto upload = [&](auto** dst, const auto& v) -> int {
+        if (v.empty()) return FDFD_OK;
+        const size_t bytes = sizeof(v[0]) * v.size();
+        HIPCHK(h, hipMalloc((void**)dst, bytes));
+        return stream_copy(h, (void*)*dst, v.data(), bytes, hipMemcpyHostToDevice);
+    };
+    if ((rc = upload(&T.form, forms)) || (rc = upload(&T.lj, ljs)) || (rc = upload(&T.gemm, gemms)))
+        return rc;
+
+    const bool fp64 = (prec == FDFD_FP64);
+    const int NB = h->panel_width ? h->panel_width : PMAX;
+    auto launch_all = [&](bool full, bool prof) {
+        hipStream_t st = h->stream;
+        auto scope = [&](int id, double fl, double by) {
+            return prof ? new ProfScope(h, id, fl, by, st) : nullptr;
+        };
+        int of = 0, ol = 0, og = 0;
+        for (const Step& s : P.fsteps) {
+            const int nj = s.count * B;
+            if (s.kind == ST_FORM) {
+                ProfScope* ps = scope(P_SCHUR, 0, (double)n * n * elem * nj);
+                if (fp64) launch_form<double>(st, T.form + of, nj, n);
+                else launch_form<float>(st, T.form + of, nj, n);
+                delete ps;
+                of += nj;
+            } else if (s.kind == ST_GEMM) {
+                double fl = 0;
+                for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
+                ProfScope* ps = scope(P_GEMM, fl, 0);
+                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n);
+                else launch_gemm<float>(st, T.gemm + og, nj, n);
+                delete ps;
+                og += nj;
+            } else {
+                const LineJob* J = T.lj + ol;
+                launch_gjreset(st, J, nj, n);
+                for (int k0 = 0; k0 < n;) {
+                    int nb;
+                    const int fresh = std::min(n, WIN + k0);
+                    {
+                        ProfScope* ps = scope(P_PANEL, 0, 0);
+                        if (!full) {
+                            nb = std::min(NB, n - k0);
+                            if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb, fresh);
+                            else launch_wpanel<float>(st, J, nj, n, k0, nb, fresh);
+                        } else {
+                            nb = 16;  // full-search panel width: powers of two
+                            while (nb > n - k0) nb >>= 1;
+                            if (fp64) launch_panel<double>(st, J, nj, n, k0, nb);
+                            else launch_panel<float>(st, J, nj, n, k0, nb);
+                        }
+                        delete ps;
+                    }
+                    {
+                        ProfScope* ps = scope(P_ZBUILD, 0, 0);
+                        if (fp64) launch_zbuild<double>(st, J, nj, n, k0, nb, fresh, !full);
+                        else launch_zbuild<float>(st, J, nj, n, k0, nb, fresh, !full);
+                        delete ps;
+                    }
+                    {
+                        ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * nb * nj,
+                                              2.0 * n * (double)n * elem * nj);
+                        if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb);
+                        else launch_trailing<float>(st, J, nj, n, k0, nb);
+                        delete ps;
+                    }
+                    k0 += nb;
+                }
+                ol += nj;
+            }
+        }
+    };
+
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    // Pass 0: windowed pivot search with verification.  Pass 1 (only if a
+    // row outside a window would have won a GEPP step): full-column search.
+    int used_full = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool full = (pass == 1) || h->full_pivot_search;
+        used_full = full ? 1 : 0;
+        HIPCHK(h, hipMemsetAsync(h->verr, 0, sizeof(int), h->stream));
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        launch_all(full, h->prof.enabled);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        int verr = 0;
+        COPY(h, &verr, h->verr, sizeof(int), hipMemcpyDeviceToHost);
+        if (full || !verr) break;
+    }
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int first_singular = -1;
+    for (Factor* F : Fs) {
+        F->t_factor_ms = ms;
+        F->pivot_fallback = used_full;
+        std::vector<double> ps(3 * (size_t)m);
+        COPY(h, ps.data(), F->pivstat, sizeof(double) * ps.size(), hipMemcpyDeviceToHost);
+        double mn = std::numeric_limits<double>::infinity(), mx = 0;
+        F->singular_line = -1;
+        for (int j = 0; j < m; ++j) {
+            mn = std::min(mn, ps[3 * j]);
+            mx = std::max(mx, ps[3 * j + 1]);
+            if (ps[3 * j + 2] != 0 && F->singular_line < 0) F->singular_line = j;
+        }
+        F->pivot_growth = (mn > 0) ? mx / mn : std::numeric_limits<double>::infinity();
+        if (F->singular_line >= 0 && first_singular < 0) first_singular = F->singular_line;
+    }
+    if (first_singular >= 0)
+        return fail(h, FDFD_ERR_SINGULAR, "zero pivot in line block " + std::to_string(first_singular));
+
+    // --- reduced-solve program of each factor
+    for (Factor* F : Fs) {
+        std::vector<MvJob> mvs;
+        std::vector<DiagJob> dgs;
+        auto vec = [&](int v, int line) {
+            double2* base = (v == VX) ? h->x_out : h->g_mid;
+            return base + (long)line * F->sj;
+        };
+        for (const SMv& s : P.mvs) {
+            MvJob J{};
+            J.nt = s.nt;
+            for (int t = 0; t < s.nt; ++t) {
+                J.M[t] = blk(F, s.M[t]);
+                J.imap[t] = s.imap[t] >= 0 ? F->prow + (long)s.imap[t] * n : nullptr;
+                J.x[t] = vec(s.xv[t], s.xl[t]);
+            }
+            J.base = s.bl >= 0 ? vec(s.bv, s.bl) : nullptr;
+            J.out = vec(s.ov, s.ol);
+            J.omap = s.omap >= 0 ? F->iprow + (long)s.omap * n : nullptr;
+            J.si = F->si;
+            mvs.push_back(J);
+        }
+        for (const SDg& s : P.dgs) {
+            DiagJob J{};
+            J.out = vec(s.ov, s.ol);
+            J.x0 = vec(s.x0v, s.x0l);
+            J.nt = s.nt;
+            for (int t = 0; t < s.nt; ++t) {
+                J.x[t] = vec(s.xv[t], s.xl[t]);
+                for (int q = 0; q < 2; ++q)
+                    J.d[t][q] = (s.d[t].pl[q] >= 0) ? plane_ptr(F, N, s.d[t].line, s.d[t].pl[q])
+                                                     : nullptr;
+            }
+            J.si = F->si;
+            dgs.push_back(J);
+        }
+        if ((rc = upload(&F->mv_tab, mvs)) || (rc = upload(&F->dg_tab, dgs))) return rc;
+        F->ssteps.clear();
+        for (const Step& s : P.ssteps) F->ssteps.push_back({s.kind == ST_MV ? 0 : 1, s.first, s.count});
+    }
+    return FDFD_OK;
+}
+
+int rt_build_factor(Handle* h, Factor* F) { return rt_build_factors(h, std::vector<Factor*>{F}); }
+
+// x_out <- A'^-1 x_out through the reduced-solve program
+static void launch_solve(Handle* h, Factor* F, bool prof) {
+    const int n = F->n;
+    const bool fp64 = (F->precision == FDFD_FP64);
+    for (const auto& s : F->ssteps) {
+        ProfScope* ps = nullptr;
+        if (prof) {
+            const double bytes = s.kind == 0 ? (double)n * n * F->elem * s.count : 64.0 * n * s.count;
+            ps = new ProfScope(h, P_SWEEP, s.kind == 0 ? 8.0 * n * n * s.count : 0, bytes);
+        }
+        if (s.kind == 0) {
+            if (fp64) launch_mv<double>(h->stream, F->mv_tab + s.first, s.count, n);
+            else launch_mv<float>(h->stream, F->mv_tab + s.first, s.count, n);
+        } else {
+            launch_diag(h->stream, F->dg_tab + s.first, s.count, n);
+        }
+        delete ps;
+    }
+}
+
+// out = A'^-1 in, or A'^-T in = W^-1 A'^-1 W in (W = diag(sxf sxf)).  The
+// launch sequence of a factor is captured once in a HIP graph and replayed
+// (direct launches while per-kernel profiling is on).
+int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out) {
+    const GridDesc& g = h->g;
+    const long N = (long)g.Nx * g.Ny;
+    if (trans) launch_wscale(h->stream, h->x_out, in, F->sxf, F->syf, g.Nx, g.Ny, 0);
+    else
+        HIPCHK(h, hipMemcpyAsync(h->x_out, in, sizeof(double2) * N, hipMemcpyDeviceToDevice,
+                                 h->stream));
+    if (h->prof.enabled || !h->use_graphs) {
+        launch_solve(h, F, h->prof.enabled);
+    } else {
+        if (!F->solve_exec) {
+            hipGraph_t gr;
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            launch_solve(h, F, false);
+            HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+            hipError_t e = hipGraphInstantiate(&F->solve_exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                F->solve_exec = nullptr;
+                return fail(h, FDFD_ERR_HIP, std::string("solve graph: ") + hipGetErrorString(e));
+            }
+        }
+        HIPCHK(h, hipGraphLaunch(F->solve_exec, h->stream));
+    }
+    HIPCHK(h, hipGetLastError());
+    if (trans) launch_wscale(h->stream, out, h->x_out, F->sxf, F->syf, g.Nx, g.Ny, 1);
+    else
+        HIPCHK(h, hipMemcpyAsync(out, h->x_out, sizeof(double2) * N, hipMemcpyDeviceToDevice,
+                                 h->stream));
+    return FDFD_OK;
+}
+
+}  // namespace fdfd

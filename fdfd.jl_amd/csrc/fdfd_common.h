@@ -92,13 +92,10 @@ struct GridDesc {
 // One line-block inversion in flight (Gauss-Jordan with partial pivoting,
 // in place, pivot rows tracked by index instead of swapped).
 struct LineJob {
-    void* Y;          // n x n row-major complex (R); input S', output Y
+    void* Y;          // n x n row-major complex (R); input D, output Y
     int* used;        // n: row already chosen as a pivot
-    int* prow;        // n: prow[c] = pivot row of column c      (= rho of the line)
-    int* iprow;       // n: inverse of prow (-1 until pivoted)   (= rho^-1)
-    const int* q;     // n: row map of the work matrix, S'[r] = S[q(r)]
-    int* kap;         // n: column map, X[a,b] = Y[rho(a), kap(b)]
-    int* kapinv;      // n: inverse of kap
+    int* prow;        // n: prow[c] = pivot row of column c
+    int* iprow;       // n: inverse of prow (-1 until pivoted)
     void* Vbuf;       // n x PMAX panel copy (original panel values)
     void* LU;         // 2 x PMAX x PMAX: unit-lower L then upper U, V[piv, :] = L U
     int* piv;         // PMAX: pivot rows of the current panel
@@ -109,37 +106,66 @@ struct LineJob {
     int n;
 };
 
-// Formation of the work matrix S' of one line from its neighbours' factors.
-//   S'[r, b] = D[q(r), b] - l[q(r)] u[b] X1[q(r), b] - l2[q(r)] u2[b] X2[q(r), b]
-// with X1[a,b] = Y1[rho1(a), kap1(b)] and q = rho1^-1 (so the X1 term reads
-// Y1 row r directly).  Chain starts have no neighbour (Y1 == nullptr).
-struct SchurJob {
-    void* S;                  // destination n x n (line slot)
-    const void* Y1; const int* rhoinv1; const int* kap1;
-    const void* Y2; const int* rho2; const int* kap2;
-    const double2* dsub; const double2* ddiag; const double2* dsup;  // line coefficient planes
-    const double2* l1; const double2* u1;   // row / column scaling of X1 term
-    const double2* l2; const double2* u2;   // row / column scaling of X2 term
-    long si;                  // element stride along the line in natural layout
-    long su1, su2;            // stride of the u vectors (neighbour line)
-    int* q;                   // out: row map of S'
-    int* used; int* iprow;    // reset for the new line
-    int* win;                 // candidate window reset to rows 0..WIN-1
-    int n;
+// ---------------------------------------------------------------------------
+// Block cyclic reduction (lines on the y-periodic ring).  All blocks are
+// n x n row-major complex (R).  A line inverted by Gauss-Jordan keeps its
+// result Y in place with X = D^-1 read as X[a, b] = Y[prow[a], iprow[b]].
+// ---------------------------------------------------------------------------
+// Dense block formed from diagonal couplings (first reduction level):
+//   out[a, b] = own(a, b) + [a == b] (dd0 + dd1)[a]
+//               - sum_t (rs_t0 + rs_t1)[a] X_t[a, b] (cs_t0 + cs_t1)[b]
+// own = the line's periodic tridiagonal operator (sub, diag, sup; nullable).
+// Row vectors are indexed with the output line's stride si, column vectors
+// with the same stride (every line has the same layout).
+struct FormJob {
+    void* out;
+    const double2 *sub, *diag, *sup;
+    const double2* dd[2];
+    const void* Y[2];
+    const int* prow[2];
+    const int* iprow[2];
+    const double2* rs[2][2];
+    const double2* cs[2][2];
+    int nt;
+    long si;
 };
 
-// One GEMV of a block sweep (forward, backward or middle step of one chain).
-struct SweepJob {
-    const void* Y;              // n x n factor (R)
-    const int* rho; const int* rhoinv; const int* kap; const int* kapinv;
-    int mode;                   // 0 forward, 1 backward, 2 middle
-    const double2* b;           // rhs line (natural layout, stride si)
-    const double2* nb1; const double2* c1; long snb1;  // neighbour line values and coupling
-    const double2* nb2; const double2* c2; long snb2;
-    const double2* g;           // backward: forward-sweep value of this line
-    double2* out;               // output line
+// out = [acc ? out : 0] -/+ sum_t A_t * B_t[bmap_t, :]   (B rows gathered;
+// sub = 1 subtracts the products, sub = 0 adds them)
+struct GemmJob {
+    void* out;
+    const void* A[2];
+    const void* B[2];
+    const int* bmap[2];
+    int nt;
+    int acc;
+    int sub;
+};
+
+// Dense matrix-vector step of the reduced solve (vectors are field-shaped,
+// a line's entry i at x[i * si]):
+//   z = sum_t M_t w_t,  w_t[c] = x_t[imap_t ? imap_t[c] : c]
+//   out[omap ? omap[r] : r] = base ? base[r] - z[r] : z[r]
+struct MvJob {
+    const void* M[2];
+    const int* imap[2];
+    const double2* x[2];
+    const double2* base;
+    double2* out;
+    const int* omap;
+    int nt;
     long si;
-    int n;
+};
+
+// Diagonal step of the reduced solve (first level):
+//   out[i] = x0[i] - sum_t (d_t0 + d_t1)[i] x_t[i]
+struct DiagJob {
+    double2* out;
+    const double2* x0;
+    const double2* x[2];
+    const double2* d[2][2];
+    int nt;
+    long si;
 };
 
 // ---------------------------------------------------------------------------
@@ -171,8 +197,6 @@ void launch_grad(hipStream_t s, double* grad, const double2* e, const double2* l
                  double2 coef, long n);
 
 template <typename R>
-void launch_schur(hipStream_t s, const SchurJob* jobs_d, int njobs, int n);
-template <typename R>
 void launch_panel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
 template <typename R>
 void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
@@ -182,10 +206,16 @@ void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k
                    int fresh, int verify);
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
-void launch_maps(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
 template <typename R>
-void launch_sweep(hipStream_t s, const SweepJob* jobs_d, int njobs, int n, int transpose,
-                  double2* partial);
+void launch_form(hipStream_t s, const FormJob* jobs_d, int njobs, int n);
+void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
+template <typename R>
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n);
+template <typename R>
+void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
+void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);
+void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2* sxf,
+                   const double2* syf, int Nx, int Ny, int inverse);
 
 // largest line-block size the panel kernel supports (one row per thread)
 constexpr int MAX_LINE = 1024;
@@ -193,5 +223,9 @@ constexpr int MAX_LINE = 1024;
 constexpr int PMAX = 32;
 // rows in the pivot-candidate window of the fast panel (one wave)
 constexpr int WIN = 64;
+// threshold of the windowed pivot search: a window pivot is accepted when no
+// unpivoted row outside the window is larger than 1 / PIV_U times it (the
+// threshold partial pivoting of sparse LU codes; 1 would be strict GEPP)
+constexpr double PIV_U = 0.1;
 
 }  // namespace fdfd

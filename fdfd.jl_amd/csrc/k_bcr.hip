@@ -1,0 +1,405 @@
+// Block cyclic reduction kernels of libfdfd (gfx950).
+//
+// The lines of the grid form a ring (the y-periodic corner blocks couple the
+// first and last line).  Each reduction level eliminates every other line of
+// the current ring: the eliminated line's block is inverted (Gauss-Jordan,
+// k_factor.hip) and its couplings fold into its two surviving neighbours,
+//   D_s -= C_sj X_j C_js,   C_new(s, u) = -C_sj X_j C_ju.
+// On the first level the couplings are diagonal and the folds are
+// elementwise (k_form); afterwards they are dense and run as batched complex
+// GEMMs on MFMA (k_gemm) in the order Z = C_sj X_j, then D_s -= Z C_js.
+// The reduced solve applies the same steps to a vector (k_mv, k_diag).
+#include "fdfd_common.h"
+#include "k_device.h"
+
+namespace fdfd {
+
+// ---------------------------------------------------------------------------
+// First-level block formation: one workgroup per output row a.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_form(const FormJob* jobs, int n) {
+    using C = cplx<R>;
+    const FormJob J = jobs[blockIdx.y];
+    __shared__ C row0[MAX_LINE];
+    __shared__ C row1[MAX_LINE];
+    const int a = blockIdx.x;
+    const long si = J.si;
+    if (J.nt > 0) {
+        const C* y = (const C*)J.Y[0] + (long)J.prow[0][a] * n;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) row0[c] = y[c];
+    }
+    if (J.nt > 1) {
+        const C* y = (const C*)J.Y[1] + (long)J.prow[1][a] * n;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) row1[c] = y[c];
+    }
+    __syncthreads();
+    const long ea = (long)a * si;
+    const double2 z = make_double2(0, 0);
+    double2 dg = J.diag ? J.diag[ea] : z;
+    const double2 up = J.sup ? J.sup[ea] : z, dn = J.sub ? J.sub[ea] : z;
+    if (J.dd[0]) dg = cadd(dg, J.dd[0][ea]);
+    if (J.dd[1]) dg = cadd(dg, J.dd[1][ea]);
+    double2 r0 = z, r1 = z;
+    if (J.nt > 0) {
+        r0 = J.rs[0][0][ea];
+        if (J.rs[0][1]) r0 = cadd(r0, J.rs[0][1][ea]);
+    }
+    if (J.nt > 1) {
+        r1 = J.rs[1][0][ea];
+        if (J.rs[1][1]) r1 = cadd(r1, J.rs[1][1][ea]);
+    }
+    const int ap = (a + 1 == n) ? 0 : a + 1, am = (a == 0) ? n - 1 : a - 1;
+    C* out = (C*)J.out + (long)a * n;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) {
+        const long eb = (long)b * si;
+        double2 v = z;
+        if (b == a) v = cadd(v, dg);
+        if (b == ap) v = cadd(v, up);
+        if (b == am) v = cadd(v, dn);
+        if (J.nt > 0) {
+            double2 c0 = J.cs[0][0][eb];
+            if (J.cs[0][1]) c0 = cadd(c0, J.cs[0][1][eb]);
+            v = csub(v, cmul(cmul(r0, c0), to_d(row0[J.iprow[0][b]])));
+        }
+        if (J.nt > 1) {
+            double2 c1 = J.cs[1][0][eb];
+            if (J.cs[1][1]) c1 = cadd(c1, J.cs[1][1][eb]);
+            v = csub(v, cmul(cmul(r1, c1), to_d(row1[J.iprow[1][b]])));
+        }
+        out[b] = from_d<C>(v);
+    }
+}
+
+template <typename R>
+void launch_form(hipStream_t s, const FormJob* jobs_d, int njobs, int n) {
+    k_form<R><<<dim3(n, njobs), 256, 0, s>>>(jobs_d, n);
+}
+template void launch_form<double>(hipStream_t, const FormJob*, int, int);
+template void launch_form<float>(hipStream_t, const FormJob*, int, int);
+
+// ---------------------------------------------------------------------------
+// Gauss-Jordan start state of a line: no row pivoted, window = rows 0..63.
+// ---------------------------------------------------------------------------
+__global__ void k_gjreset(const LineJob* jobs, int n) {
+    const LineJob J = jobs[blockIdx.y];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) {
+        J.used[r] = 0;
+        J.iprow[r] = -1;
+    }
+    if (r < WIN) J.win[r] = (r < n) ? r : -1;
+    if (r == 0) {
+        J.pivstat[0] = __builtin_huge_val();
+        J.pivstat[1] = 0;
+        J.pivstat[2] = 0;
+    }
+}
+
+void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n) {
+    const int t = n > WIN ? n : WIN;
+    k_gjreset<<<dim3((t + 255) / 256, njobs), 256, 0, s>>>(jobs_d, n);
+}
+
+// ---------------------------------------------------------------------------
+// Batched complex GEMM on MFMA:  out = [acc ? out : 0] -/+ sum_t A_t B_t[bmap_t]
+// Workgroup tile TM x TN (2 x 2 waves), K staged GKT at a time through LDS
+// (double-buffered, re and im planes split, A stored k-major so that both
+// operands are read as MFMA fragments without bank conflicts).  The complex
+// product is four real MFMAs per 16x16 block:
+//   re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br.
+// ---------------------------------------------------------------------------
+template <typename R> struct gemm_cfg;
+template <> struct gemm_cfg<float> { static constexpr int TM = 128, TN = 128; };
+template <> struct gemm_cfg<double> { static constexpr int TM = 128, TN = 64; };
+constexpr int GKT = 16;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
+    using C = cplx<R>;
+    using M = mfma_t<R>;
+    using acc_t = typename M::acc;
+    constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
+    constexpr int WM = TM / 2, WN = TN / 2;
+    constexpr int BM = WM / 16, BN = WN / 16;
+    constexpr int PA = TM + 16, PB = TN + 16;
+    constexpr int AE = TM * GKT / 256;  // A elements per thread per stage
+    constexpr int BE = TN * GKT / 256;  // B elements per thread per stage
+    static_assert(AE == 8, "A loader assumes two threads per tile row");
+    __shared__ R sAr[2][GKT][PA], sAi[2][GKT][PA];
+    __shared__ R sBr[2][GKT][PB], sBi[2][GKT][PB];
+    const GemmJob J = jobs[blockIdx.z];
+    const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    // loader coordinates
+    const int arow = t >> 1, ak = (t & 1) * AE;   // A: row, first k of 8
+    const int bk = t >> 4, bc = t & 15;           // B: k row, column lane (stride 16)
+    const int spt = (n + GKT - 1) / GKT;
+    const int ns = J.nt * spt;
+    C ra[AE], rb[BE];
+    auto load = [&](int s) {
+        const int ts = s / spt, kb = (s - ts * spt) * GKT;
+        const C* A = (const C*)(ts ? J.A[1] : J.A[0]);
+        const C* B = (const C*)(ts ? J.B[1] : J.B[0]);
+        const int* bm = ts ? J.bmap[1] : J.bmap[0];
+        const int i = i0 + arow;
+        // (guarded assignments: a select between two complex structs would
+        // be lowered through a stack slot)
+        if (i < n && kb + ak + AE <= n) {
+            const C* src = A + (long)i * n + kb + ak;
+#pragma unroll
+            for (int e = 0; e < AE; ++e) ra[e] = src[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < AE; ++e) {
+                const int k = kb + ak + e;
+                ra[e].x = R(0);
+                ra[e].y = R(0);
+                if (i < n && k < n) ra[e] = A[(long)i * n + k];
+            }
+        }
+        const int k = kb + bk;
+        const C* srcb = B + (long)((k < n) ? (bm ? bm[k] : k) : 0) * n;
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int j = j0 + bc + 16 * e;
+            rb[e].x = R(0);
+            rb[e].y = R(0);
+            if (k < n && j < n) rb[e] = srcb[j];
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            sAr[buf][ak + e][arow] = ra[e].x;
+            sAi[buf][ak + e][arow] = ra[e].y;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            sBr[buf][bk][bc + 16 * e] = rb[e].x;
+            sBi[buf][bk][bc + 16 * e] = rb[e].y;
+        }
+    };
+    acc_t cre[BM][BN], cim[BM][BN];
+#pragma unroll
+    for (int bi = 0; bi < BM; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < BN; ++bj) {
+            cre[bi][bj] = acc_t{0, 0, 0, 0};
+            cim[bi][bj] = acc_t{0, 0, 0, 0};
+        }
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int s = 0; s < ns; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < ns) load(s + 1);
+#pragma unroll
+        for (int kk = 0; kk < GKT; kk += 4) {
+            R ar[BM], an[BM], ai[BM], br[BN], bi[BN];
+#pragma unroll
+            for (int q = 0; q < BM; ++q) {
+                ar[q] = sAr[cur][kk + lk][wr * WM + q * 16 + lr];
+                ai[q] = sAi[cur][kk + lk][wr * WM + q * 16 + lr];
+                an[q] = -ai[q];
+            }
+#pragma unroll
+            for (int q = 0; q < BN; ++q) {
+                br[q] = sBr[cur][kk + lk][wc * WN + q * 16 + lr];
+                bi[q] = sBi[cur][kk + lk][wc * WN + q * 16 + lr];
+            }
+            // four passes so that consecutive MFMAs use different accumulators
+#pragma unroll
+            for (int p = 0; p < BM; ++p)
+#pragma unroll
+                for (int q = 0; q < BN; ++q) cre[p][q] = M::run(ar[p], br[q], cre[p][q]);
+#pragma unroll
+            for (int p = 0; p < BM; ++p)
+#pragma unroll
+                for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ar[p], bi[q], cim[p][q]);
+#pragma unroll
+            for (int p = 0; p < BM; ++p)
+#pragma unroll
+                for (int q = 0; q < BN; ++q) cre[p][q] = M::run(an[p], bi[q], cre[p][q]);
+#pragma unroll
+            for (int p = 0; p < BM; ++p)
+#pragma unroll
+                for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ai[p], br[q], cim[p][q]);
+        }
+        if (s + 1 < ns) store(cur ^ 1);
+        __syncthreads();
+    }
+    C* out = (C*)J.out;
+#pragma unroll
+    for (int p = 0; p < BM; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + wr * WM + p * 16 + M::row(lane, v);
+            if (i >= n) continue;
+#pragma unroll
+            for (int q = 0; q < BN; ++q) {
+                const int j = j0 + wc * WN + q * 16 + lr;
+                if (j >= n) continue;
+                C* o = out + (long)i * n + j;
+                R ox = R(0), oy = R(0);
+                if (J.acc) {
+                    const C old = *o;
+                    ox = old.x;
+                    oy = old.y;
+                }
+                const R re = cre[p][q][v], im = cim[p][q][v];
+                *o = J.sub ? mk<C>(ox - re, oy - im) : mk<C>(ox + re, oy + im);
+            }
+        }
+}
+
+template <typename R>
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n) {
+    constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
+    k_gemm<R><<<dim3((n + TN - 1) / TN, (n + TM - 1) / TM, njobs), 256, 0, s>>>(jobs_d, n);
+}
+template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int);
+template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int);
+
+// ---------------------------------------------------------------------------
+// Batched dense matrix-vector step of the reduced solve.  The gathered input
+// vectors sit in LDS; each wave computes two rows at a time, four columns
+// per lane per trip (eight loads in flight per lane).
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ inline void mv_rows(const cplx<R>* ya, const cplx<R>* yb, const cplx<R>* w, int n,
+                               int lane, R& ar, R& ai, R& br, R& bi) {
+    using C = cplx<R>;
+    int c = lane;
+    for (; c + 192 < n; c += 256) {
+        const C a0 = ya[c], a1 = ya[c + 64], a2 = ya[c + 128], a3 = ya[c + 192];
+        const C b0 = yb[c], b1 = yb[c + 64], b2 = yb[c + 128], b3 = yb[c + 192];
+        const C x0 = w[c], x1 = w[c + 64], x2 = w[c + 128], x3 = w[c + 192];
+        ar += a0.x * x0.x - a0.y * x0.y + a1.x * x1.x - a1.y * x1.y + a2.x * x2.x -
+              a2.y * x2.y + a3.x * x3.x - a3.y * x3.y;
+        ai += a0.x * x0.y + a0.y * x0.x + a1.x * x1.y + a1.y * x1.x + a2.x * x2.y + a2.y * x2.x +
+              a3.x * x3.y + a3.y * x3.x;
+        br += b0.x * x0.x - b0.y * x0.y + b1.x * x1.x - b1.y * x1.y + b2.x * x2.x -
+              b2.y * x2.y + b3.x * x3.x - b3.y * x3.y;
+        bi += b0.x * x0.y + b0.y * x0.x + b1.x * x1.y + b1.y * x1.x + b2.x * x2.y + b2.y * x2.x +
+              b3.x * x3.y + b3.y * x3.x;
+    }
+    for (; c < n; c += 64) {
+        const C a = ya[c], b = yb[c], x = w[c];
+        ar += a.x * x.x - a.y * x.y;
+        ai += a.x * x.y + a.y * x.x;
+        br += b.x * x.x - b.y * x.y;
+        bi += b.x * x.y + b.y * x.x;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_mv(const MvJob* jobs, int n) {
+    using C = cplx<R>;
+    const MvJob J = jobs[blockIdx.y];
+    __shared__ C w0[MAX_LINE];
+    __shared__ C w1[MAX_LINE];
+    const long si = J.si;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const int s0 = J.imap[0] ? J.imap[0][c] : c;
+        w0[c] = from_d<C>(J.x[0][(long)s0 * si]);
+        if (J.nt > 1) {
+            const int s1 = J.imap[1] ? J.imap[1][c] : c;
+            w1[c] = from_d<C>(J.x[1][(long)s1 * si]);
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rows_per_block = (n + gridDim.x - 1) / gridDim.x;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = min(n, r0 + rows_per_block);
+    for (int r = r0 + 2 * wv; r < r1; r += 8) {
+        const bool two = (r + 1) < r1;
+        const int rb = two ? r + 1 : r;
+        R ar = 0, ai = 0, br = 0, bi = 0;
+        {
+            const C* Mt = (const C*)J.M[0];
+            mv_rows<R>(Mt + (long)r * n, Mt + (long)rb * n, w0, n, lane, ar, ai, br, bi);
+        }
+        if (J.nt > 1) {
+            const C* Mt = (const C*)J.M[1];
+            mv_rows<R>(Mt + (long)r * n, Mt + (long)rb * n, w1, n, lane, ar, ai, br, bi);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ar += shfl_x(ar, o);
+            ai += shfl_x(ai, o);
+            br += shfl_x(br, o);
+            bi += shfl_x(bi, o);
+        }
+        if (lane == 0) {
+            long e = (long)(J.omap ? J.omap[r] : r) * si;
+            double2 z = make_double2((double)ar, (double)ai);
+            J.out[e] = J.base ? csub(J.base[e], z) : z;
+            if (two) {
+                e = (long)(J.omap ? J.omap[r + 1] : r + 1) * si;
+                z = make_double2((double)br, (double)bi);
+                J.out[e] = J.base ? csub(J.base[e], z) : z;
+            }
+        }
+    }
+}
+
+template <typename R>
+void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n) {
+    // enough workgroups to cover the chip, at least 8 rows per workgroup
+    int nblk = (4096 + njobs - 1) / njobs;
+    nblk = std::max(1, std::min(nblk, (n + 7) / 8));
+    k_mv<R><<<dim3(nblk, njobs), 256, 0, s>>>(jobs_d, n);
+}
+template void launch_mv<double>(hipStream_t, const MvJob*, int, int);
+template void launch_mv<float>(hipStream_t, const MvJob*, int, int);
+
+// ---------------------------------------------------------------------------
+// First-level (diagonal) step of the reduced solve.
+// ---------------------------------------------------------------------------
+__global__ void k_diag(const DiagJob* jobs, int n) {
+    const DiagJob J = jobs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long e = (long)i * J.si;
+    double2 v = J.x0[e];
+    if (J.nt > 0) {
+        double2 d = J.d[0][0][e];
+        if (J.d[0][1]) d = cadd(d, J.d[0][1][e]);
+        v = csub(v, cmul(d, J.x[0][e]));
+    }
+    if (J.nt > 1) {
+        double2 d = J.d[1][0][e];
+        if (J.d[1][1]) d = cadd(d, J.d[1][1][e]);
+        v = csub(v, cmul(d, J.x[1][e]));
+    }
+    J.out[e] = v;
+}
+
+void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n) {
+    k_diag<<<dim3((n + 255) / 256, njobs), 256, 0, s>>>(jobs_d, n);
+}
+
+// ---------------------------------------------------------------------------
+// out = w .* in (or in ./ w), w[ix + iy Nx] = sxf[ix] syf[iy].  With the
+// periodic differences A'^T = W^-1 A' W, so a transposed solve is a plain
+// solve between two such scalings.
+// ---------------------------------------------------------------------------
+__global__ void k_wscale(double2* out, const double2* in, const double2* sxf, const double2* syf,
+                         int Nx, int Ny, int inverse) {
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= (long)Nx * Ny) return;
+    const int ix = (int)(k % Nx), iy = (int)(k / Nx);
+    const double2 w = cmul(sxf[ix], syf[iy]);
+    out[k] = inverse ? cdiv(in[k], w) : cmul(in[k], w);
+}
+
+void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2* sxf,
+                   const double2* syf, int Nx, int Ny, int inverse) {
+    const long N = (long)Nx * Ny;
+    k_wscale<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(out, in, sxf, syf, Nx, Ny, inverse);
+}
+
+}  // namespace fdfd

@@ -1,89 +1,26 @@
-// Block-tridiagonal factorization kernels of libfdfd (gfx950).
+// Batched line-block inversion kernels of libfdfd (gfx950).
 //
-// Each grid line is a dense n x n block.  Its Schur complement S_j is formed
-// from the neighbouring line's factor (k_schur) and inverted in place by
-// blocked Gauss-Jordan elimination with partial pivoting.  Pivot rows are not
-// swapped; they are tracked by index.  For a panel of nb columns K with
-// original values V = A[:, K] and pivot rows p:
+// Each job is a dense n x n block (one grid line of a reduction level),
+// inverted in place by blocked Gauss-Jordan elimination with partial
+// pivoting.  Pivot rows are not swapped; they are tracked by index.  For a
+// panel of nb columns K with original values V = A[:, K] and pivot rows p:
 //   * k_panel    : GEPP pivot selection on the not-yet-pivoted rows of V
 //                  (one workgroup, one row per thread in registers), then the
 //                  inverse Vpinv of the pivot block V[p, :] = L U;
 //   * k_zbuild   : Z = Vpinv * [A[p, C] | I_K]              (nb x n);
 //   * k_trailing : A[i, :] = A'[i, :] - V[i, :] Z  (i not in p), A[p, :] = Z,
 //                  a rank-nb complex update on MFMA (A' = A with K zeroed).
-// After the last panel the stored Y satisfies S'^-1[a, b] = Y[prow[a],
-// iprow[b]]; k_maps folds the row map of S' into the column map kap so that
-// X_j = S_j^-1 is read as X[a, b] = Y[rho[a], kap[b]].
-// The sweeps (k_sweep) apply X_j or X_j^T to a vector with that indexing.
+// After the last panel the stored Y holds the inverse X = D^-1 as
+// X[a, b] = Y[prow[a], iprow[b]].
 #include "fdfd_common.h"
+#include "k_device.h"
 
 #include <algorithm>
 #include <type_traits>
 
 namespace fdfd {
 
-typedef double d4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 
-template <typename R> __device__ inline R shfl_x(R v, int o) { return __shfl_xor(v, o, 64); }
-
-// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
-template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Schur formation: one block per row r of S'.
-// ---------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_schur(const SchurJob* jobs, int n) {
-    using C = cplx<R>;
-    const SchurJob J = jobs[blockIdx.y];
-    __shared__ C row1[MAX_LINE];
-    __shared__ C row2[MAX_LINE];
-    const int r = blockIdx.x;
-    const int a = J.Y1 ? J.rhoinv1[r] : r;  // q(r)
-    if (threadIdx.x == 0) {
-        J.q[r] = a;
-        J.used[r] = 0;
-        J.iprow[r] = -1;
-        if (r < WIN) J.win[r] = r;
-    }
-    if (J.Y1) {
-        const C* y1 = (const C*)J.Y1 + (long)r * n;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) row1[c] = y1[c];
-    }
-    if (J.Y2) {
-        const C* y2 = (const C*)J.Y2 + (long)J.rho2[a] * n;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) row2[c] = y2[c];
-    }
-    __syncthreads();
-    const long si = J.si;
-    const double2 dg = J.ddiag[a * si], up = J.dsup[a * si], dn = J.dsub[a * si];
-    const int ap = (a + 1 == n) ? 0 : a + 1, am = (a == 0) ? n - 1 : a - 1;
-    const double2 la1 = J.Y1 ? J.l1[a * si] : make_double2(0, 0);
-    const double2 la2 = J.Y2 ? J.l2[a * si] : make_double2(0, 0);
-    C* S = (C*)J.S + (long)r * n;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) {
-        double2 v = make_double2(0, 0);
-        if (b == a) v = cadd(v, dg);
-        if (b == ap) v = cadd(v, up);
-        if (b == am) v = cadd(v, dn);
-        if (J.Y1) v = csub(v, cmul(cmul(la1, J.u1[b * si]), to_d(row1[J.kap1[b]])));
-        if (J.Y2) v = csub(v, cmul(cmul(la2, J.u2[b * si]), to_d(row2[J.kap2[b]])));
-        S[b] = from_d<C>(v);
-    }
-}
-
-template <typename R>
-void launch_schur(hipStream_t s, const SchurJob* jobs_d, int njobs, int n) {
-    k_schur<R><<<dim3(n, njobs), 256, 0, s>>>(jobs_d, n);
-}
-template void launch_schur<double>(hipStream_t, const SchurJob*, int, int);
-template void launch_schur<float>(hipStream_t, const SchurJob*, int, int);
 
 // ---------------------------------------------------------------------------
 // Broadcast of a lane's value to the whole wave (lane index wave-uniform).
@@ -137,9 +74,11 @@ template <typename C> __device__ inline void put_lu_row(C* L, C* U, int cc, int 
 
 // ---------------------------------------------------------------------------
 // Fast panel (one wave): GEPP on the candidate window, the first WIN
-// unpivoted rows by index.  Lane l owns window row win[l].  The pivot
-// sequence equals GEPP's on all unpivoted rows whenever no row outside the
-// window would have won a step; k_zbuild checks exactly that (verr).
+// unpivoted rows by index.  Lane l owns window row win[l].  Every chosen
+// pivot is then checked against all rows outside the window (k_zbuild):
+// threshold partial pivoting, |pivot| >= PIV_U * max |candidate| over every
+// unpivoted row, or verr is raised and the factorization is redone with the
+// full-column search.
 // Outputs: pivots, L and U of V[piv, :], updated window list.
 // ---------------------------------------------------------------------------
 template <typename R, int NB>
@@ -360,7 +299,8 @@ template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, in
 //   blocks [nzb, ..): copy V = A[:, K] to Vbuf (every row) and, if verify,
 //                     check rows r >= fresh: the value GEPP would compare at
 //                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
-//                     y_c = a_c / U[c][c], must not exceed |U[c][c]|.
+//                     y_c = a_c / U[c][c], must satisfy
+//                     PIV_U * |a_c| <= |U[c][c]| (threshold pivoting).
 // ---------------------------------------------------------------------------
 template <typename R>
 __device__ inline cplx<R> bcast_c(cplx<R> v, int l) {
@@ -424,7 +364,7 @@ __global__ __launch_bounds__(512) void k_zbuild(const LineJob* jobs, int k0, int
                 // the value GEPP compares at step k: a_k = V[r][k] - sum y_c U[c][k]
                 for (int k = 0; k < nb; ++k) {
                     const C ak = bcast_c<R>(v, k);  // wave-uniform
-                    if (cabs1(ak) > cabs1(Us[k][k])) bad = 1;
+                    if (R(PIV_U) * cabs1(ak) > cabs1(Us[k][k])) bad = 1;
                     const C yk = cmul(ak, Ud[k]);
                     if (il && i > k) v = csub(v, cmul(yk, Us[k][i]));
                 }
@@ -454,23 +394,6 @@ template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, i
 // The old tile values are loaded before the MFMA loop so their latency
 // overlaps the operand staging.
 // ---------------------------------------------------------------------------
-template <typename R> struct mfma_t;
-// Output row of result element v held by a lane: the f32 form keeps four
-// consecutive rows per lane group, the f64 form interleaves them.
-template <> struct mfma_t<double> {
-    using acc = d4v;
-    static __device__ inline acc run(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-template <> struct mfma_t<float> {
-    using acc = f4v;
-    static __device__ inline acc run(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, int nb) {
@@ -575,162 +498,5 @@ void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int
 template void launch_trailing<double>(hipStream_t, const LineJob*, int, int, int, int);
 template void launch_trailing<float>(hipStream_t, const LineJob*, int, int, int, int);
 
-// ---------------------------------------------------------------------------
-// kapinv[c] = q[prow[c]], kap[kapinv[c]] = c
-// ---------------------------------------------------------------------------
-__global__ void k_maps(const LineJob* jobs) {
-    const LineJob J = jobs[blockIdx.y];
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= J.n) return;
-    int b = J.q[J.prow[c]];
-    J.kapinv[c] = b;
-    J.kap[b] = c;
-}
-void launch_maps(hipStream_t s, const LineJob* jobs_d, int njobs, int n) {
-    k_maps<<<dim3((n + 255) / 256, njobs), 256, 0, s>>>(jobs_d);
-}
-
-// ---------------------------------------------------------------------------
-// Sweep step.  Non-transposed: w[c] = v[kapinv[c]], z = Y w,
-// out[rhoinv[r]] = z[r] (forward/middle) or g - z[r] (backward).
-// Transposed: w[r] = v[rhoinv[r]], z = Y^T w, out[kapinv[c]] = z[c] (or g - z).
-//   forward : v = b - c1 .* nb1           (nb1 may be null)
-//   middle  : v = b - c1 .* nb1 - c2 .* nb2
-//   backward: v = c1 .* nb1
-// ---------------------------------------------------------------------------
-template <typename R>
-__device__ inline void build_w(const SweepJob& J, const int* map, cplx<R>* w, int n) {
-    using C = cplx<R>;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const long ea = (long)map[c] * J.si;
-        double2 v;
-        if (J.mode == 1) {
-            v = cmul(J.c1[ea], J.nb1[ea]);
-        } else {
-            v = J.b[ea];
-            if (J.nb1) v = csub(v, cmul(J.c1[ea], J.nb1[ea]));
-            if (J.mode == 2 && J.nb2) v = csub(v, cmul(J.c2[ea], J.nb2[ea]));
-        }
-        w[c] = from_d<C>(v);
-    }
-}
-
-template <typename R>
-__global__ __launch_bounds__(256) void k_sweep(const SweepJob* jobs, int n) {
-    using C = cplx<R>;
-    const SweepJob J = jobs[blockIdx.y];
-    __shared__ C w[MAX_LINE];
-    build_w<R>(J, J.kapinv, w, n);
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int rows_per_block = (n + gridDim.x - 1) / gridDim.x;
-    const int r0 = blockIdx.x * rows_per_block;
-    const int r1 = min(n, r0 + rows_per_block);
-    const C* Y = (const C*)J.Y;
-    // two rows per wave at a time, four columns per lane per trip: eight
-    // independent loads in flight per lane
-    for (int r = r0 + 2 * wv; r < r1; r += 8) {
-        const bool two = (r + 1) < r1;
-        const C* ya = Y + (long)r * n;
-        const C* yb = Y + (long)(two ? r + 1 : r) * n;
-        R ar = 0, ai = 0, br = 0, bi = 0;
-        int c = lane;
-        for (; c + 192 < n; c += 256) {
-            C a0 = ya[c], a1 = ya[c + 64], a2 = ya[c + 128], a3 = ya[c + 192];
-            C b0 = yb[c], b1 = yb[c + 64], b2 = yb[c + 128], b3 = yb[c + 192];
-            C x0 = w[c], x1 = w[c + 64], x2 = w[c + 128], x3 = w[c + 192];
-            ar += a0.x * x0.x - a0.y * x0.y + a1.x * x1.x - a1.y * x1.y + a2.x * x2.x -
-                  a2.y * x2.y + a3.x * x3.x - a3.y * x3.y;
-            ai += a0.x * x0.y + a0.y * x0.x + a1.x * x1.y + a1.y * x1.x + a2.x * x2.y +
-                  a2.y * x2.x + a3.x * x3.y + a3.y * x3.x;
-            br += b0.x * x0.x - b0.y * x0.y + b1.x * x1.x - b1.y * x1.y + b2.x * x2.x -
-                  b2.y * x2.y + b3.x * x3.x - b3.y * x3.y;
-            bi += b0.x * x0.y + b0.y * x0.x + b1.x * x1.y + b1.y * x1.x + b2.x * x2.y +
-                  b2.y * x2.x + b3.x * x3.y + b3.y * x3.x;
-        }
-        for (; c < n; c += 64) {
-            C a = ya[c], b = yb[c], x = w[c];
-            ar += a.x * x.x - a.y * x.y;
-            ai += a.x * x.y + a.y * x.x;
-            br += b.x * x.x - b.y * x.y;
-            bi += b.x * x.y + b.y * x.x;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            ar += shfl_x(ar, o);
-            ai += shfl_x(ai, o);
-            br += shfl_x(br, o);
-            bi += shfl_x(bi, o);
-        }
-        if (lane == 0) {
-            long ea = (long)J.rhoinv[r] * J.si;
-            double2 z = make_double2((double)ar, (double)ai);
-            J.out[ea] = (J.mode == 1) ? csub(J.g[ea], z) : z;
-            if (two) {
-                ea = (long)J.rhoinv[r + 1] * J.si;
-                z = make_double2((double)br, (double)bi);
-                J.out[ea] = (J.mode == 1) ? csub(J.g[ea], z) : z;
-            }
-        }
-    }
-}
-
-// transposed: blocks (column group of 64) x (row split) write partial sums;
-// k_sweep_t_finish adds the TSPLIT partials of each column in a fixed order.
-constexpr int TSPLIT = 16;
-template <typename R>
-__global__ __launch_bounds__(256) void k_sweep_t(const SweepJob* jobs, int n, double2* partial) {
-    using C = cplx<R>;
-    const int jid = blockIdx.z;
-    const SweepJob J = jobs[jid];
-    __shared__ C w[MAX_LINE];
-    __shared__ double2 red[4][64];
-    build_w<R>(J, J.rhoinv, w, n);
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + lane;
-    const int rows = (n + TSPLIT - 1) / TSPLIT;
-    const int r0 = blockIdx.y * rows, r1 = min(n, r0 + rows);
-    const C* Y = (const C*)J.Y;
-    R sr = 0, sim = 0;
-    if (c < n) {
-        for (int r = r0 + wv; r < r1; r += 4) {
-            C y = Y[(long)r * n + c], x = w[r];
-            sr += y.x * x.x - y.y * x.y;
-            sim += y.x * x.y + y.y * x.x;
-        }
-    }
-    red[wv][lane] = make_double2((double)sr, (double)sim);
-    __syncthreads();
-    if (wv == 0 && c < n) {
-        double2 s = red[0][lane];
-        for (int q = 1; q < 4; ++q) s = cadd(s, red[q][lane]);
-        partial[((long)jid * TSPLIT + blockIdx.y) * MAX_LINE + c] = s;
-    }
-}
-
-__global__ void k_sweep_t_finish(const SweepJob* jobs, int n, const double2* partial) {
-    const int jid = blockIdx.y;
-    const SweepJob J = jobs[jid];
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    double2 s = make_double2(0, 0);
-    for (int q = 0; q < TSPLIT; ++q) s = cadd(s, partial[((long)jid * TSPLIT + q) * MAX_LINE + c]);
-    const long ea = (long)J.kapinv[c] * J.si;
-    J.out[ea] = (J.mode == 1) ? csub(J.g[ea], s) : s;
-}
-
-template <typename R>
-void launch_sweep(hipStream_t s, const SweepJob* jobs_d, int njobs, int n, int transpose,
-                  double2* partial) {
-    if (!transpose) {
-        int nblk = std::max(1, std::min(256 / std::max(1, njobs), (n + 3) / 4));
-        k_sweep<R><<<dim3(nblk, njobs), 256, 0, s>>>(jobs_d, n);
-    } else {
-        k_sweep_t<R><<<dim3((n + 63) / 64, TSPLIT, njobs), 256, 0, s>>>(jobs_d, n, partial);
-        k_sweep_t_finish<<<dim3((n + 255) / 256, njobs), 256, 0, s>>>(jobs_d, n, partial);
-    }
-}
-template void launch_sweep<double>(hipStream_t, const SweepJob*, int, int, int, double2*);
-template void launch_sweep<float>(hipStream_t, const SweepJob*, int, int, int, double2*);
 
 }  // namespace fdfd

@@ -1,7 +1,7 @@
 // Host runtime and C ABI of libfdfd: handle lifecycle, operator assembly,
-// the twisted block-Thomas factor driver, block sweeps, FP64 flexible GMRES
-// refinement, field recovery, adjoint gradient.  Shift-invert Arnoldi is in
-// eigen.cpp, batched solves in batch.cpp.
+// FP64 flexible GMRES refinement, field recovery, adjoint gradient.  The
+// block cyclic reduction factorization and its solve are in bcr.cpp,
+// shift-invert Arnoldi in eigen.cpp, batched solves in batch.cpp.
 //
 // Reference path replaced: solve(d::Device, pol) (src/solver/driven.jl:4-59)
 // with dolinearsolve -> lu(A)\b (src/solver/solver.jl:4-41).
@@ -23,13 +23,6 @@ int fail(Handle* h, int code, const std::string& msg) {
     return code;
 }
 
-#define HIPCHK(h, call)                                                              \
-    do {                                                                             \
-        hipError_t e_ = (call);                                                      \
-        if (e_ != hipSuccess)                                                        \
-            return fail((h), e_ == hipErrorOutOfMemory ? FDFD_ERR_OOM : FDFD_ERR_HIP, \
-                        std::string(#call) + ": " + hipGetErrorString(e_));          \
-    } while (0)
 
 int stream_copy(Handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
     HIPCHK(h, hipMemcpyAsync(dst, src, bytes, kind, h->stream));
@@ -37,16 +30,11 @@ int stream_copy(Handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKi
     return FDFD_OK;
 }
 
-#define COPY(h, dst, src, bytes, kind)                         \
-    do {                                                       \
-        int rc_ = stream_copy((h), (dst), (src), (bytes), (kind)); \
-        if (rc_) return rc_;                                   \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // profiling: HIP events around launches on the handle's stream
 // ---------------------------------------------------------------------------
-static hipEvent_t ev_get(Prof& p) {
+hipEvent_t ev_get(Prof& p) {
     if (!p.pool.empty()) {
         hipEvent_t e = p.pool.back();
         p.pool.pop_back();
@@ -56,7 +44,7 @@ static hipEvent_t ev_get(Prof& p) {
     (void)hipEventCreate(&e);
     return e;
 }
-static void prof_resolve(Prof& p) {
+void prof_resolve(Prof& p) {
     for (auto& q : p.pending) {
         (void)hipEventSynchronize(q.b);
         float ms = 0;
@@ -70,24 +58,6 @@ static void prof_resolve(Prof& p) {
     }
     p.pending.clear();
 }
-struct ProfScope {
-    Handle* h; int id; double flops, bytes; hipStream_t st; hipEvent_t a{}, b{};
-    ProfScope(Handle* h_, int id_, double f, double by, hipStream_t s = nullptr)
-        : h(h_), id(id_), flops(f), bytes(by), st(s ? s : h_->stream) {
-        if (h->prof.enabled) {
-            a = ev_get(h->prof);
-            (void)hipEventRecord(a, st);
-        }
-    }
-    ~ProfScope() {
-        if (h->prof.enabled) {
-            b = ev_get(h->prof);
-            (void)hipEventRecord(b, st);
-            h->prof.pending.push_back({a, b, id, flops, bytes});
-            if (h->prof.pending.size() > 4096) prof_resolve(h->prof);
-        }
-    }
-};
 
 // ---------------------------------------------------------------------------
 // work buffers
@@ -96,27 +66,16 @@ static void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-static void clear_graphs(Handle* h) {
-    for (auto& g : h->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        dfree(g.lj_d);
-        dfree(g.sj_d);
-    }
-    h->graphs.clear();
-}
-
-static int ensure_factor_work(Handle* h, int n, int nch) {
+int ensure_factor_work(Handle* h, int n, int nch) {
     if (h->work_n >= n && h->work_ch >= nch) return FDFD_OK;
     n = std::max(n, h->work_n);
     nch = std::max(nch, h->work_ch);
-    clear_graphs(h);
-    dfree(h->used); dfree(h->q); dfree(h->piv); dfree(h->Vbuf); dfree(h->LU); dfree(h->Zb);
+    dfree(h->used); dfree(h->piv); dfree(h->Vbuf); dfree(h->LU); dfree(h->Zb);
     dfree(h->win); dfree(h->verr);
-    h->used = h->q = h->piv = h->win = h->verr = nullptr;
+    h->used = h->piv = h->win = h->verr = nullptr;
     h->Vbuf = h->LU = h->Zb = nullptr;
     const int NBmax = PMAX;
     HIPCHK(h, hipMalloc(&h->used, sizeof(int) * nch * n));
-    HIPCHK(h, hipMalloc(&h->q, sizeof(int) * nch * n));
     HIPCHK(h, hipMalloc(&h->piv, sizeof(int) * nch * NBmax));
     HIPCHK(h, hipMalloc(&h->win, sizeof(int) * nch * WIN));
     HIPCHK(h, hipMalloc(&h->verr, sizeof(int)));
@@ -130,13 +89,12 @@ static int ensure_factor_work(Handle* h, int n, int nch) {
 
 static const int KDIM = 30;
 
-static int ensure_solve_work(Handle* h, long N) {
+int ensure_solve_work(Handle* h, long N) {
     if (h->work_N >= N) return FDFD_OK;
-    void** bufs[] = {(void**)&h->r_in, (void**)&h->g_mid, (void**)&h->x_out, (void**)&h->V,
+    void** bufs[] = {(void**)&h->g_mid, (void**)&h->x_out, (void**)&h->V,
                      (void**)&h->Zk, (void**)&h->tmp, (void**)&h->tmp2, (void**)&h->bvec,
                      (void**)&h->xvec};
     for (auto b : bufs) { dfree(*b); *b = nullptr; }
-    HIPCHK(h, hipMalloc(&h->r_in, sizeof(double2) * N));
     HIPCHK(h, hipMalloc(&h->g_mid, sizeof(double2) * N));
     HIPCHK(h, hipMalloc(&h->x_out, sizeof(double2) * N));
     HIPCHK(h, hipMalloc(&h->V, sizeof(double2) * N * (KDIM + 1)));
@@ -150,7 +108,6 @@ static int ensure_solve_work(Handle* h, long N) {
         HIPCHK(h, hipMalloc(&h->dpart, sizeof(double2) * h->nblk * (KDIM + 2)));
         HIPCHK(h, hipMalloc(&h->rpart, sizeof(double) * h->nblk));
         HIPCHK(h, hipMalloc(&h->hdev, sizeof(double2) * (KDIM + 2)));
-        HIPCHK(h, hipMalloc(&h->tpart, sizeof(double2) * 2 * 16 * MAX_LINE));
     }
     h->work_N = N;
     h->kdim = KDIM;
@@ -160,17 +117,16 @@ static int ensure_solve_work(Handle* h, long N) {
 static void free_factor(Factor* F) {
     if (!F) return;
     dfree(F->coef); dfree(F->sxf); dfree(F->sxb); dfree(F->syf); dfree(F->syb);
-    dfree(F->Y); dfree(F->rho); dfree(F->rhoinv); dfree(F->kap); dfree(F->kapinv);
-    dfree(F->pivstat); dfree(F->sweep_tab[0]); dfree(F->sweep_tab[1]);
-    for (auto& e : F->sweep_exec)
-        if (e) (void)hipGraphExecDestroy(e);
+    dfree(F->blocks); dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
+    dfree(F->mv_tab); dfree(F->dg_tab);
+    if (F->solve_exec) (void)hipGraphExecDestroy(F->solve_exec);
     delete F;
 }
 
 // ---------------------------------------------------------------------------
 // operator assembly
 // ---------------------------------------------------------------------------
-static int assemble(Handle* h, Factor* F) {
+int rt_assemble(Handle* h, Factor* F) {
     const GridDesc& g = h->g;
     const long N = (long)g.Nx * g.Ny;
     const double eps0n = EPS0 * g.L0, mu0n = MU0 * g.L0;
@@ -198,489 +154,6 @@ static int assemble(Handle* h, Factor* F) {
     return FDFD_OK;
 }
 
-// coefficient plane pointers for the line orientation
-struct Planes { const double2 *L, *U, *sub, *sup, *diag; };
-static Planes planes(const Factor* F, long N) {
-    const double2* c = F->coef;
-    if (F->orient == 0) return {c + CS * N, c + CN * N, c + CW * N, c + CE * N, c + CC * N};
-    return {c + CW * N, c + CE * N, c + CS * N, c + CN * N, c + CC * N};
-}
-
-static void* yline(const Factor* F, int j) {
-    return (char*)F->Y + (size_t)j * F->n * F->n * F->elem;
-}
-
-// ---------------------------------------------------------------------------
-// sweep tables (see the twisted block-Thomas recurrences in build_factor)
-// ---------------------------------------------------------------------------
-static int build_sweep_tables(Handle* h, Factor* F) {
-    const long N = (long)h->g.Nx * h->g.Ny;
-    const int m = F->m, n = F->n, k = F->k;
-    const int nsteps = std::max(k, m - 1 - k);
-    Planes P = planes(F, N);
-    const long sj = F->sj;
-    auto base = [&](const int j, const int trans) {
-        SweepJob J{};
-        J.Y = yline(F, j);
-        J.rho = F->rho + (long)j * n; J.rhoinv = F->rhoinv + (long)j * n;
-        J.kap = F->kap + (long)j * n; J.kapinv = F->kapinv + (long)j * n;
-        J.si = F->si; J.n = n;
-        (void)trans;
-        return J;
-    };
-    F->nfwd = nsteps;
-    F->nbwd = nsteps;
-    F->fwd_jobs.assign(nsteps, 0);
-    F->bwd_jobs.assign(nsteps, 0);
-    F->fwd_has_a.assign(nsteps, 0);
-    F->bwd_has_a.assign(nsteps, 0);
-    for (int tr = 0; tr < 2; ++tr) {
-        std::vector<SweepJob> tab((size_t)2 * (2 * nsteps + 1));
-        for (int s = 0; s < nsteps; ++s) {
-            int nj = 0;
-            if (s < k) {  // forward A, line j = s
-                int j = s;
-                SweepJob J = base(j, tr);
-                J.mode = 0;
-                J.b = h->r_in + j * sj;
-                if (j > 0) {
-                    J.nb1 = h->g_mid + (j - 1) * sj;
-                    J.c1 = tr ? P.U + (j - 1) * sj : P.L + j * sj;
-                }
-                J.out = h->g_mid + j * sj;
-                tab[2 * s + nj++] = J;
-                F->fwd_has_a[s] = 1;
-            }
-            if (s < m - 1 - k) {  // forward B, line j = m-1-s
-                int j = m - 1 - s;
-                SweepJob J = base(j, tr);
-                J.mode = 0;
-                J.b = h->r_in + j * sj;
-                if (j < m - 1) {
-                    J.nb1 = h->g_mid + (j + 1) * sj;
-                    J.c1 = tr ? P.L + (j + 1) * sj : P.U + j * sj;
-                }
-                J.out = h->g_mid + j * sj;
-                tab[2 * s + nj++] = J;
-            }
-            F->fwd_jobs[s] = nj;
-        }
-        {  // middle line k
-            SweepJob J = base(k, tr);
-            J.mode = 2;
-            J.b = h->r_in + k * sj;
-            if (k >= 1) {
-                J.nb1 = h->g_mid + (k - 1) * sj;
-                J.c1 = tr ? P.U + (k - 1) * sj : P.L + k * sj;
-            }
-            if (k + 1 <= m - 1) {
-                J.nb2 = h->g_mid + (k + 1) * sj;
-                J.c2 = tr ? P.L + (k + 1) * sj : P.U + k * sj;
-            }
-            J.out = h->x_out + k * sj;
-            tab[2 * nsteps] = J;
-        }
-        for (int s = 0; s < nsteps; ++s) {
-            int nj = 0;
-            int ja = k - 1 - s;
-            if (ja >= 0) {  // backward A
-                SweepJob J = base(ja, tr);
-                J.mode = 1;
-                J.nb1 = h->x_out + (ja + 1) * sj;
-                J.c1 = tr ? P.L + (ja + 1) * sj : P.U + ja * sj;
-                J.g = h->g_mid + ja * sj;
-                J.out = h->x_out + ja * sj;
-                tab[2 * (nsteps + 1 + s) + nj++] = J;
-                F->bwd_has_a[s] = 1;
-            }
-            int jb = k + 1 + s;
-            if (jb <= m - 1) {  // backward B
-                SweepJob J = base(jb, tr);
-                J.mode = 1;
-                J.nb1 = h->x_out + (jb - 1) * sj;
-                J.c1 = tr ? P.U + (jb - 1) * sj : P.L + jb * sj;
-                J.g = h->g_mid + jb * sj;
-                J.out = h->x_out + jb * sj;
-                tab[2 * (nsteps + 1 + s) + nj++] = J;
-            }
-            F->bwd_jobs[s] = nj;
-        }
-        HIPCHK(h, hipMalloc(&F->sweep_tab[tr], sizeof(SweepJob) * tab.size()));
-        COPY(h, F->sweep_tab[tr], tab.data(), sizeof(SweepJob) * tab.size(),
-             hipMemcpyHostToDevice);
-    }
-    return FDFD_OK;
-}
-
-// out = M^-1 in (transpose: M^-T in), M = block-tridiagonal part of A' without
-// the periodic corner blocks between the first and last line.
-static void launch_sweeps(Handle* h, Factor* F, int trans, bool prof) {
-    const int n = F->n;
-    const SweepJob* tab = F->sweep_tab[trans];
-    const int nsteps = F->nfwd;
-    const double bpl = (double)n * n * F->elem;  // factor bytes per line
-    auto run = [&](hipStream_t st, const SweepJob* t, double2* part) {
-        ProfScope* ps = prof ? new ProfScope(h, P_SWEEP, 8.0 * n * n, bpl, st) : nullptr;
-        if (F->precision == FDFD_FP64) launch_sweep<double>(st, t, 1, n, trans, part);
-        else launch_sweep<float>(st, t, 1, n, trans, part);
-        delete ps;
-    };
-    // transposed sweeps keep their partial sums in one buffer slot per stream
-    double2* partB = h->tpart + (size_t)16 * MAX_LINE;
-    auto fork = [&]() {
-        (void)hipEventRecord(h->ev_fork, h->stream);
-        (void)hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-    };
-    auto join = [&]() {
-        (void)hipEventRecord(h->ev_join, h->stream2);
-        (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
-    };
-    // chain A (listed first in a step) on the handle's stream, chain B on the
-    // second stream; the middle line needs both forward sweeps
-    fork();
-    for (int s = 0; s < nsteps; ++s) {
-        const SweepJob* t = tab + 2 * s;
-        if (F->fwd_has_a[s]) run(h->stream, t, h->tpart);
-        if (F->fwd_jobs[s] > F->fwd_has_a[s]) run(h->stream2, t + F->fwd_has_a[s], partB);
-    }
-    join();
-    run(h->stream, tab + 2 * nsteps, h->tpart);
-    fork();
-    for (int s = 0; s < nsteps; ++s) {
-        const SweepJob* t = tab + 2 * (nsteps + 1 + s);
-        if (F->bwd_has_a[s]) run(h->stream, t, h->tpart);
-        if (F->bwd_jobs[s] > F->bwd_has_a[s]) run(h->stream2, t + F->bwd_has_a[s], partB);
-    }
-    join();
-}
-
-// out = M^-1 in (transpose: M^-T in), M = block-tridiagonal part of A' without
-// the periodic corner blocks between the first and last line.  The sweep
-// launch sequence of a factor is captured once in a HIP graph and replayed
-// (direct launches while per-kernel profiling is on).
-static int prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out) {
-    const long N = (long)h->g.Nx * h->g.Ny;
-    HIPCHK(h, hipMemcpyAsync(h->r_in, in, sizeof(double2) * N, hipMemcpyDeviceToDevice,
-                             h->stream));
-    if (h->prof.enabled || !h->use_graphs) {
-        launch_sweeps(h, F, trans, h->prof.enabled);
-    } else {
-        if (!F->sweep_exec[trans]) {
-            hipGraph_t gr;
-            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            launch_sweeps(h, F, trans, false);
-            HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
-            hipError_t e = hipGraphInstantiate(&F->sweep_exec[trans], gr, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(gr);
-            if (e != hipSuccess) {
-                F->sweep_exec[trans] = nullptr;
-                return fail(h, FDFD_ERR_HIP, std::string("sweep graph: ") + hipGetErrorString(e));
-            }
-        }
-        HIPCHK(h, hipGraphLaunch(F->sweep_exec[trans], h->stream));
-    }
-    HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipMemcpyAsync(out, h->x_out, sizeof(double2) * N, hipMemcpyDeviceToDevice,
-                             h->stream));
-    return FDFD_OK;
-}
-
-// ---------------------------------------------------------------------------
-// twisted block-Thomas factorization
-//   chain A (lines 0..k-1, upward):   S_j = D_j - L_j X_{j-1} U_{j-1}
-//   chain B (lines m-1..k+1, down):   S_j = D_j - U_j X_{j+1} L_{j+1}
-//   middle line k:                    S_k = D_k - L_k X_{k-1} U_{k-1} - U_k X_{k+1} L_{k+1}
-// X_j = S_j^-1 by in-place Gauss-Jordan with partial pivoting.
-// ---------------------------------------------------------------------------
-static int build_factors(Handle* h, const std::vector<Factor*>& Fs) {
-    const GridDesc& g = h->g;
-    const long N = (long)g.Nx * g.Ny;
-    const int B = (int)Fs.size();
-    if (B < 1) return fail(h, FDFD_ERR_INVALID, "no factor to build");
-    const int prec = Fs[0]->precision;
-    for (Factor* F : Fs) {
-        if (F->precision != prec)
-            return fail(h, FDFD_ERR_INVALID, "factors built together share one precision");
-        if (g.Nx <= g.Ny) { F->orient = 0; F->n = g.Nx; F->m = g.Ny; F->si = 1; F->sj = g.Nx; }
-        else { F->orient = 1; F->n = g.Ny; F->m = g.Nx; F->si = g.Nx; F->sj = 1; }
-        F->elem = (F->precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
-        F->k = F->m / 2;
-    }
-    const Factor* F0 = Fs[0];
-    const int n = F0->n, m = F0->m, k = F0->k;
-    const size_t elem = F0->elem;
-    if (n > MAX_LINE)
-        return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
-    int rc;
-    if ((rc = ensure_factor_work(h, n, 2 * B))) return rc;
-    if ((rc = ensure_solve_work(h, N))) return rc;
-    for (Factor* F : Fs) {
-        if ((rc = assemble(h, F))) return rc;
-        HIPCHK(h, hipMalloc(&F->Y, (size_t)m * n * n * elem));
-        HIPCHK(h, hipMalloc(&F->rho, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->rhoinv, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->kap, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->kapinv, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
-    }
-    std::vector<double> ps0(3 * (size_t)m);
-    for (int j = 0; j < m; ++j) {
-        ps0[3 * j] = std::numeric_limits<double>::infinity();
-        ps0[3 * j + 1] = 0;
-        ps0[3 * j + 2] = 0;
-    }
-    const int NB = h->panel_width ? h->panel_width : PMAX;  // fast-panel width
-    const int nsteps = std::max(k, m - 1 - k);
-    const int JS = 2 * B;  // job slots per step
-    const long sj = F0->sj;
-    std::vector<LineJob> lj((size_t)JS * (nsteps + 1));
-    std::vector<SchurJob> sjb((size_t)JS * (nsteps + 1));
-    std::vector<int> njobs(nsteps + 1, 0);
-    // chain c (2 per factor) owns slices of the handle's work buffers
-    auto line_job = [&](const Factor* F, int j, int c) {
-        LineJob J{};
-        J.Y = yline(F, j);
-        J.used = h->used + (long)c * n;
-        J.prow = F->rho + (long)j * n;
-        J.iprow = F->rhoinv + (long)j * n;
-        J.q = h->q + (long)c * n;
-        J.kap = F->kap + (long)j * n;
-        J.kapinv = F->kapinv + (long)j * n;
-        J.Vbuf = (char*)h->Vbuf + (size_t)c * n * PMAX * elem;
-        J.LU = (char*)h->LU + (size_t)c * 2 * PMAX * PMAX * elem;
-        J.piv = h->piv + c * PMAX;
-        J.Z = (char*)h->Zb + (size_t)c * PMAX * n * elem;
-        J.win = h->win + c * WIN;
-        J.verr = h->verr;
-        J.pivstat = F->pivstat + 3 * (long)j;
-        J.n = n;
-        return J;
-    };
-    auto schur_base = [&](const Factor* F, const Planes& P, int j, int c) {
-        SchurJob S{};
-        S.S = yline(F, j);
-        S.dsub = P.sub + j * sj; S.ddiag = P.diag + j * sj; S.dsup = P.sup + j * sj;
-        S.si = F->si;
-        S.q = h->q + (long)c * n;
-        S.used = h->used + (long)c * n;
-        S.iprow = F->rhoinv + (long)j * n;
-        S.win = h->win + c * WIN;
-        S.n = n;
-        return S;
-    };
-    // per step: the chain-A jobs of all factors, then the chain-B jobs
-    std::vector<int> nA(nsteps + 1, 0), nB(nsteps + 1, 0);
-    for (int s = 0; s <= nsteps; ++s) {
-        int nj = 0;
-        for (int chain = 0; chain < 2; ++chain) {
-            for (int f = 0; f < B; ++f) {
-                const Factor* F = Fs[f];
-                const Planes P = planes(F, N);
-                if (s == nsteps) {  // middle line k (listed once)
-                    if (chain == 1) continue;
-                    const int c = 2 * f;
-                    lj[(size_t)JS * s + nj] = line_job(F, k, c);
-                    SchurJob S = schur_base(F, P, k, c);
-                    if (k >= 1) {
-                        S.Y1 = yline(F, k - 1);
-                        S.rhoinv1 = F->rhoinv + (long)(k - 1) * n;
-                        S.kap1 = F->kap + (long)(k - 1) * n;
-                        S.l1 = P.L + k * sj;
-                        S.u1 = P.U + (k - 1) * sj;
-                    }
-                    if (k + 1 <= m - 1) {
-                        S.Y2 = yline(F, k + 1);
-                        S.rho2 = F->rho + (long)(k + 1) * n;
-                        S.kap2 = F->kap + (long)(k + 1) * n;
-                        S.l2 = P.U + k * sj;
-                        S.u2 = P.L + (k + 1) * sj;
-                    }
-                    sjb[(size_t)JS * s + nj] = S;
-                    ++nj;
-                    ++nA[s];
-                    continue;
-                }
-                if (chain == 0 && s < k) {  // chain A, line j = s
-                    const int j = s, c = 2 * f;
-                    lj[(size_t)JS * s + nj] = line_job(F, j, c);
-                    SchurJob S = schur_base(F, P, j, c);
-                    if (j > 0) {  // S_j = D_j - L_j X_{j-1} U_{j-1}
-                        S.Y1 = yline(F, j - 1);
-                        S.rhoinv1 = F->rhoinv + (long)(j - 1) * n;
-                        S.kap1 = F->kap + (long)(j - 1) * n;
-                        S.l1 = P.L + j * sj;
-                        S.u1 = P.U + (j - 1) * sj;
-                    }
-                    sjb[(size_t)JS * s + nj] = S;
-                    ++nj;
-                    ++nA[s];
-                }
-                if (chain == 1 && s < m - 1 - k) {  // chain B, line j = m-1-s
-                    const int j = m - 1 - s, c = 2 * f + 1;
-                    lj[(size_t)JS * s + nj] = line_job(F, j, c);
-                    SchurJob S = schur_base(F, P, j, c);
-                    if (j < m - 1) {  // S_j = D_j - U_j X_{j+1} L_{j+1}
-                        S.Y1 = yline(F, j + 1);
-                        S.rhoinv1 = F->rhoinv + (long)(j + 1) * n;
-                        S.kap1 = F->kap + (long)(j + 1) * n;
-                        S.l1 = P.U + j * sj;
-                        S.u1 = P.L + (j + 1) * sj;
-                    }
-                    sjb[(size_t)JS * s + nj] = S;
-                    ++nj;
-                    ++nB[s];
-                }
-            }
-        }
-        njobs[s] = nj;
-    }
-    const bool fp64 = (prec == FDFD_FP64);
-    // one pass of the factorization as a launch sequence
-    // chain A runs on the handle's stream and chain B on the second stream
-    // (they are independent until the middle line); joined before the middle
-    auto launch_pass = [&](const LineJob* lj_d, const SchurJob* sj_d, bool full, bool prof) {
-        auto line = [&](hipStream_t st, const LineJob* J, const SchurJob* SJ, int nj) {
-            auto scope = [&](int id, double fl, double by) {
-                return prof ? new ProfScope(h, id, fl, by, st) : nullptr;
-            };
-            {
-                ProfScope* ps = scope(P_SCHUR, 0, 2.0 * n * n * elem * nj);
-                if (fp64) launch_schur<double>(st, SJ, nj, n);
-                else launch_schur<float>(st, SJ, nj, n);
-                delete ps;
-            }
-            for (int k0 = 0; k0 < n;) {
-                int nb;
-                const int fresh = std::min(n, WIN + k0);
-                {
-                    ProfScope* ps = scope(P_PANEL, 0, 0);
-                    if (!full) {
-                        nb = std::min(NB, n - k0);
-                        if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb, fresh);
-                        else launch_wpanel<float>(st, J, nj, n, k0, nb, fresh);
-                    } else {
-                        nb = 16;  // full-search panel width: powers of two
-                        while (nb > n - k0) nb >>= 1;
-                        if (fp64) launch_panel<double>(st, J, nj, n, k0, nb);
-                        else launch_panel<float>(st, J, nj, n, k0, nb);
-                    }
-                    delete ps;
-                }
-                {
-                    ProfScope* ps = scope(P_ZBUILD, 0, 0);
-                    if (fp64) launch_zbuild<double>(st, J, nj, n, k0, nb, fresh, !full);
-                    else launch_zbuild<float>(st, J, nj, n, k0, nb, fresh, !full);
-                    delete ps;
-                }
-                {
-                    ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * nb * nj,
-                                          2.0 * n * (double)n * elem * nj);
-                    if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb);
-                    else launch_trailing<float>(st, J, nj, n, k0, nb);
-                    delete ps;
-                }
-                k0 += nb;
-            }
-            launch_maps(st, J, nj, n);
-        };
-        (void)hipEventRecord(h->ev_fork, h->stream);
-        (void)hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-        for (int s = 0; s < nsteps; ++s) {
-            const LineJob* J = lj_d + (size_t)JS * s;
-            const SchurJob* SJ = sj_d + (size_t)JS * s;
-            if (nA[s]) line(h->stream, J, SJ, nA[s]);
-            if (nB[s]) line(h->stream2, J + nA[s], SJ + nA[s], nB[s]);
-        }
-        (void)hipEventRecord(h->ev_join, h->stream2);
-        (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
-        line(h->stream, lj_d + (size_t)JS * nsteps, sj_d + (size_t)JS * nsteps, njobs[nsteps]);
-    };
-
-    hipEvent_t e0, e1;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
-    // Pass 0: windowed pivot search with verification.  Pass 1 (only if a
-    // row outside a window would have won a GEPP step): full-column search.
-    int used_full = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool full = (pass == 1) || h->full_pivot_search;
-        used_full = full ? 1 : 0;
-        // job tables live at fixed addresses per shape so that a captured
-        // launch sequence can be replayed for every factorization of it
-        FactorGraph* G = nullptr;
-        for (auto& gr : h->graphs)
-            if (gr.n == n && gr.m == m && gr.prec == prec && gr.nb == NB && gr.full == full &&
-                gr.nfac == B)
-                G = &gr;
-        if (!G) {
-            h->graphs.push_back(FactorGraph{});
-            G = &h->graphs.back();
-            G->n = n; G->m = m; G->prec = prec; G->nb = NB; G->full = full; G->nfac = B;
-            HIPCHK(h, hipMalloc(&G->lj_d, sizeof(LineJob) * lj.size()));
-            HIPCHK(h, hipMalloc(&G->sj_d, sizeof(SchurJob) * sjb.size()));
-        }
-        HIPCHK(h, hipMemcpyAsync(G->lj_d, lj.data(), sizeof(LineJob) * lj.size(),
-                                 hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(G->sj_d, sjb.data(), sizeof(SchurJob) * sjb.size(),
-                                 hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemsetAsync(h->verr, 0, sizeof(int), h->stream));
-        for (Factor* F : Fs)
-            HIPCHK(h, hipMemcpyAsync(F->pivstat, ps0.data(), sizeof(double) * ps0.size(),
-                                     hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));  // host tables are stack-local
-        HIPCHK(h, hipEventRecord(e0, h->stream));
-        if (h->prof.enabled || !h->use_graphs) {
-            launch_pass(G->lj_d, G->sj_d, full, h->prof.enabled);
-        } else {
-            if (!G->exec) {
-                hipGraph_t gr;
-                HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-                launch_pass(G->lj_d, G->sj_d, full, false);
-                HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
-                hipError_t e = hipGraphInstantiate(&G->exec, gr, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(gr);
-                if (e != hipSuccess) {
-                    G->exec = nullptr;
-                    return fail(h, FDFD_ERR_HIP, std::string("factor graph: ") + hipGetErrorString(e));
-                }
-            }
-            HIPCHK(h, hipGraphLaunch(G->exec, h->stream));
-        }
-        HIPCHK(h, hipGetLastError());
-        HIPCHK(h, hipEventRecord(e1, h->stream));
-        HIPCHK(h, hipEventSynchronize(e1));
-        int verr = 0;
-        COPY(h, &verr, h->verr, sizeof(int), hipMemcpyDeviceToHost);
-        if (full || !verr) break;
-    }
-    float ms = 0;
-    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    int first_singular = -1;
-    for (Factor* F : Fs) {
-        F->t_factor_ms = ms;
-        F->pivot_fallback = used_full;
-        std::vector<double> ps(3 * (size_t)m);
-        COPY(h, ps.data(), F->pivstat, sizeof(double) * ps.size(), hipMemcpyDeviceToHost);
-        double mn = std::numeric_limits<double>::infinity(), mx = 0;
-        F->singular_line = -1;
-        for (int j = 0; j < m; ++j) {
-            mn = std::min(mn, ps[3 * j]);
-            mx = std::max(mx, ps[3 * j + 1]);
-            if (ps[3 * j + 2] != 0 && F->singular_line < 0) F->singular_line = j;
-        }
-        F->pivot_growth = (mn > 0) ? mx / mn : std::numeric_limits<double>::infinity();
-        if (F->singular_line >= 0 && first_singular < 0) first_singular = F->singular_line;
-    }
-    if (first_singular >= 0)
-        return fail(h, FDFD_ERR_SINGULAR, "zero pivot in line block " + std::to_string(first_singular));
-    for (Factor* F : Fs)
-        if ((rc = build_sweep_tables(h, F))) return rc;
-    return FDFD_OK;
-}
-
-static int build_factor(Handle* h, Factor* F) { return build_factors(h, std::vector<Factor*>{F}); }
 
 // ---------------------------------------------------------------------------
 // FP64 Krylov building blocks
@@ -764,7 +237,7 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
             double2* vj = V + (long)j * N;
             double2* zj = Zk + (long)j * N;
             double2* w = V + (long)(j + 1) * N;
-            if ((rc = prec_apply(h, F, trans, vj, zj))) return rc;
+            if ((rc = rt_prec_apply(h, F, trans, vj, zj))) return rc;
             {
                 ProfScope ps(h, P_STENCIL, 0, 64.0 * N);
                 launch_apply(h->stream, w, zj, F->coef, g.Nx, g.Ny, trans, nullptr);
@@ -858,10 +331,7 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.device = device_ordinal;
     const char* eg = std::getenv("FDFD_GRAPHS");
     hh->impl.use_graphs = !(eg && eg[0] == '0');
-    if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&hh->impl.stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&hh->impl.ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&hh->impl.ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;
     }
@@ -875,17 +345,12 @@ int fdfd_destroy(fdfd_handle* hh) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     for (auto& kv : h->factors) free_factor(kv.second);
-    clear_graphs(h);
-    void* bufs[] = {h->eps, h->used, h->q, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
-                    h->r_in, h->g_mid,
-                    h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec, h->dpart,
-                    h->rpart, h->hdev, h->tpart};
+    void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
+                    h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
+                    h->dpart, h->rpart, h->hdev};
     for (void* b : bufs) dfree(b);
     prof_resolve(h->prof);
     for (auto e : h->prof.pool) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(h->ev_fork);
-    (void)hipEventDestroy(h->ev_join);
-    (void)hipStreamDestroy(h->stream2);
     (void)hipStreamDestroy(h->stream);
     delete hh;
     return FDFD_OK;
@@ -951,7 +416,7 @@ int fdfd_factor(fdfd_handle* hh, int pol, double omega, int precision, int* fact
     F->pol = pol;
     F->omega = omega;
     F->precision = precision;
-    int rc = build_factor(h, F);
+    int rc = rt_build_factor(h, F);
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->t_factor_ms = F->t_factor_ms;
@@ -1208,8 +673,6 @@ int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, doub
     if (resid) *resid = st.resid;
     return rc;
 }
-int rt_build_factor(Handle* h, Factor* F) { return build_factor(h, F); }
-int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) { return build_factors(h, Fs); }
 void rt_free_factor(Factor* F) { free_factor(F); }
 int rt_norm(Handle* h, const double2* x, long N, double* out) { return dev_norm(h, x, N, out); }
 int rt_multidot(Handle* h, const double2* V, long N, int k, const double2* w, cd* out) {

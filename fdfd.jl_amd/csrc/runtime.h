@@ -8,14 +8,17 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fdfd.h"
 #include "fdfd_common.h"
 
 namespace fdfd {
 
 using cd = std::complex<double>;
 
+// per-kernel profile slots: Gauss-Jordan trailing update, panel, block
+// formation, reduced-solve steps, stencil/residual, Z build, complex GEMM
 enum ProfId { P_TRAIL = 0, P_PANEL = 1, P_SCHUR = 2, P_SWEEP = 3, P_STENCIL = 4, P_ZBUILD = 5,
-              P_COUNT = 6 };
+              P_GEMM = 6, P_COUNT = 7 };
 
 struct Prof {
     bool enabled = false;
@@ -28,46 +31,35 @@ struct Prof {
     double bytes[P_COUNT] = {0};
 };
 
-// Block-tridiagonal factors of one operator (pol, omega) on the handle's grid.
+// Factors of one operator (pol, omega) on the handle's grid: the blocks of
+// the block cyclic reduction and its reduced-solve program.
 struct Factor {
     int id = 0, pol = 1, precision = 0;
     double omega = 0;
-    int n = 0, m = 0, orient = 0;
-    long si = 1, sj = 1;
-    double scale = 1;       // A' = scale * A
-    double2* coef = nullptr;  // 5 planes of N
+    int n = 0, m = 0, orient = 0;  // line length, number of lines, 0: lines along x
+    long si = 1, sj = 1;           // stride of a line's cells / of consecutive lines
+    double scale = 1;              // A' = scale * A
+    double2* coef = nullptr;       // 5 planes of N
     double2 *sxf = nullptr, *sxb = nullptr, *syf = nullptr, *syb = nullptr;
-    void* Y = nullptr;        // m * n * n complex (double2 or float2)
-    size_t elem = 16;
-    int *rho = nullptr, *rhoinv = nullptr, *kap = nullptr, *kapinv = nullptr;
-    double* pivstat = nullptr;  // 3 per line
-    int k = 0;                  // middle line of the twisted factorization
-    // device sweep tables: [transpose][phase] with phase 0 forward, 1 middle, 2 backward
-    SweepJob* sweep_tab[2] = {nullptr, nullptr};
-    hipGraphExec_t sweep_exec[2] = {nullptr, nullptr};
-    int nfwd = 0, nbwd = 0;     // number of forward / backward steps
-    std::vector<int> fwd_jobs, bwd_jobs;  // jobs per step
-    std::vector<int> fwd_has_a, bwd_has_a;  // step has a chain-A job (listed first)
+    size_t elem = 16;              // bytes per complex factor entry
+    void* blocks = nullptr;        // nblocks * n * n complex (double2 or float2)
+    int nblocks = 0;
+    int *prow = nullptr, *iprow = nullptr;  // m * n Gauss-Jordan pivot maps per line
+    double* pivstat = nullptr;     // 3 per line
+    struct SolveStep { int kind, first, count; };  // kind 0: MvJob range, 1: DiagJob range
+    std::vector<SolveStep> ssteps;
+    MvJob* mv_tab = nullptr;
+    DiagJob* dg_tab = nullptr;
+    hipGraphExec_t solve_exec = nullptr;
     double t_factor_ms = 0;
     double pivot_growth = 0;
     int singular_line = -1;
     int pivot_fallback = 0;   // 1 if the full-column pivot search was used
 };
 
-// captured launch sequence of one factorization pass for a shape
-struct FactorGraph {
-    int n = 0, m = 0, prec = 0, nb = 0, nfac = 1;
-    bool full = false;
-    hipGraphExec_t exec = nullptr;
-    LineJob* lj_d = nullptr;
-    SchurJob* sj_d = nullptr;
-};
-
 struct Handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;          // second elimination chain
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     GridDesc g{};
     bool has_grid = false, has_eps = false;
     double2* eps = nullptr;
@@ -75,23 +67,21 @@ struct Handle {
     int next_id = 1;
     std::string err;
     int panel_width = 0;  // 0 = default per precision
-    // factorization work (two chains)
+    // Gauss-Jordan work, one channel per line inverted concurrently
     int work_n = 0, work_ch = 0;
-    int *used = nullptr, *q = nullptr, *piv = nullptr, *win = nullptr, *verr = nullptr;
+    int *used = nullptr, *piv = nullptr, *win = nullptr, *verr = nullptr;
     void *Vbuf = nullptr, *LU = nullptr, *Zb = nullptr;
     bool full_pivot_search = false;
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
-    std::vector<FactorGraph> graphs;
     // sweep / Krylov work
     long work_N = 0;
-    double2 *r_in = nullptr, *g_mid = nullptr, *x_out = nullptr;  // sweep vectors
+    double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors
     double2 *V = nullptr, *Zk = nullptr;  // Krylov bases
     int kdim = 0;
     double2 *tmp = nullptr, *tmp2 = nullptr, *bvec = nullptr, *xvec = nullptr;
     double2* dpart = nullptr;   // multidot partials
     double* rpart = nullptr;    // norm partials
     double2* hdev = nullptr;    // small device vector of coefficients
-    double2* tpart = nullptr;   // transposed-sweep partials
     int nblk = 0;
     Prof prof;
 };
@@ -99,11 +89,62 @@ struct Handle {
 // error helper: record message, return code
 int fail(Handle* h, int code, const std::string& msg);
 
+#define HIPCHK(h, call)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail((h), e_ == hipErrorOutOfMemory ? FDFD_ERR_OOM : FDFD_ERR_HIP, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define COPY(h, dst, src, bytes, kind)                             \
+    do {                                                           \
+        int rc_ = stream_copy((h), (dst), (src), (bytes), (kind)); \
+        if (rc_) return rc_;                                       \
+    } while (0)
+
 // Copy ordered on the handle's stream, complete on return.  The handle's
 // streams are non-blocking, so a plain hipMemcpy (null stream) would not wait
 // for kernels still queued on them; every host<->device transfer of the
 // library goes through here instead.
 int stream_copy(Handle* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+
+
+// HIP events around launches on a stream while profiling is enabled
+hipEvent_t ev_get(Prof& p);
+void prof_resolve(Prof& p);
+struct ProfScope {
+    Handle* h;
+    int id;
+    double flops, bytes;
+    hipStream_t st;
+    hipEvent_t a{}, b{};
+    ProfScope(Handle* h_, int id_, double f, double by, hipStream_t s = nullptr)
+        : h(h_), id(id_), flops(f), bytes(by), st(s ? s : h_->stream) {
+        if (h->prof.enabled) {
+            a = ev_get(h->prof);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~ProfScope() {
+        if (h->prof.enabled) {
+            b = ev_get(h->prof);
+            (void)hipEventRecord(b, st);
+            h->prof.pending.push_back({a, b, id, flops, bytes});
+            if (h->prof.pending.size() > 4096) prof_resolve(h->prof);
+        }
+    }
+};
+
+// work buffers of the handle (grown on demand)
+int ensure_factor_work(Handle* h, int n, int nch);
+int ensure_solve_work(Handle* h, long N);
+
+// block cyclic reduction (bcr.cpp)
+int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);
+int rt_build_factor(Handle* h, Factor* F);
+int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out);
+size_t rt_factor_bytes(int n, int m, int precision);
 
 }  // namespace fdfd
 
