@@ -150,24 +150,24 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
         if (i < n && kb + ak + AE <= n) {
             const C* src = A + (long)i * n + kb + ak;
 #pragma unroll
-            for (int e = 0; e < AE; ++e) ra[e] = src[e];
+            for (int e = 0; e < AE; ++e) ra[e] = gload(src + e);
         } else {
 #pragma unroll
             for (int e = 0; e < AE; ++e) {
                 const int k = kb + ak + e;
                 ra[e].x = R(0);
                 ra[e].y = R(0);
-                if (i < n && k < n) ra[e] = A[(long)i * n + k];
+                if (i < n && k < n) ra[e] = gload(A + (long)i * n + k);
             }
         }
         const int k = kb + bk;
-        const C* srcb = B + (long)((k < n) ? (bm ? bm[k] : k) : 0) * n;
+        const C* srcb = B + (long)((k < n) ? (bm ? gload_s(bm + k) : k) : 0) * n;
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
             const int j = j0 + bc + 16 * e;
             rb[e].x = R(0);
             rb[e].y = R(0);
-            if (k < n && j < n) rb[e] = srcb[j];
+            if (k < n && j < n) rb[e] = gload(srcb + j);
         }
     };
     auto store = [&](int buf) {
@@ -245,12 +245,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
                 C* o = out + (long)i * n + j;
                 R ox = R(0), oy = R(0);
                 if (J.acc) {
-                    const C old = *o;
+                    const C old = gload(o);
                     ox = old.x;
                     oy = old.y;
                 }
                 const R re = cre[p][q][v], im = cim[p][q][v];
-                *o = J.sub ? mk<C>(ox - re, oy - im) : mk<C>(ox + re, oy + im);
+                gstore(o, J.sub ? mk<C>(ox - re, oy - im) : mk<C>(ox + re, oy + im));
             }
         }
 }
@@ -274,8 +274,10 @@ __device__ inline void mv_rows(const cplx<R>* ya, const cplx<R>* yb, const cplx<
     using C = cplx<R>;
     int c = lane;
     for (; c + 192 < n; c += 256) {
-        const C a0 = ya[c], a1 = ya[c + 64], a2 = ya[c + 128], a3 = ya[c + 192];
-        const C b0 = yb[c], b1 = yb[c + 64], b2 = yb[c + 128], b3 = yb[c + 192];
+        const C a0 = gload(ya + c), a1 = gload(ya + c + 64), a2 = gload(ya + c + 128),
+                a3 = gload(ya + c + 192);
+        const C b0 = gload(yb + c), b1 = gload(yb + c + 64), b2 = gload(yb + c + 128),
+                b3 = gload(yb + c + 192);
         const C x0 = w[c], x1 = w[c + 64], x2 = w[c + 128], x3 = w[c + 192];
         ar += a0.x * x0.x - a0.y * x0.y + a1.x * x1.x - a1.y * x1.y + a2.x * x2.x -
               a2.y * x2.y + a3.x * x3.x - a3.y * x3.y;
@@ -287,7 +289,7 @@ __device__ inline void mv_rows(const cplx<R>* ya, const cplx<R>* yb, const cplx<
               b3.x * x3.y + b3.y * x3.x;
     }
     for (; c < n; c += 64) {
-        const C a = ya[c], b = yb[c], x = w[c];
+        const C a = gload(ya + c), b = gload(yb + c), x = w[c];
         ar += a.x * x.x - a.y * x.y;
         ai += a.x * x.y + a.y * x.x;
         br += b.x * x.x - b.y * x.y;

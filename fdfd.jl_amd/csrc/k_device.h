@@ -12,6 +12,31 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 template <typename R> __device__ inline R shfl_x(R v, int o) { return __shfl_xor(v, o, 64); }
 
+// Pointers read from job tables are generic; accesses through them compile to
+// flat instructions, which also count against the LDS wait counter and so
+// serialize LDS reads behind outstanding global loads.  These helpers access
+// complex values through the global address space (global_load/store).
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <typename R> struct ev2;
+template <> struct ev2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct ev2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename C> __device__ inline C gload(const C* p) {
+    using V = typename ev2<decltype(C{}.x)>::type;
+    const V v = *(const GLOBAL_AS V*)p;
+    C c;
+    c.x = v.x;
+    c.y = v.y;
+    return c;
+}
+template <typename C> __device__ inline void gstore(C* p, C c) {
+    using V = typename ev2<decltype(C{}.x)>::type;
+    V v;
+    v.x = c.x;
+    v.y = c.y;
+    *(GLOBAL_AS V*)p = v;
+}
+template <typename T> __device__ inline T gload_s(const T* p) { return *(const GLOBAL_AS T*)p; }
+
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (B < E) {

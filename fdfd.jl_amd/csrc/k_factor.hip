@@ -425,18 +425,22 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
             for (int bj = 0; bj < 2; ++bj) {
                 const int j = j0 + wc * 32 + bj * 16 + lr;
                 const bool ld = i < n && j < n && !(j >= k0 && j < k0 + nb);
-                old[bi][v][bj] = ld ? A[(long)i * n + j] : mk<C>(R(0), R(0));
+                old[bi][v][bj].x = R(0);
+                old[bi][v][bj].y = R(0);
+                if (ld) old[bi][v][bj] = gload(A + (long)i * n + j);
             }
         }
     for (int e = t; e < 64 * NB; e += 256) {
         const int ii = e / NB, k = e % NB, i = i0 + ii;
-        const C v = (i < n && k < nb) ? Vb[(long)i * PMAX + k] : mk<C>(R(0), R(0));
+        C v = mk<C>(R(0), R(0));
+        if (i < n && k < nb) v = gload(Vb + (long)i * PMAX + k);
         Vr[ii][k] = v.x;
         Vi[ii][k] = v.y;
     }
     for (int e = t; e < 64 * NB; e += 256) {
         const int k = e / 64, jj = e % 64, j = j0 + jj;
-        const C v = (j < n && k < nb) ? Zg[(long)k * n + j] : mk<C>(R(0), R(0));
+        C v = mk<C>(R(0), R(0));
+        if (j < n && k < nb) v = gload(Zg + (long)k * n + j);
         Zr[k][jj] = v.x;
         Zi[k][jj] = v.y;
     }
@@ -483,9 +487,9 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
                 const int j = j0 + jj;
                 if (j >= n) continue;
                 C out;
-                if (pivrow) out = Zg[(long)sl * n + j];
+                if (pivrow) out = gload(Zg + (long)sl * n + j);
                 else out = mk<C>(old[bi][v][bj].x - are[bi][bj][v], old[bi][v][bj].y - aim[bi][bj][v]);
-                A[(long)i * n + j] = out;
+                gstore(A + (long)i * n + j, out);
             }
         }
 }
