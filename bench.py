@@ -1,9 +1,10 @@
 """Benchmark: TM solves/s at 1024^2 to 1e-10 relative residual, mixed
 (complex64 block factors + FP64 FGMRES refinement) vs plain FP64 block factors.
 
-One step = set eps (device-resident) -> assemble + factor on the GPU ->
-refined solve of one source -> field recovery (Ez, Hx, Hy) into a
-device-resident output.  Inputs are in HBM before the timed region starts.
+One step = set eps (device-resident) -> assemble + factor (block cyclic
+reduction of the line ring) on the GPU -> refined solve of one source ->
+field recovery (Ez, Hx, Hy) into a device-resident output.  Inputs are in
+HBM before the timed region starts.
 
 Prints one JSON line (see the keys below).  The per-kernel roofline figures
 come from HIP events recorded around every launch on the library's stream
@@ -40,6 +41,7 @@ class Hip:
         self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         self.lib.hipFree.argtypes = [C.c_void_p]
         self.lib.hipDeviceSynchronize.argtypes = []
+        self.lib.hipMemcpyDtoD.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
 
     def upload(self, a: np.ndarray) -> C.c_void_p:
         p = C.c_void_p()
@@ -107,10 +109,25 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="mixed,fp64")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
+                    help="only run 4 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if a.probe_copy:
+        hip = Hip()
+        nbytes = a.probe_copy << 20
+        src, dst = hip.alloc(nbytes), hip.alloc(nbytes)
+        for _ in range(4):
+            assert hip.lib.hipMemcpyDtoD(dst, src, nbytes) == 0
+        hip.sync()
+        hip.free(src)
+        hip.free(dst)
+        print(json.dumps({"probe": "copy", "bytes_read_per_copy": nbytes,
+                          "bytes_written_per_copy": nbytes, "copies": 4}))
+        return
 
     import fdfd_oracle as O
     from fdfd import _lib
@@ -186,7 +203,7 @@ def main():
     dom = max(R["kernels"], key=lambda k: R["kernels"][k]["ms"])
     kd = R["kernels"][dom]
     avg_s = kd["ms"] / kd["launches"] / 1e3
-    if kd["flops"] > 0 and dom == "trailing":
+    if kd["flops"] > 0 and dom in ("gemm", "trailing"):
         achieved = kd["flops"] / kd["launches"] / avg_s
         peak = PEAK["fp32"] if head == "mixed" else PEAK["fp64"]
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
@@ -199,6 +216,7 @@ def main():
         roof = {"bound": "latency", "achieved": None, "peak": None, "unit": None, "frac": None,
                 "traffic": None, "kernel": dom,
                 "note": "dominant kernel has no algorithmic flop/byte count (pivot search)"}
+    roof["launches"] = kd["launches"]
     roof["avg_launch_us"] = avg_s * 1e6
     # whole-factorization MFMA figure: 8 n^3 m real flops per factorization
     roof["factor_tflops"] = 8.0 * n ** 3 * n / (R["factor_ms"] / 1e3) / 1e12

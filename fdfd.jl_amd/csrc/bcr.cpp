@@ -26,6 +26,7 @@
 #include "../../include/fdfd.h"
 #include "bcr_plan.h"
 #include "runtime.h"
+#include "tri_qr.h"
 
 namespace fdfd {
 
@@ -43,10 +44,12 @@ size_t rt_factor_bytes(int n, int m, int precision) {
 // device tables for a set of factors built together (same grid and shape)
 // ---------------------------------------------------------------------------
 struct Tables {
+    TriJob* tri = nullptr;
     FormJob* form = nullptr;
     LineJob* lj = nullptr;
     GemmJob* gemm = nullptr;
     ~Tables() {
+        if (tri) (void)hipFree(tri);
         if (form) (void)hipFree(form);
         if (lj) (void)hipFree(lj);
         if (gemm) (void)hipFree(gemm);
@@ -98,6 +101,9 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
     const Plan P = make_plan(m);
+    // the QR work of a first-level line lives in its channel's panel buffer
+    if ((size_t)tri::work_doubles(n) * sizeof(double) > (size_t)n * PMAX * elem)
+        return fail(h, FDFD_ERR_INVALID, "internal: QR work does not fit the panel buffer");
     int rc;
     if ((rc = ensure_factor_work(h, n, std::max(1, P.max_gj * B)))) return rc;
     if ((rc = ensure_solve_work(h, N))) return rc;
@@ -115,6 +121,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
     }
     // --- factorization job tables: per step, the jobs of factor 0, 1, ...
+    std::vector<TriJob> tris;
     std::vector<FormJob> forms;
     std::vector<LineJob> ljs;
     std::vector<GemmJob> gemms;
@@ -127,7 +134,20 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     if (d.line >= 0 && d.pl[q] >= 0) out2[q] = plane_ptr(F, N, d.line, d.pl[q]);
             };
             for (int i = 0; i < st.count; ++i) {
-                if (st.kind == ST_FORM) {
+                if (st.kind == ST_TRI) {
+                    const STri& s = P.tris[st.first + i];
+                    const int c = f * st.count + i;  // work channel
+                    TriJob J{};
+                    J.out = blk(F, s.line);
+                    own_planes(F, N, s.line, &J.sub, &J.diag, &J.sup);
+                    dvec(s.dd, J.dd);
+                    J.prow = F->prow + (long)s.line * n;
+                    J.iprow = F->iprow + (long)s.line * n;
+                    J.pivstat = F->pivstat + 3 * (long)s.line;
+                    J.work = (double*)((char*)h->Vbuf + (size_t)c * n * PMAX * elem);
+                    J.si = F->si;
+                    tris.push_back(J);
+                } else if (st.kind == ST_FORM) {
                     const SForm& s = P.forms[st.first + i];
                     FormJob J{};
                     J.out = blk(F, s.out);
@@ -184,7 +204,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         HIPCHK(h, hipMalloc((void**)dst, bytes));
         return stream_copy(h, (void*)*dst, v.data(), bytes, hipMemcpyHostToDevice);
     };
-    if ((rc = upload(&T.form, forms)) || (rc = upload(&T.lj, ljs)) || (rc = upload(&T.gemm, gemms)))
+    if ((rc = upload(&T.tri, tris)) || (rc = upload(&T.form, forms)) || (rc = upload(&T.lj, ljs)) ||
+        (rc = upload(&T.gemm, gemms)))
         return rc;
 
     const bool fp64 = (prec == FDFD_FP64);
@@ -194,10 +215,17 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         auto scope = [&](int id, double fl, double by) {
             return prof ? new ProfScope(h, id, fl, by, st) : nullptr;
         };
-        int of = 0, ol = 0, og = 0;
+        int ot = 0, of = 0, ol = 0, og = 0;
         for (const Step& s : P.fsteps) {
             const int nj = s.count * B;
-            if (s.kind == ST_FORM) {
+            if (s.kind == ST_TRI) {
+                ProfScope* ps = scope(P_SCHUR, 0, 3.0 * n * n * elem * nj);
+                launch_trifactor(st, T.tri + ot, nj, n);
+                if (fp64) launch_triinv<double>(st, T.tri + ot, nj, n);
+                else launch_triinv<float>(st, T.tri + ot, nj, n);
+                delete ps;
+                ot += nj;
+            } else if (s.kind == ST_FORM) {
                 ProfScope* ps = scope(P_SCHUR, 0, (double)n * n * elem * nj);
                 if (fp64) launch_form<double>(st, T.form + of, nj, n);
                 else launch_form<float>(st, T.form + of, nj, n);
@@ -207,8 +235,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 double fl = 0;
                 for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
                 ProfScope* ps = scope(P_GEMM, fl, 0);
-                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n);
-                else launch_gemm<float>(st, T.gemm + og, nj, n);
+                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m);
+                else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m);
                 delete ps;
                 og += nj;
             } else {
@@ -224,7 +252,9 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                             if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb, fresh);
                             else launch_wpanel<float>(st, J, nj, n, k0, nb, fresh);
                         } else {
-                            nb = 16;  // full-search panel width: powers of two
+                            // full-search panel width: powers of two (16 columns of
+                            // complex128 per thread would not fit 1024 threads' registers)
+                            nb = fp64 ? 8 : 16;
                             while (nb > n - k0) nb >>= 1;
                             if (fp64) launch_panel<double>(st, J, nj, n, k0, nb);
                             else launch_panel<float>(st, J, nj, n, k0, nb);

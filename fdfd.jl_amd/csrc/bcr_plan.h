@@ -4,6 +4,8 @@
 // lines' own blocks (line j is inverted in place in block j); coupling and
 // product blocks are drawn from a pool and recycled once consumed.
 // Factorization steps (in order):
+//   ST_TRI   first-level inversion of the listed lines' own periodic
+//            tridiagonal blocks (Givens QR; natural order, identity maps)
 //   ST_FORM  first-level block formation from the coefficient planes
 //   ST_GJ    batched Gauss-Jordan inversion of the listed lines
 //   ST_GEMM  batched complex products out = [out] -/+ sum_t A_t B_t[prow_t]
@@ -34,6 +36,10 @@ struct SCoup {
     SDiag d;
     int blk = -1;
 };
+struct STri {
+    int line = -1;
+    SDiag dd;  // added to the diagonal (only for a single line: its own wrap)
+};
 struct SForm {
     int out = -1, own = -1;
     SDiag dd;
@@ -61,13 +67,14 @@ struct SDg {
     int xv[2] = {0, 0}, xl[2] = {-1, -1};
     SDiag d[2];
 };
-enum StepKind { ST_FORM = 0, ST_GJ = 1, ST_GEMM = 2, ST_MV = 3, ST_DIAG = 4 };
+enum StepKind { ST_FORM = 0, ST_GJ = 1, ST_GEMM = 2, ST_MV = 3, ST_DIAG = 4, ST_TRI = 5 };
 struct Step {
     int kind, first, count;
 };
 
 struct Plan {
     int m = 0, nblocks = 0, max_gj = 0;
+    std::vector<STri> tris;
     std::vector<SForm> forms;
     std::vector<int> gj;
     std::vector<SGemm> gemms;
@@ -94,6 +101,11 @@ inline Plan make_plan(int m) {
         if (v.empty()) return;
         P.fsteps.push_back({ST_FORM, (int)P.forms.size(), (int)v.size()});
         P.forms.insert(P.forms.end(), v.begin(), v.end());
+    };
+    auto step_tri = [&](const std::vector<STri>& v) {
+        P.fsteps.push_back({ST_TRI, (int)P.tris.size(), (int)v.size()});
+        P.tris.insert(P.tris.end(), v.begin(), v.end());
+        P.max_gj = std::max(P.max_gj, (int)v.size());
     };
     auto step_gj = [&](const std::vector<int>& v) {
         P.fsteps.push_back({ST_GJ, (int)P.gj.size(), (int)v.size()});
@@ -143,14 +155,12 @@ inline Plan make_plan(int m) {
         return v;
     };
     if (m == 1) {
-        SForm f;
-        f.out = 0;
-        f.own = 0;
-        f.dd.line = 0;
-        f.dd.pl[0] = 0;
-        f.dd.pl[1] = 1;
-        step_form({f});
-        step_gj({0});
+        STri t;
+        t.line = 0;
+        t.dd.line = 0;
+        t.dd.pl[0] = 0;
+        t.dd.pl[1] = 1;
+        step_tri({t});
         SolveBlock b;
         SDg c;
         c.ov = VT; c.ol = 0; c.x0v = VX; c.x0l = 0;
@@ -200,15 +210,13 @@ inline Plan make_plan(int m) {
         std::map<std::pair<int, int>, SCoup> NC;  // couplings of the new ring
         SolveBlock bk;
         if (first) {
-            std::vector<SForm> pre;
+            std::vector<STri> inv;
             for (int j : elines) {
-                SForm f;
-                f.out = j;
-                f.own = j;
-                pre.push_back(f);
+                STri t;
+                t.line = j;
+                inv.push_back(t);
             }
-            step_form(pre);
-            step_gj(elines);
+            step_tri(inv);
             std::vector<SForm> post;
             auto add_term = [&](SForm& f, int j, const SDiag& rs, const SDiag& cs) {
                 f.jl[f.nt] = j;

@@ -130,6 +130,21 @@ struct FormJob {
     long si;
 };
 
+// Inverse of a line's own periodic tridiagonal block (first reduction level)
+// by Givens QR: T[a][a] = diag + dd0 + dd1, T[a][a+1] = sup, T[a][a-1] = sub
+// (indices wrap).  out receives X = T^-1 in natural order (prow = iprow =
+// identity); work holds the rotations and R (FP64).
+struct TriJob {
+    void* out;
+    const double2 *sub, *diag, *sup;
+    const double2* dd[2];
+    int* prow;
+    int* iprow;
+    double* pivstat;
+    double* work;
+    long si;
+};
+
 // out = [acc ? out : 0] -/+ sum_t A_t * B_t[bmap_t, :]   (B rows gathered;
 // sub = 1 subtracts the products, sub = 0 adds them)
 struct GemmJob {
@@ -209,8 +224,11 @@ void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int
 template <typename R>
 void launch_form(hipStream_t s, const FormJob* jobs_d, int njobs, int n);
 void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
+void launch_trifactor(hipStream_t s, const TriJob* jobs_d, int njobs, int n);
 template <typename R>
-void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n);
+void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n);
+template <typename R>
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3);
 template <typename R>
 void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);

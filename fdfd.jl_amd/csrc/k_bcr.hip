@@ -11,6 +11,7 @@
 // The reduced solve applies the same steps to a vector (k_mv, k_diag).
 #include "fdfd_common.h"
 #include "k_device.h"
+#include "tri_qr.h"
 
 namespace fdfd {
 
@@ -79,6 +80,65 @@ template void launch_form<double>(hipStream_t, const FormJob*, int, int);
 template void launch_form<float>(hipStream_t, const FormJob*, int, int);
 
 // ---------------------------------------------------------------------------
+// First-level inversion of the lines' own periodic tridiagonal blocks
+// (Givens QR, tri_qr.h): one thread factors one line (O(n), FP64), then one
+// thread per column forms X[:, b] = R^-1 Q e_b (O(n) each), writing the
+// column twice (Q e_b, then the solution) in coalesced rows of a wave.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_trifactor(const TriJob* jobs, int njobs, int n) {
+    const int jid = blockIdx.x * 64 + threadIdx.x;
+    if (jid >= njobs) return;
+    const TriJob J = jobs[jid];
+    auto get = [&](int i, tri::cz& d, tri::cz& e, tri::cz& f) {
+        const long ei = (long)i * J.si;
+        double2 dv = J.diag[ei];
+        if (J.dd[0]) dv = cadd(dv, J.dd[0][ei]);
+        if (J.dd[1]) dv = cadd(dv, J.dd[1][ei]);
+        const double2 ev = J.sup[ei], fv = J.sub[ei];
+        d = tri::mkz(dv.x, dv.y);
+        e = tri::mkz(ev.x, ev.y);
+        f = tri::mkz(fv.x, fv.y);
+    };
+    double pmin, pmax;
+    const int rc = tri::factor(n, get, J.work, pmin, pmax);
+    J.pivstat[0] = pmin;
+    J.pivstat[1] = pmax;
+    J.pivstat[2] = rc ? 1.0 : 0.0;
+}
+
+void launch_trifactor(hipStream_t s, const TriJob* jobs_d, int njobs, int n) {
+    k_trifactor<<<(njobs + 63) / 64, 64, 0, s>>>(jobs_d, njobs, n);
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_triinv(const TriJob* jobs, int n) {
+    using C = cplx<R>;
+    const TriJob J = jobs[blockIdx.y];
+    const int b0 = blockIdx.x * 64, b = b0 + threadIdx.x;
+    // wave-uniform start of the rotation pass (the column n-1 is dense)
+    const int kstart = (b0 + 63 >= n - 1) ? 0 : tri::first_nonzero(n, b0);
+    if (b >= n) return;
+    C* X = (C*)J.out;
+    const double* w = J.work;
+    auto put = [&](int k, tri::cz v) { gstore(X + (long)k * n + b, mk<C>(R(v.re), R(v.im))); };
+    auto get = [&](int k) {
+        const C y = gload(X + (long)k * n + b);
+        return tri::mkz((double)y.x, (double)y.y);
+    };
+    tri::apply_q(n, w, b, kstart, put);
+    tri::apply_rinv(n, w, b, get, put);
+    J.prow[b] = b;
+    J.iprow[b] = b;
+}
+
+template <typename R>
+void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n) {
+    k_triinv<R><<<dim3((n + 63) / 64, njobs), 64, 0, s>>>(jobs_d, n);
+}
+template void launch_triinv<double>(hipStream_t, const TriJob*, int, int);
+template void launch_triinv<float>(hipStream_t, const TriJob*, int, int);
+
+// ---------------------------------------------------------------------------
 // Gauss-Jordan start state of a line: no row pivoted, window = rows 0..63.
 // ---------------------------------------------------------------------------
 __global__ void k_gjreset(const LineJob* jobs, int n) {
@@ -107,14 +167,17 @@ void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n) {
 // (double-buffered, re and im planes split, A stored k-major so that both
 // operands are read as MFMA fragments without bank conflicts).  The complex
 // product is four real MFMAs per 16x16 block:
-//   re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br.
+//   re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br,
+// or, with G3 (Gauss's three-multiplication form), three:
+//   t1 += Ar Br, t2 += Ai Bi, t3 += (Ar + Ai)(Br + Bi);
+//   re = t1 - t2, im = t3 - t1 - t2.
 // ---------------------------------------------------------------------------
 template <typename R> struct gemm_cfg;
 template <> struct gemm_cfg<float> { static constexpr int TM = 128, TN = 128; };
 template <> struct gemm_cfg<double> { static constexpr int TM = 128, TN = 64; };
 constexpr int GKT = 16;
 
-template <typename R>
+template <typename R, bool G3>
 __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
     using C = cplx<R>;
     using M = mfma_t<R>;
@@ -182,13 +245,15 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
             sBi[buf][bk][bc + 16 * e] = rb[e].y;
         }
     };
-    acc_t cre[BM][BN], cim[BM][BN];
+    // G3: cre = t1, cim = t3, c2 = t2; otherwise cre = re, cim = im
+    acc_t cre[BM][BN], cim[BM][BN], c2[G3 ? BM : 1][G3 ? BN : 1];
 #pragma unroll
     for (int bi = 0; bi < BM; ++bi)
 #pragma unroll
         for (int bj = 0; bj < BN; ++bj) {
             cre[bi][bj] = acc_t{0, 0, 0, 0};
             cim[bi][bj] = acc_t{0, 0, 0, 0};
+            if constexpr (G3) c2[bi][bj] = acc_t{0, 0, 0, 0};
         }
     load(0);
     store(0);
@@ -210,23 +275,44 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
                 br[q] = sBr[cur][kk + lk][wc * WN + q * 16 + lr];
                 bi[q] = sBi[cur][kk + lk][wc * WN + q * 16 + lr];
             }
-            // four passes so that consecutive MFMAs use different accumulators
+            // passes over all blocks so that consecutive MFMAs use different
+            // accumulators
+            if constexpr (G3) {
+                R as[BM], bs[BN];
 #pragma unroll
-            for (int p = 0; p < BM; ++p)
+                for (int p = 0; p < BM; ++p) as[p] = ar[p] + ai[p];
 #pragma unroll
-                for (int q = 0; q < BN; ++q) cre[p][q] = M::run(ar[p], br[q], cre[p][q]);
+                for (int q = 0; q < BN; ++q) bs[q] = br[q] + bi[q];
 #pragma unroll
-            for (int p = 0; p < BM; ++p)
+                for (int p = 0; p < BM; ++p)
 #pragma unroll
-                for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ar[p], bi[q], cim[p][q]);
+                    for (int q = 0; q < BN; ++q) cre[p][q] = M::run(ar[p], br[q], cre[p][q]);
 #pragma unroll
-            for (int p = 0; p < BM; ++p)
+                for (int p = 0; p < BM; ++p)
 #pragma unroll
-                for (int q = 0; q < BN; ++q) cre[p][q] = M::run(an[p], bi[q], cre[p][q]);
+                    for (int q = 0; q < BN; ++q) c2[p][q] = M::run(ai[p], bi[q], c2[p][q]);
 #pragma unroll
-            for (int p = 0; p < BM; ++p)
+                for (int p = 0; p < BM; ++p)
 #pragma unroll
-                for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ai[p], br[q], cim[p][q]);
+                    for (int q = 0; q < BN; ++q) cim[p][q] = M::run(as[p], bs[q], cim[p][q]);
+            } else {
+#pragma unroll
+                for (int p = 0; p < BM; ++p)
+#pragma unroll
+                    for (int q = 0; q < BN; ++q) cre[p][q] = M::run(ar[p], br[q], cre[p][q]);
+#pragma unroll
+                for (int p = 0; p < BM; ++p)
+#pragma unroll
+                    for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ar[p], bi[q], cim[p][q]);
+#pragma unroll
+                for (int p = 0; p < BM; ++p)
+#pragma unroll
+                    for (int q = 0; q < BN; ++q) cre[p][q] = M::run(an[p], bi[q], cre[p][q]);
+#pragma unroll
+                for (int p = 0; p < BM; ++p)
+#pragma unroll
+                    for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ai[p], br[q], cim[p][q]);
+            }
         }
         if (s + 1 < ns) store(cur ^ 1);
         __syncthreads();
@@ -249,19 +335,28 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
                     ox = old.x;
                     oy = old.y;
                 }
-                const R re = cre[p][q][v], im = cim[p][q][v];
+                R re, im;
+                if constexpr (G3) {
+                    re = cre[p][q][v] - c2[p][q][v];
+                    im = cim[p][q][v] - cre[p][q][v] - c2[p][q][v];
+                } else {
+                    re = cre[p][q][v];
+                    im = cim[p][q][v];
+                }
                 gstore(o, J.sub ? mk<C>(ox - re, oy - im) : mk<C>(ox + re, oy + im));
             }
         }
 }
 
 template <typename R>
-void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n) {
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3) {
     constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
-    k_gemm<R><<<dim3((n + TN - 1) / TN, (n + TM - 1) / TM, njobs), 256, 0, s>>>(jobs_d, n);
+    const dim3 grid((n + TN - 1) / TN, (n + TM - 1) / TM, njobs);
+    if (gauss3) k_gemm<R, true><<<grid, 256, 0, s>>>(jobs_d, n);
+    else k_gemm<R, false><<<grid, 256, 0, s>>>(jobs_d, n);
 }
-template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int);
-template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int);
+template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int);
+template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Batched dense matrix-vector step of the reduced solve.  The gathered input

@@ -65,11 +65,43 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long k) {
 __device__ inline unsigned long long mag_key(double m) { return (unsigned long long)__double_as_longlong(m); }
 __device__ inline unsigned long long mag_key(float m) { return (unsigned long long)(unsigned)__float_as_int(m); }
 
-// store the L/U rows of panel step cc and the pivot bookkeeping
-template <typename C> __device__ inline void put_lu_row(C* L, C* U, int cc, int c, C v) {
+// Inverse of the pivot block V[piv, :] = L U (unit lower L, upper U, nb x nb,
+// in LDS).  Thread c < nb computes column c, Vpinv[:, c] = U^-1 L^-1 e_c, in
+// Xs.  Then every thread helps write Vpinv (first PMAX x PMAX block of LU)
+// and U (second block) to the job's LU buffer, rows k < nb.
+template <typename C>
+__device__ inline void finish_pivot_block(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
+                                          C (*Xs)[PMAX + 1], C* Ud, int nb, int t, int nthr,
+                                          C* LUg) {
     using R = decltype(C{}.x);
-    L[cc * PMAX + c] = mk<C>((c < cc) ? v.x : (c == cc ? R(1) : R(0)), (c < cc) ? v.y : R(0));
-    U[cc * PMAX + c] = mk<C>((c >= cc) ? v.x : R(0), (c >= cc) ? v.y : R(0));
+    if (t < nb) Ud[t] = crcp(Us[t][t]);
+    __syncthreads();
+    if (t < nb) {
+        const int c = t;
+        for (int k = 0; k < nb; ++k) {  // y = L^-1 e_c (zero above row c)
+            C y = mk<C>(k == c ? R(1) : R(0), R(0));
+            for (int q = c; q < k; ++q) y = csub(y, cmul(Ls[k][q], Xs[q][c]));
+            Xs[k][c] = y;
+        }
+        for (int k = nb - 1; k >= 0; --k) {  // x = U^-1 y
+            C x = Xs[k][c];
+            for (int q = k + 1; q < nb; ++q) x = csub(x, cmul(Us[k][q], Xs[q][c]));
+            Xs[k][c] = cmul(x, Ud[k]);
+        }
+    }
+    __syncthreads();
+    C* Vg = LUg;
+    C* Ug = LUg + PMAX * PMAX;
+    for (int e = t; e < nb * PMAX; e += nthr) {
+        const int k = e / PMAX, c = e % PMAX;
+        C v = mk<C>(R(0), R(0)), u = mk<C>(R(0), R(0));
+        if (c < nb) {
+            v = Xs[k][c];
+            u = Us[k][c];
+        }
+        gstore(Vg + e, v);
+        gstore(Ug + e, u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -79,7 +111,8 @@ template <typename C> __device__ inline void put_lu_row(C* L, C* U, int cc, int 
 // threshold partial pivoting, |pivot| >= PIV_U * max |candidate| over every
 // unpivoted row, or verr is raised and the factorization is redone with the
 // full-column search.
-// Outputs: pivots, L and U of V[piv, :], updated window list.
+// Outputs: pivots, the inverse of the pivot block V[piv, :] and its U factor,
+// updated window list.
 // ---------------------------------------------------------------------------
 template <typename R, int NB>
 __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh) {
@@ -90,16 +123,20 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     const int nwin = min(WIN, n - k0);
     __shared__ int prs[NB];
     __shared__ R pmags[NB];
-    const int row = (lane < nwin) ? J.win[lane] : -1;
+    __shared__ C Ls[PMAX][PMAX + 1];
+    __shared__ C Us[PMAX][PMAX + 1];
+    __shared__ C Xs[PMAX][PMAX + 1];
+    __shared__ C Ud[PMAX];
+    const int row = (lane < nwin) ? gload_s(J.win + lane) : -1;
     bool act = row >= 0;
-    bool chosen = false;
+    int mystep = -1;  // panel step at which this lane's row became the pivot
     const C* A = (const C*)J.Y;
-    C* L = (C*)J.LU;
-    C* U = L + PMAX * PMAX;
     C W[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-        W[c] = (act && c < nb) ? A[(long)row * n + k0 + c] : mk<C>(R(0), R(0));
+    for (int c = 0; c < NB; ++c) {
+        W[c] = mk<C>(R(0), R(0));
+        if (act && c < nb) W[c] = gload(A + (long)row * n + k0 + c);
+    }
     static_for<0, NB>([&](auto ic) {
         constexpr int cc = decltype(ic)::value;
         if (cc < nb) {
@@ -117,9 +154,7 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
                 prs[cc] = row;
                 pmags[cc] = bm;
                 act = false;
-                chosen = true;
-#pragma unroll
-                for (int c = 0; c < NB; ++c) put_lu_row(L, U, cc, c, W[c]);
+                mystep = cc;
             }
             // pivot-row entries are broadcast one at a time (uniform control
             // flow); the pivot lane's registers are not modified in this step
@@ -134,14 +169,20 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
             }
         }
     });
-    // identity padding of L and U beyond nb
-    if (lane >= nb && lane < PMAX) {
-        for (int c = 0; c < PMAX; ++c) {
-            L[lane * PMAX + c] = mk<C>(c == lane ? R(1) : R(0), R(0));
-            U[lane * PMAX + c] = mk<C>(c == lane ? R(1) : R(0), R(0));
+    // A pivot lane's registers are frozen from its step on: they hold its row
+    // of L (multipliers, c < step) and of U (c >= step).
+    if (mystep >= 0) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < nb) {
+                const bool lo = c < mystep;
+                Ls[mystep][c] = csel(lo, W[c], mk<C>(c == mystep ? R(1) : R(0), R(0)));
+                Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[c]);
+            }
         }
     }
     __syncthreads();
+    finish_pivot_block<C>(Ls, Us, Xs, Ud, nb, lane, 64, (C*)J.LU);
     if (lane < nb) {
         const int p = prs[lane];
         J.piv[lane] = p;
@@ -161,7 +202,7 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
         J.pivstat[1] = mx;
     }
     // new window: surviving rows, then the next nb fresh rows
-    const bool keep = (row >= 0) && !chosen;
+    const bool keep = (row >= 0) && mystep < 0;
     const unsigned long long km = __ballot(keep);
     const int below = __popcll(km & ((1ull << lane) - 1ull));
     const int nkeep = __popcll(km);
@@ -194,9 +235,11 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
     __shared__ C cval[2][16][NB];
     __shared__ int prs[NB];
     __shared__ R pmags[NB];
+    __shared__ C Ls[PMAX][PMAX + 1];
+    __shared__ C Us[PMAX][PMAX + 1];
+    __shared__ C Xs[PMAX][PMAX + 1];
+    __shared__ C Ud[PMAX];
     const C* A = (const C*)J.Y;
-    C* Lg = (C*)J.LU;
-    C* Ug = Lg + PMAX * PMAX;
     const int r = t;
     const bool valid = r < n;
     bool act = valid && (J.used[r] == 0);
@@ -237,7 +280,11 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
             prs[cc] = pr;
             pmags[cc] = pm;
         }
-        if (t < PMAX) put_lu_row(Lg, Ug, cc, t, (t < NB) ? cval[buf][pw][t < NB ? t : 0] : zero);
+        if (t < NB) {
+            const C v = cval[buf][pw][t];
+            Ls[cc][t] = csel(t < cc, v, mk<C>(t == cc ? R(1) : R(0), R(0)));
+            Us[cc][t] = csel(t < cc, zero, v);
+        }
         if (r == pr) act = false;
         if (act) {
             const C ucc = cval[buf][pw][cc];
@@ -250,12 +297,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    if (t >= NB && t < PMAX) {
-        for (int c = 0; c < PMAX; ++c) {
-            Lg[t * PMAX + c] = mk<C>(c == t ? R(1) : R(0), R(0));
-            Ug[t * PMAX + c] = mk<C>(c == t ? R(1) : R(0), R(0));
-        }
-    }
+    finish_pivot_block<C>(Ls, Us, Xs, Ud, NB, t, 1024, (C*)J.LU);
     if (t < NB) {
         const int p = prs[t];
         J.piv[t] = p;
@@ -291,96 +333,93 @@ template void launch_panel<double>(hipStream_t, const LineJob*, int, int, int, i
 template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// Z build, panel copy and pivot check.
-//   blocks [0, nzb):  Z[:, col] = U^-1 L^-1 T[:, col], T[k][col] = A[piv[k]][col]
-//                     (delta(k, col-k0) for col in K).  A column is solved by a
-//                     group of PMAX lanes with shuffle substitution; a block
-//                     of 256 threads handles 8 columns.
-//   blocks [nzb, ..): copy V = A[:, K] to Vbuf (every row) and, if verify,
+// Z build, panel copy and pivot check (one thread per column or row).
+//   blocks [0, nzb):  Z[:, col] = Vpinv T[:, col], T[k][col] = A[piv[k]][col]
+//                     (delta(k, col-k0) for col in K), Vpinv = V[piv, :]^-1.
+//   blocks [nzb, ..): copy V = A[r, K] to Vbuf (every row r) and, if verify,
 //                     check rows r >= fresh: the value GEPP would compare at
 //                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
 //                     y_c = a_c / U[c][c], must satisfy
 //                     PIV_U * |a_c| <= |U[c][c]| (threshold pivoting).
 // ---------------------------------------------------------------------------
 template <typename R>
-__device__ inline cplx<R> bcast_c(cplx<R> v, int l) {
-    return mk<cplx<R>>(bcast(v.x, l), bcast(v.y, l));
-}
-
-// blockDim = 512 (8 waves); each wave handles ZB columns (Z part) or ZB rows
-// (copy / check part), lane i < nb holding entry i of the small vector.
-constexpr int ZB = 2;
-template <typename R>
-__global__ __launch_bounds__(512) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
+__global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
                                                 int verify, int nzb) {
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ C Ls[PMAX][PMAX + 1];
+    __shared__ C Vi[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
-    __shared__ C Ud[PMAX];  // 1 / U[k][k]
-    const C* Lg = (const C*)J.LU;
-    const C* Ug = Lg + PMAX * PMAX;
-    for (int e = threadIdx.x; e < PMAX * PMAX; e += blockDim.x) {
-        Ls[e / PMAX][e % PMAX] = Lg[e];
-        Us[e / PMAX][e % PMAX] = Ug[e];
+    __shared__ C Ud[PMAX];
+    __shared__ int pv[PMAX];
+    const C* LUg = (const C*)J.LU;
+    for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
+        Vi[e / PMAX][e % PMAX] = gload(LUg + e);
+        Us[e / PMAX][e % PMAX] = gload(LUg + PMAX * PMAX + e);
     }
+    if ((int)threadIdx.x < nb) pv[threadIdx.x] = gload_s(J.piv + threadIdx.x);
     __syncthreads();
-    if (threadIdx.x < PMAX) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
+    if ((int)threadIdx.x < nb) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
     __syncthreads();
     const C* A = (const C*)J.Y;
-    const int i = threadIdx.x & 63;  // lane = row of the small system
-    const int wv = threadIdx.x >> 6;
-    const bool il = i < nb;
     if ((int)blockIdx.x < nzb) {
-        for (int q = 0; q < ZB; ++q) {
-            const int col = (blockIdx.x * 8 + wv) * ZB + q;
-            if (col >= n) break;
-            C x = mk<C>(R(0), R(0));
-            if (il) {
-                if (col >= k0 && col < k0 + nb) x = mk<C>(col - k0 == i ? R(1) : R(0), R(0));
-                else x = A[(long)J.piv[i] * n + col];
+        const int col = blockIdx.x * blockDim.x + threadIdx.x;
+        if (col >= n) return;
+        const bool inK = col >= k0 && col < k0 + nb;
+        C t[PMAX];
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            t[k] = mk<C>(R(0), R(0));
+            if (k < nb) {
+                if (inK) t[k].x = (col - k0 == k) ? R(1) : R(0);
+                else t[k] = gload(A + (long)pv[k] * n + col);
             }
-            for (int k = 0; k < nb; ++k) {  // L y = t (unit lower)
-                const C xk = bcast_c<R>(x, k);
-                if (il && i > k) x = csub(x, cmul(Ls[i][k], xk));
-            }
-            for (int k = nb - 1; k >= 0; --k) {  // U z = y
-                if (i == k) x = cmul(x, Ud[k]);
-                const C xk = bcast_c<R>(x, k);
-                if (il && i < k) x = csub(x, cmul(Us[i][k], xk));
-            }
-            if (il) ((C*)J.Z)[(long)i * n + col] = x;
+        }
+        C* Zg = (C*)J.Z;
+        for (int k = 0; k < nb; ++k) {
+            C z = mk<C>(R(0), R(0));
+#pragma unroll
+            for (int q = 0; q < PMAX; ++q)
+                if (q < nb) z = cadd(z, cmul(Vi[k][q], t[q]));
+            gstore(Zg + (long)k * n + col, z);
         }
     } else {
-        C* Vb = (C*)J.Vbuf;
-        int bad = 0;
-        for (int q = 0; q < ZB; ++q) {
-            const int r = (((int)blockIdx.x - nzb) * 8 + wv) * ZB + q;
-            if (r >= n) break;
-            C v = il ? A[(long)r * n + k0 + i] : mk<C>(R(0), R(0));
-            if (i < PMAX) Vb[(long)r * PMAX + i] = v;
-            if (verify && r >= fresh) {
-                // the value GEPP compares at step k: a_k = V[r][k] - sum y_c U[c][k]
-                for (int k = 0; k < nb; ++k) {
-                    const C ak = bcast_c<R>(v, k);  // wave-uniform
+        const int r = ((int)blockIdx.x - nzb) * blockDim.x + threadIdx.x;
+        if (r >= n) return;
+        C v[PMAX];
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c) {
+            v[c] = mk<C>(R(0), R(0));
+            if (c < nb) v[c] = gload(A + (long)r * n + k0 + c);
+        }
+        C* Vb = (C*)J.Vbuf + (long)r * PMAX;
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c)
+            if (c < nb) gstore(Vb + c, v[c]);
+        if (verify && r >= fresh) {
+            int bad = 0;
+#pragma unroll
+            for (int k = 0; k < PMAX; ++k) {
+                if (k < nb) {
+                    const C ak = v[k];
                     if (R(PIV_U) * cabs1(ak) > cabs1(Us[k][k])) bad = 1;
                     const C yk = cmul(ak, Ud[k]);
-                    if (il && i > k) v = csub(v, cmul(yk, Us[k][i]));
+#pragma unroll
+                    for (int c = k + 1; c < PMAX; ++c)
+                        if (c < nb) v[c] = csub(v[c], cmul(yk, Us[k][c]));
                 }
             }
+            if (bad) *(GLOBAL_AS int*)J.verr = 1;
         }
-        if (bad && i == 0) *J.verr = 1;
     }
 }
 
 template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int verify) {
-    const int per_block = 8 * ZB;
-    const int nzb = (n + per_block - 1) / per_block;
-    const int nvb = (n + per_block - 1) / per_block;
-    k_zbuild<R><<<dim3(nzb + nvb, njobs), 512, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb);
+    const int nzb = (n + 255) / 256;
+    const int nvb = (n + 255) / 256;
+    k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb);
 }
 template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
 template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);

@@ -331,6 +331,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.device = device_ordinal;
     const char* eg = std::getenv("FDFD_GRAPHS");
     hh->impl.use_graphs = !(eg && eg[0] == '0');
+    const char* e3 = std::getenv("FDFD_GEMM3M");
+    hh->impl.gemm3m = (e3 && e3[0] == '0') ? 0 : 1;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;

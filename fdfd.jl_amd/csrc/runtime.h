@@ -73,6 +73,7 @@ struct Handle {
     void *Vbuf = nullptr, *LU = nullptr, *Zb = nullptr;
     bool full_pivot_search = false;
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
+    int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
     // sweep / Krylov work
     long work_N = 0;
     double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors

@@ -87,7 +87,19 @@ static double run_case(int n, int m, unsigned seed) {
     auto X = [&](int j, int a, int b) { return blk[j][(size_t)prow[j][a] * n + iprow[j][b]]; };
     for (const Step& st : P.fsteps) {
         for (int c = 0; c < st.count; ++c) {
-            if (st.kind == ST_FORM) {
+            if (st.kind == ST_TRI) {
+                const STri& t = P.tris[st.first + c];
+                std::vector<cd> a((size_t)n * n, cd(0));
+                for (int i = 0; i < n; ++i) {
+                    const int k = t.line * n + i;
+                    a[(size_t)i * n + i] += A.pl[4][k] + dvec(t.dd, i);
+                    a[(size_t)i * n + (i + 1) % n] += A.pl[1][k];
+                    a[(size_t)i * n + (i + n - 1) % n] += A.pl[0][k];
+                }
+                if (!invert(a, n)) return 1e300;
+                blk[t.line] = a;
+                for (int i = 0; i < n; ++i) prow[t.line][i] = iprow[t.line][i] = i;
+            } else if (st.kind == ST_FORM) {
                 const SForm& f = P.forms[st.first + c];
                 std::vector<cd> out((size_t)n * n, cd(0));
                 for (int a = 0; a < n; ++a)
