@@ -202,6 +202,8 @@ def main():
     # dominant kernel of the headline path, with its roofline
     dom = max(R["kernels"], key=lambda k: R["kernels"][k]["ms"])
     kd = R["kernels"][dom]
+    if not kd["launches"]:  # --profile-steps 0: no per-kernel events were recorded
+        kd = {"ms": 0.0, "launches": 1, "flops": 0.0, "bytes": 0.0}
     avg_s = kd["ms"] / kd["launches"] / 1e3
     if kd["flops"] > 0 and dom in ("gemm", "trailing"):
         achieved = kd["flops"] / kd["launches"] / avg_s

@@ -235,8 +235,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 double fl = 0;
                 for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
                 ProfScope* ps = scope(P_GEMM, fl, 0);
-                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m);
-                else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m);
+                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
+                else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
                 delete ps;
                 og += nj;
             } else {

@@ -228,7 +228,7 @@ void launch_trifactor(hipStream_t s, const TriJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n);
 template <typename R>
-void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3);
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3, int waves);
 template <typename R>
 void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);

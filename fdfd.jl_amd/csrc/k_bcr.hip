@@ -177,28 +177,32 @@ template <> struct gemm_cfg<float> { static constexpr int TM = 128, TN = 128; };
 template <> struct gemm_cfg<double> { static constexpr int TM = 128, TN = 64; };
 constexpr int GKT = 16;
 
-template <typename R, bool G3>
-__global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
+// WR x WC waves per workgroup, each owning a (TM/WR) x (TN/WC) sub-tile.
+template <typename R, bool G3, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC) void k_gemm(const GemmJob* jobs, int n) {
     using C = cplx<R>;
     using M = mfma_t<R>;
     using acc_t = typename M::acc;
+    constexpr int NT = 64 * WR * WC;
     constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
-    constexpr int WM = TM / 2, WN = TN / 2;
+    constexpr int WM = TM / WR, WN = TN / WC;
     constexpr int BM = WM / 16, BN = WN / 16;
     constexpr int PA = TM + 16, PB = TN + 16;
-    constexpr int AE = TM * GKT / 256;  // A elements per thread per stage
-    constexpr int BE = TN * GKT / 256;  // B elements per thread per stage
-    static_assert(AE == 8, "A loader assumes two threads per tile row");
+    constexpr int AE = TM * GKT / NT;   // A elements per thread per stage (consecutive k)
+    constexpr int TPR = GKT / AE;       // threads per A row
+    constexpr int TPK = NT / GKT;       // threads per B k-row
+    constexpr int BE = TN / TPK;        // B elements per thread per stage (stride TPK)
+    static_assert(AE * TPR == GKT && BE * TPK == TN && BM >= 1 && BN >= 1, "tile shape");
     __shared__ R sAr[2][GKT][PA], sAi[2][GKT][PA];
     __shared__ R sBr[2][GKT][PB], sBi[2][GKT][PB];
     const GemmJob J = jobs[blockIdx.z];
     const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wr = wv >> 1, wc = wv & 1;
+    const int wr = wv / WC, wc = wv % WC;
     const int lr = lane & 15, lk = lane >> 4;
     // loader coordinates
-    const int arow = t >> 1, ak = (t & 1) * AE;   // A: row, first k of 8
-    const int bk = t >> 4, bc = t & 15;           // B: k row, column lane (stride 16)
+    const int arow = t / TPR, ak = (t % TPR) * AE;  // A: row, first of AE k's
+    const int bk = t / TPK, bc = t % TPK;           // B: k row, column lane (stride TPK)
     const int spt = (n + GKT - 1) / GKT;
     const int ns = J.nt * spt;
     C ra[AE], rb[BE];
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
         const C* srcb = B + (long)((k < n) ? (bm ? gload_s(bm + k) : k) : 0) * n;
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
-            const int j = j0 + bc + 16 * e;
+            const int j = j0 + bc + TPK * e;
             rb[e].x = R(0);
             rb[e].y = R(0);
             if (k < n && j < n) rb[e] = gload(srcb + j);
@@ -241,8 +245,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
         }
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
-            sBr[buf][bk][bc + 16 * e] = rb[e].x;
-            sBi[buf][bk][bc + 16 * e] = rb[e].y;
+            sBr[buf][bk][bc + TPK * e] = rb[e].x;
+            sBi[buf][bk][bc + TPK * e] = rb[e].y;
         }
     };
     // G3: cre = t1, cim = t3, c2 = t2; otherwise cre = re, cim = im
@@ -349,14 +353,19 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmJob* jobs, int n) {
 }
 
 template <typename R>
-void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3) {
+void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3, int waves) {
     constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
     const dim3 grid((n + TN - 1) / TN, (n + TM - 1) / TM, njobs);
-    if (gauss3) k_gemm<R, true><<<grid, 256, 0, s>>>(jobs_d, n);
-    else k_gemm<R, false><<<grid, 256, 0, s>>>(jobs_d, n);
+    if (waves == 8) {
+        if (gauss3) k_gemm<R, true, 4, 2><<<grid, 512, 0, s>>>(jobs_d, n);
+        else k_gemm<R, false, 4, 2><<<grid, 512, 0, s>>>(jobs_d, n);
+    } else {
+        if (gauss3) k_gemm<R, true, 2, 2><<<grid, 256, 0, s>>>(jobs_d, n);
+        else k_gemm<R, false, 2, 2><<<grid, 256, 0, s>>>(jobs_d, n);
+    }
 }
-template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int);
-template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int);
+template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int, int);
+template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Batched dense matrix-vector step of the reduced solve.  The gathered input

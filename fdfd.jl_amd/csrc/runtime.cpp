@@ -333,6 +333,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.use_graphs = !(eg && eg[0] == '0');
     const char* e3 = std::getenv("FDFD_GEMM3M");
     hh->impl.gemm3m = (e3 && e3[0] == '0') ? 0 : 1;
+    const char* ew = std::getenv("FDFD_GEMM_WAVES");
+    hh->impl.gemm_waves = (ew && ew[0] == '4') ? 4 : 8;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;

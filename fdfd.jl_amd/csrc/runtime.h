@@ -74,6 +74,7 @@ struct Handle {
     bool full_pivot_search = false;
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
+    int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)
     // sweep / Krylov work
     long work_N = 0;
     double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors
