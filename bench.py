@@ -219,6 +219,20 @@ def main():
                 "traffic": None, "kernel": dom,
                 "note": "dominant kernel has no algorithmic flop/byte count (pivot search)"}
     roof["launches"] = kd["launches"]
+    # HBM traffic per launch of the dominant kernel from the committed PMC
+    # summary (FETCH_SIZE / WRITE_SIZE runs scaled by a copy of known size)
+    pmc = os.path.join(ROOT, "profiles", f"pmc_traffic_{n}_tm_{head}.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            P = json.load(fh)
+        prefix = {"gemm": "k_gemm<", "trailing": "k_trailing<", "solve": "k_mv<"}.get(dom)
+        rt = "float" if head == "mixed" else "double"
+        for name, v in P["kernels"].items():
+            if prefix and name.startswith(prefix + rt):
+                roof["traffic"] = v["read_bytes"] + v["write_bytes"]
+                roof["traffic_unit"] = "bytes/launch"
+                roof["traffic_source"] = os.path.relpath(pmc, ROOT)
+                break
     roof["avg_launch_us"] = avg_s * 1e6
     # whole-factorization MFMA figure: 8 n^3 m real flops per factorization
     roof["factor_tflops"] = 8.0 * n ** 3 * n / (R["factor_ms"] / 1e3) / 1e12
