@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import numpy as np  # noqa: E402
 
-PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12}
+PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 KERNELS = ["trailing", "panel", "form", "solve", "stencil", "zbuild", "gemm"]
 
 
@@ -108,7 +108,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precisions", default="mixed,fp64")
+    ap.add_argument("--precisions", default="mixed,fp64,mixed-fp32gemm",
+                    help="mixed: complex64 factors with bf16x3 GEMM products (default mixed "
+                         "path); mixed-fp32gemm: complex64 factors with FP32 MFMA products; "
+                         "fp64: complex128 factors")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only run 4 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -139,16 +142,23 @@ def main():
     g = d.grid
     om = float(d.omega[0])
     N = n * n
+    grid = Grid(0.02, g.Npml, (g.bounds[0][0], g.bounds[1][0]), (g.bounds[0][1], g.bounds[1][1]),
+                g.L0)
     eng = Engine(0)
-    eng.set_grid(Grid(0.02, g.Npml, (g.bounds[0][0], g.bounds[1][0]),
-                      (g.bounds[0][1], g.bounds[1][1]), g.L0))
+    eng.set_grid(grid)
+    eng_x = None  # handle with FP32 GEMM products on the mixed path (read at handle creation)
+    if "mixed-fp32gemm" in a.precisions.split(","):
+        os.environ["FDFD_MIXED_GEMM"] = "fp32"
+        eng_x = Engine(0)
+        eng_x.set_grid(grid)
+        del os.environ["FDFD_MIXED_GEMM"]
     hip = Hip()
     eps_d = hip.upload(np.ascontiguousarray(d.eps_r.ravel(order="F")))
     src_d = hip.upload(np.ascontiguousarray(d.src.ravel(order="F")))
     out_d = hip.alloc(16 * 3 * N)
-    lib, h = eng.lib, eng.h
 
-    def step(prec: int):
+    def step(prec: int, e):
+        lib, h = e.lib, e.h
         st = _lib.fdfd_stats()
         fid = C.c_int()
         _lib.check(lib, h, lib.fdfd_set_eps_d(h, eps_d))
@@ -162,23 +172,24 @@ def main():
 
     results = {}
     for pname in a.precisions.split(","):
-        prec = PRECISIONS[pname]
+        e = eng_x if pname == "mixed-fp32gemm" else eng
+        prec = PRECISIONS["mixed" if pname == "mixed-fp32gemm" else pname]
         for _ in range(a.warmup):
-            step(prec)
+            step(prec, e)
         hip.sync()
         t0 = time.perf_counter()
-        stats = [step(prec) for _ in range(a.steps)]
+        stats = [step(prec, e) for _ in range(a.steps)]
         hip.sync()
         dt = time.perf_counter() - t0
         worst = max(s["residual"] for s in stats)
         if not worst <= a.tol:
             raise SystemExit(f"{pname}: residual {worst:.2e} above {a.tol:.0e}: result invalid")
         # profiled steps: per-kernel device time from HIP events on the stream
-        eng.profile(True)
+        e.profile(True)
         for _ in range(a.profile_steps):
-            step(prec)
-        kern = {k: eng.profile_read(i) for i, k in enumerate(KERNELS)}
-        eng.profile(False)
+            step(prec, e)
+        kern = {k: e.profile_read(i) for i, k in enumerate(KERNELS)}
+        e.profile(False)
         results[pname] = {
             "solves_per_s": a.steps / dt, "ms_per_step": 1e3 * dt / a.steps,
             "factor_ms": float(np.mean([s["t_factor_ms"] for s in stats])),
@@ -205,11 +216,18 @@ def main():
     if not kd["launches"]:  # --profile-steps 0: no per-kernel events were recorded
         kd = {"ms": 0.0, "launches": 1, "flops": 0.0, "bytes": 0.0}
     avg_s = kd["ms"] / kd["launches"] / 1e3
-    if kd["flops"] > 0 and dom in ("gemm", "trailing"):
+    if kd["flops"] > 0 and dom == "gemm":
+        # algorithmic complex-GEMM flops (8 n^3 per job).  The default mixed
+        # path forms each real product from three bf16 MFMA products, so its
+        # ceiling is the bf16 dense peak / 3; FP32 / FP64 MFMA otherwise.
         achieved = kd["flops"] / kd["launches"] / avg_s
-        peak = PEAK["fp32"] if head == "mixed" else PEAK["fp64"]
+        if head == "mixed":
+            peak, pnote = PEAK["bf16"] / 3, "bf16 dense MFMA peak / 3 (bf16x3 products)"
+        else:
+            peak, pnote = PEAK["fp64"], "FP64 dense MFMA peak"
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom}
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom,
+                "peak_note": pnote}
     elif kd["bytes"] > 0:
         achieved = kd["bytes"] / kd["launches"] / avg_s
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
@@ -249,7 +267,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 factors + c128 refinement" if head == "mixed" else "c128",
+        "dtype": ("c64 factors (GEMM products bf16x3, FP32 accumulation) + c128 refinement"
+                  if head == "mixed" else "c128"),
         "data": "synthetic (seeded random-pixel coupler, generated in-process)",
         "config": {
             "workload": (f"TM driven solve, {n}x{n} cells, dl=0.02 um, 20-cell SC-PML, 600x600 "
@@ -257,6 +276,7 @@ def main():
                          "25-cell line source, lambda=1.55 um"),
             "n": n, "tol": a.tol,
             "plain_fp64_solves_per_s": results.get("fp64", {}).get("solves_per_s"),
+            "mixed_fp32_gemm_solves_per_s": results.get("mixed-fp32gemm", {}).get("solves_per_s"),
             "mixed_over_plain": (results["mixed"]["solves_per_s"] / results["fp64"]["solves_per_s"]
                                  if "mixed" in results and "fp64" in results else None),
             "factor_ms": {k: v["factor_ms"] for k, v in results.items()},
@@ -271,6 +291,8 @@ def main():
     for p in (eps_d, src_d, out_d):
         hip.free(p)
     eng.close()
+    if eng_x is not None:
+        eng_x.close()
     print(json.dumps(line))
 
 

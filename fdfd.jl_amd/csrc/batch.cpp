@@ -20,7 +20,6 @@ namespace fdfd {
 int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, double tol, int* iters,
              double* resid);
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);
-void rt_free_factor(Factor* F);
 }  // namespace fdfd
 
 using namespace fdfd;
@@ -168,7 +167,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
             }
-            for (Factor* F : Fs) rt_free_factor(F);
+            for (Factor* F : Fs) rt_free_factor(h, F);
         }
         (void)hipFree(dsrc);
         if (rc) break;

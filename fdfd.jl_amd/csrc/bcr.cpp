@@ -44,11 +44,13 @@ size_t rt_factor_bytes(int n, int m, int precision) {
 // device tables for a set of factors built together (same grid and shape)
 // ---------------------------------------------------------------------------
 struct Tables {
+    SplitJob* split = nullptr;
     TriJob* tri = nullptr;
     FormJob* form = nullptr;
     LineJob* lj = nullptr;
     GemmJob* gemm = nullptr;
     ~Tables() {
+        if (split) (void)hipFree(split);
         if (tri) (void)hipFree(tri);
         if (form) (void)hipFree(form);
         if (lj) (void)hipFree(lj);
@@ -82,6 +84,10 @@ static void own_planes(const Factor* F, long N, int line, const double2** sub,
 static void* blk(const Factor* F, int id) {
     return (char*)F->blocks + (size_t)id * F->n * F->n * F->elem;
 }
+// bf16 planes of block id (bf16x3 mode): 4 x n x n bf16
+static void* blk_b(const Factor* F, int id) {
+    return (char*)F->blocks_b + (size_t)id * F->n * F->n * 8;
+}
 
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     const GridDesc& g = h->g;
@@ -98,6 +104,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     }
     const int n = Fs[0]->n, m = Fs[0]->m;
     const size_t elem = Fs[0]->elem;
+    // mixed path with bf16x3 GEMM products: operands are split once per step
+    const bool b3 = (prec != FDFD_FP64) && h->gemm_b3;
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
     const Plan P = make_plan(m);
@@ -111,10 +119,17 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         if ((rc = rt_assemble(h, F))) return rc;
         F->nblocks = P.nblocks;
         const size_t bytes = (size_t)P.nblocks * n * n * elem;
-        if (hipMalloc(&F->blocks, bytes) != hipSuccess) {
-            F->blocks = nullptr;
+        F->blocks = arena_take(h, bytes);
+        if (!F->blocks)
             return fail(h, FDFD_ERR_OOM, "factor storage (" + std::to_string(bytes >> 20) +
                                              " MiB) does not fit in device memory");
+        F->blocks_bytes = bytes;
+        if (b3) {
+            const size_t bb = (size_t)P.nblocks * n * n * 8;
+            F->blocks_b = arena_take(h, bb);
+            if (!F->blocks_b)
+                return fail(h, FDFD_ERR_OOM, "bf16 operand storage does not fit in device memory");
+            F->blocks_b_bytes = bb;
         }
         HIPCHK(h, hipMalloc(&F->prow, sizeof(int) * (size_t)m * n));
         HIPCHK(h, hipMalloc(&F->iprow, sizeof(int) * (size_t)m * n));
@@ -122,10 +137,40 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     }
     // --- factorization job tables: per step, the jobs of factor 0, 1, ...
     std::vector<TriJob> tris;
+    std::vector<SplitJob> splits;
+    std::vector<int> nsplit;  // split jobs per GEMM step (all factors)
     std::vector<FormJob> forms;
     std::vector<LineJob> ljs;
     std::vector<GemmJob> gemms;
     for (const Step& st : P.fsteps) {
+        if (st.kind == ST_GEMM && b3) {
+            // distinct operands of the step: A blocks as is, (B block, map) transposed
+            int cnt = 0;
+            for (int f = 0; f < B; ++f) {
+                const Factor* F = Fs[f];
+                std::vector<int> seenA;
+                std::vector<std::pair<int, int>> seenB;
+                for (int i = 0; i < st.count; ++i) {
+                    const SGemm& g = P.gemms[st.first + i];
+                    for (int t = 0; t < g.nt; ++t) {
+                        if (std::find(seenA.begin(), seenA.end(), g.A[t]) == seenA.end()) {
+                            seenA.push_back(g.A[t]);
+                            splits.push_back({blk(F, g.A[t]), blk_b(F, g.A[t]), nullptr, 0});
+                            ++cnt;
+                        }
+                        const std::pair<int, int> kb(g.B[t], g.bmap[t]);
+                        if (std::find(seenB.begin(), seenB.end(), kb) == seenB.end()) {
+                            seenB.push_back(kb);
+                            splits.push_back({blk(F, g.B[t]), blk_b(F, g.B[t]),
+                                              g.bmap[t] >= 0 ? F->prow + (long)g.bmap[t] * n : nullptr,
+                                              1});
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+            nsplit.push_back(cnt);
+        }
         for (int f = 0; f < B; ++f) {
             const Factor* F = Fs[f];
             auto dvec = [&](const SDiag& d, const double2** out2) {
@@ -188,9 +233,15 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.acc = s.acc;
                     J.sub = s.sub;
                     for (int t = 0; t < s.nt; ++t) {
-                        J.A[t] = blk(F, s.A[t]);
-                        J.B[t] = blk(F, s.B[t]);
-                        J.bmap[t] = s.bmap[t] >= 0 ? F->prow + (long)s.bmap[t] * n : nullptr;
+                        if (b3) {  // split planes; the B gather is applied by the split
+                            J.A[t] = blk_b(F, s.A[t]);
+                            J.B[t] = blk_b(F, s.B[t]);
+                            J.bmap[t] = nullptr;
+                        } else {
+                            J.A[t] = blk(F, s.A[t]);
+                            J.B[t] = blk(F, s.B[t]);
+                            J.bmap[t] = s.bmap[t] >= 0 ? F->prow + (long)s.bmap[t] * n : nullptr;
+                        }
                     }
                     gemms.push_back(J);
                 }
@@ -205,7 +256,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         return stream_copy(h, (void*)*dst, v.data(), bytes, hipMemcpyHostToDevice);
     };
     if ((rc = upload(&T.tri, tris)) || (rc = upload(&T.form, forms)) || (rc = upload(&T.lj, ljs)) ||
-        (rc = upload(&T.gemm, gemms)))
+        (rc = upload(&T.gemm, gemms)) || (rc = upload(&T.split, splits)))
         return rc;
 
     const bool fp64 = (prec == FDFD_FP64);
@@ -215,7 +266,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         auto scope = [&](int id, double fl, double by) {
             return prof ? new ProfScope(h, id, fl, by, st) : nullptr;
         };
-        int ot = 0, of = 0, ol = 0, og = 0;
+        int ot = 0, of = 0, ol = 0, og = 0, os = 0, ogs = 0;
         for (const Step& s : P.fsteps) {
             const int nj = s.count * B;
             if (s.kind == ST_TRI) {
@@ -232,10 +283,18 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 delete ps;
                 of += nj;
             } else if (s.kind == ST_GEMM) {
+                if (b3) {
+                    const int ns = nsplit[ogs++];
+                    ProfScope* pss = scope(P_SCHUR, 0, 16.0 * n * n * ns);
+                    launch_split(st, T.split + os, ns, n);
+                    delete pss;
+                    os += ns;
+                }
                 double fl = 0;
                 for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
                 ProfScope* ps = scope(P_GEMM, fl, 0);
                 if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
+                else if (b3) launch_gemm_b3(st, T.gemm + og, nj, n);
                 else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
                 delete ps;
                 og += nj;

@@ -26,7 +26,6 @@ namespace fdfd {
 int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, double tol, int* iters,
              double* resid);
 int rt_build_factor(Handle* h, Factor* F);
-void rt_free_factor(Factor* F);
 int rt_norm(Handle* h, const double2* x, long N, double* out);
 int rt_multidot(Handle* h, const double2* V, long N, int k, const double2* w, cd* out);
 int rt_update(Handle* h, double2* w, const double2* V, long N, int k, const cd* c, double sign);
@@ -62,7 +61,7 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
     F->omega = omega0;
     F->precision = precision;
     int rc = rt_build_factor(h, F);
-    if (rc) { rt_free_factor(F); return rc; }
+    if (rc) { rt_free_factor(h, F); return rc; }
     const double sigma = (pol == FDFD_TM) ? -omega0 * omega0 * mu0n * eps0n : -omega0 * omega0 * mu0n;
     const double opscale = (pol == FDFD_TM) ? g.dx * g.dx : eps0n * g.dx * g.dx;
     // accuracy of each shift-invert application (FP64 FGMRES on the factors)
@@ -73,7 +72,7 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
     auto cleanup = [&]() {
         if (Vb) (void)hipFree(Vb);
         if (Wb) (void)hipFree(Wb);
-        rt_free_factor(F);
+        rt_free_factor(h, F);
     };
     if (hipMalloc(&Vb, sizeof(double2) * N * (ncv + 1)) != hipSuccess ||
         hipMalloc(&Wb, sizeof(double2) * N * ncv) != hipSuccess) {

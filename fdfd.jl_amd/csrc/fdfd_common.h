@@ -145,6 +145,15 @@ struct TriJob {
     long si;
 };
 
+// bf16 planes (re hi, re lo, im hi, im lo; [4][n][n]) of an n x n complex64
+// block: as is, or transposed with a row gather, dst[c][k] = src[map[k]][c].
+struct SplitJob {
+    const void* src;
+    void* dst;
+    const int* map;
+    int trans;
+};
+
 // out = [acc ? out : 0] -/+ sum_t A_t * B_t[bmap_t, :]   (B rows gathered;
 // sub = 1 subtracts the products, sub = 0 adds them)
 struct GemmJob {
@@ -229,6 +238,8 @@ template <typename R>
 void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_gemm(hipStream_t s, const GemmJob* jobs_d, int njobs, int n, int gauss3, int waves);
+void launch_gemm_b3(hipStream_t s, const GemmJob* jobs_d, int njobs, int n);
+void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);

@@ -368,6 +368,213 @@ template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int, in
 template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
+// bf16x3 products for the mixed path (opt-in).  Each FP32 value x is split
+// once into bf16 hi = bf16(x), lo = bf16(x - hi); a real product is
+// a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16 (relative error ~2^-16 per product, removed by the
+// FP64 refinement).
+//
+// k_split writes the four bf16 planes (re hi, re lo, im hi, im lo) of an
+// n x n complex block, either as is ([row][k], an A operand) or transposed
+// with the row gather of the B operand applied ([col][k] = B[map[k]][col]),
+// so that the GEMM reads every MFMA operand (8 consecutive k) as 16 bytes.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_bf16(float x, __bf16& h, __bf16& l) {
+    h = (__bf16)x;
+    l = (__bf16)(x - (float)h);
+}
+
+__global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
+    const SplitJob J = jobs[blockIdx.z];
+    __shared__ float2 tile[64][65];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const float2* src = (const float2*)J.src;
+    __bf16* dst = (__bf16*)J.dst;
+    const long plane = (long)n * n;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (!J.trans) {
+        for (int rr = ty; rr < 64; rr += 4) {
+            const int r = r0 + rr, c = c0 + tx;
+            if (r < n && c < n) {
+                const float2 v = gload(src + (long)r * n + c);
+                __bf16 h, l;
+                const long o = (long)r * n + c;
+                split_bf16(v.x, h, l);
+                dst[o] = h;
+                dst[plane + o] = l;
+                split_bf16(v.y, h, l);
+                dst[2 * plane + o] = h;
+                dst[3 * plane + o] = l;
+            }
+        }
+        return;
+    }
+    // transposed: out[col][k] = B[map[k]][col], tile (k rows r0.., cols c0..)
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int k = r0 + rr, c = c0 + tx;
+        float2 v = make_float2(0.f, 0.f);
+        if (k < n && c < n) v = gload(src + (long)(J.map ? gload_s(J.map + k) : k) * n + c);
+        tile[rr][tx] = v;
+    }
+    __syncthreads();
+    for (int cc = ty; cc < 64; cc += 4) {
+        const int c = c0 + cc, k = r0 + tx;
+        if (c < n && k < n) {
+            const float2 v = tile[tx][cc];
+            __bf16 h, l;
+            const long o = (long)c * n + k;
+            split_bf16(v.x, h, l);
+            dst[o] = h;
+            dst[plane + o] = l;
+            split_bf16(v.y, h, l);
+            dst[2 * plane + o] = h;
+            dst[3 * plane + o] = l;
+        }
+    }
+}
+
+void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n) {
+    k_split<<<dim3((n + 63) / 64, (n + 63) / 64, njobs), 256, 0, s>>>(jobs_d, n);
+}
+
+// GEMM on split operands: A = planes [4][n][n] as [row][k], B = planes of
+// B^T as [col][k] (gather already applied), both bf16.  Complex products in
+// the four-multiplication form (the three-multiplication form would need six
+// operand planes):  re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br.
+// Tile 128 x 128, 2 x 4 waves of 64 x 32, K staged 32 at a time through
+// double-buffered LDS; a 64-byte LDS row holds the 32 k of one row and
+// plane, its four 16-byte chunks XOR-swizzled by (row / 4) so that the 16
+// rows of an MFMA fragment read fall in distinct banks.
+constexpr int B3_TM = 128, B3_TN = 128, B3_KT = 32;
+
+__device__ inline int b3_chunk(int row, int c) { return c ^ ((row >> 2) & 3); }
+
+__global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n) {
+    __shared__ __align__(16) bf8v sA[2][4][B3_TM][4];
+    __shared__ __align__(16) bf8v sB[2][4][B3_TN][4];
+    const GemmJob J = jobs[blockIdx.z];
+    const int i0 = blockIdx.y * B3_TM, j0 = blockIdx.x * B3_TN;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = wv >> 2, wc = wv & 3;
+    const int lr = lane & 15, lg = lane >> 4;
+    const long plane = (long)n * n;
+    const bool vec = (n % 8) == 0;  // 16-byte chunks are aligned
+    const int spt = (n + B3_KT - 1) / B3_KT;
+    const int ns = J.nt * spt;
+    // loader: 2048 chunks per operand per stage (4 planes x 128 rows x 4
+    // chunks), four per thread
+    bf8v ra[4], rb[4];
+    auto ld = [&](const __bf16* base, int row0, int kb, int e, bf8v& out) {
+        const int id = t + 512 * e;  // chunk id
+        const int pl = id >> 9, row = (id >> 2) & 127, c = id & 3;
+        const int r = row0 + row, k = kb + 8 * c;
+        const __bf16* p = base + pl * plane + (long)r * n + k;
+        if (r < n && k + 8 <= n && vec) {
+            out = *(const GLOBAL_AS bf8v*)p;
+        } else {
+            for (int q = 0; q < 8; ++q) out[q] = (r < n && k + q < n) ? p[q] : (__bf16)0.f;
+        }
+    };
+    auto load = [&](int s) {
+        const int ts = s / spt, kb = (s - ts * spt) * B3_KT;
+        const __bf16* A = (const __bf16*)(ts ? J.A[1] : J.A[0]);
+        const __bf16* B = (const __bf16*)(ts ? J.B[1] : J.B[0]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            ld(A, i0, kb, e, ra[e]);
+            ld(B, j0, kb, e, rb[e]);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int id = t + 512 * e;
+            const int pl = id >> 9, row = (id >> 2) & 127, c = id & 3;
+            sA[buf][pl][row][b3_chunk(row, c)] = ra[e];
+            sB[buf][pl][row][b3_chunk(row, c)] = rb[e];
+        }
+    };
+    f4v cre[4][2], cim[4][2];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cre[p][q] = f4v{0, 0, 0, 0};
+            cim[p][q] = f4v{0, 0, 0, 0};
+        }
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int s = 0; s < ns; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < ns) load(s + 1);
+        bf8v a[4][4], b[2][4];  // [block][plane]: re hi, re lo, im hi, im lo
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int row = wr * 64 + p * 16 + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[p][v] = sA[cur][v][row][b3_chunk(row, lg)];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int col = wc * 32 + q * 16 + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[q][v] = sB[cur][v][col][b3_chunk(col, lg)];
+        }
+        bf8v na[4][2];  // -Ai (hi, lo)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) na[p][v] = -a[p][2 + v];
+        // (x_hi y_hi + x_hi y_lo + x_lo y_hi) for the four real products,
+        // blocks innermost so consecutive MFMAs use different accumulators
+#define B3_MM(ACC, X0, X1, Y0, Y1)                                                             \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, Y0, ACC[p][q], 0, 0, 0);         \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, Y1, ACC[p][q], 0, 0, 0);         \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, Y0, ACC[p][q], 0, 0, 0);
+        B3_MM(cre, a[p][0], a[p][1], b[q][0], b[q][1])    // + Ar Br
+        B3_MM(cre, na[p][0], na[p][1], b[q][2], b[q][3])  // - Ai Bi
+        B3_MM(cim, a[p][0], a[p][1], b[q][2], b[q][3])    // + Ar Bi
+        B3_MM(cim, a[p][2], a[p][3], b[q][0], b[q][1])    // + Ai Br
+#undef B3_MM
+        if (s + 1 < ns) store(cur ^ 1);
+        __syncthreads();
+    }
+    float2* out = (float2*)J.out;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + wr * 64 + p * 16 + 4 * lg + v;
+            if (i >= n) continue;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int j = j0 + wc * 32 + q * 16 + lr;
+                if (j >= n) continue;
+                float2* o = out + (long)i * n + j;
+                float ox = 0.f, oy = 0.f;
+                if (J.acc) {
+                    const float2 old = gload(o);
+                    ox = old.x;
+                    oy = old.y;
+                }
+                const float re = cre[p][q][v], im = cim[p][q][v];
+                gstore(o, J.sub ? make_float2(ox - re, oy - im) : make_float2(ox + re, oy + im));
+            }
+        }
+}
+
+void launch_gemm_b3(hipStream_t s, const GemmJob* jobs_d, int njobs, int n) {
+    const dim3 grid((n + B3_TN - 1) / B3_TN, (n + B3_TM - 1) / B3_TM, njobs);
+    k_gemm_b3<<<grid, 512, 0, s>>>(jobs_d, n);
+}
+
+// ---------------------------------------------------------------------------
 // Batched dense matrix-vector step of the reduced solve.  The gathered input
 // vectors sit in LDS; each wave computes two rows at a time, four columns
 // per lane per trip (eight loads in flight per lane).

@@ -114,10 +114,52 @@ int ensure_solve_work(Handle* h, long N) {
     return FDFD_OK;
 }
 
-static void free_factor(Factor* F) {
+void* arena_take(Handle* h, size_t bytes) {
+    int best = -1;
+    for (int i = 0; i < (int)h->arenas.size(); ++i)
+        if (h->arenas[i].second >= bytes && (best < 0 || h->arenas[i].second < h->arenas[best].second))
+            best = i;
+    if (best >= 0) {
+        void* p = h->arenas[best].first;
+        h->arenas.erase(h->arenas.begin() + best);
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {  // retry once without the pooled arenas
+        (void)hipGetLastError();
+        arena_clear(h);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    return p;
+}
+
+void arena_give(Handle* h, void* p, size_t bytes) {
+    if (!p) return;
+    h->arenas.push_back({p, bytes});
+    while ((int)h->arenas.size() > ARENA_POOL) {  // drop the smallest
+        int sm = 0;
+        for (int i = 1; i < (int)h->arenas.size(); ++i)
+            if (h->arenas[i].second < h->arenas[sm].second) sm = i;
+        (void)hipFree(h->arenas[sm].first);
+        h->arenas.erase(h->arenas.begin() + sm);
+    }
+}
+
+void arena_clear(Handle* h) {
+    for (auto& a : h->arenas) (void)hipFree(a.first);
+    h->arenas.clear();
+}
+
+static void free_factor(Handle* h, Factor* F) {
     if (!F) return;
+    (void)hipStreamSynchronize(h->stream);  // the arena may be reused by the next factor
     dfree(F->coef); dfree(F->sxf); dfree(F->sxb); dfree(F->syf); dfree(F->syb);
-    dfree(F->blocks); dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
+    arena_give(h, F->blocks, F->blocks_bytes);
+    arena_give(h, F->blocks_b, F->blocks_b_bytes);
+    dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
     dfree(F->mv_tab); dfree(F->dg_tab);
     if (F->solve_exec) (void)hipGraphExecDestroy(F->solve_exec);
     delete F;
@@ -335,6 +377,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.gemm3m = (e3 && e3[0] == '0') ? 0 : 1;
     const char* ew = std::getenv("FDFD_GEMM_WAVES");
     hh->impl.gemm_waves = (ew && ew[0] == '4') ? 4 : 8;
+    const char* eb = std::getenv("FDFD_MIXED_GEMM");
+    hh->impl.gemm_b3 = (eb && std::strcmp(eb, "fp32") == 0) ? 0 : 1;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;
@@ -348,7 +392,8 @@ int fdfd_destroy(fdfd_handle* hh) {
     if (!h) return FDFD_ERR_INVALID;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    for (auto& kv : h->factors) free_factor(kv.second);
+    for (auto& kv : h->factors) free_factor(h, kv.second);
+    arena_clear(h);
     void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
                     h->dpart, h->rpart, h->hdev};
@@ -369,8 +414,9 @@ int fdfd_set_grid(fdfd_handle* hh, int Nx, int Ny, double dx, double dy, int npm
     if (std::min(Nx, Ny) > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "min(Nx, Ny) must be <= 1024");
     (void)hipSetDevice(h->device);
-    for (auto& kv : h->factors) free_factor(kv.second);
+    for (auto& kv : h->factors) free_factor(h, kv.second);
     h->factors.clear();
+    arena_clear(h);
     dfree(h->eps);
     h->eps = nullptr;
     h->g = GridDesc{Nx, Ny, dx, dy, npml_x, npml_y, L0};
@@ -429,7 +475,7 @@ int fdfd_factor(fdfd_handle* hh, int pol, double omega, int precision, int* fact
         stats->pivot_fallback = F->pivot_fallback;
     }
     if (rc) {
-        free_factor(F);
+        free_factor(h, F);
         return rc;
     }
     F->id = h->next_id++;
@@ -446,7 +492,7 @@ int fdfd_release_factor(fdfd_handle* hh, int factor_id) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->factors.erase(factor_id);
-    free_factor(F);
+    free_factor(h, F);
     return FDFD_OK;
 }
 
@@ -677,7 +723,7 @@ int rt_gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, doub
     if (resid) *resid = st.resid;
     return rc;
 }
-void rt_free_factor(Factor* F) { free_factor(F); }
+void rt_free_factor(Handle* h, Factor* F) { free_factor(h, F); }
 int rt_norm(Handle* h, const double2* x, long N, double* out) { return dev_norm(h, x, N, out); }
 int rt_multidot(Handle* h, const double2* V, long N, int k, const double2* w, cd* out) {
     return dev_multidot(h, V, N, k, w, out);

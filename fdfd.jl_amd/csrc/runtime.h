@@ -44,6 +44,8 @@ struct Factor {
     size_t elem = 16;              // bytes per complex factor entry
     void* blocks = nullptr;        // nblocks * n * n complex (double2 or float2)
     int nblocks = 0;
+    void* blocks_b = nullptr;      // bf16x3 mode: bf16 planes of GEMM operands, 8 B per entry
+    size_t blocks_bytes = 0, blocks_b_bytes = 0;
     int *prow = nullptr, *iprow = nullptr;  // m * n Gauss-Jordan pivot maps per line
     double* pivstat = nullptr;     // 3 per line
     struct SolveStep { int kind, first, count; };  // kind 0: MvJob range, 1: DiagJob range
@@ -75,6 +77,7 @@ struct Handle {
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
     int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)
+    int gemm_b3 = 1;                   // mixed path: bf16x3 GEMM products (FDFD_MIXED_GEMM=fp32: FP32)
     // sweep / Krylov work
     long work_N = 0;
     double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors
@@ -86,6 +89,9 @@ struct Handle {
     double2* hdev = nullptr;    // small device vector of coefficients
     int nblk = 0;
     Prof prof;
+    // released factor arenas kept for reuse (at most ARENA_POOL), freed by
+    // fdfd_set_grid and fdfd_destroy
+    std::vector<std::pair<void*, size_t>> arenas;
 };
 
 // error helper: record message, return code
@@ -137,6 +143,14 @@ struct ProfScope {
         }
     }
 };
+
+// factor arenas: take one of at least `bytes` from the pool or allocate;
+// give one back (the pool keeps the largest ARENA_POOL); free them all
+constexpr int ARENA_POOL = 2;
+void* arena_take(Handle* h, size_t bytes);
+void arena_give(Handle* h, void* p, size_t bytes);
+void arena_clear(Handle* h);
+void rt_free_factor(Handle* h, Factor* F);
 
 // work buffers of the handle (grown on demand)
 int ensure_factor_work(Handle* h, int n, int nch);

@@ -87,6 +87,34 @@ def test_linsolve_matches_oracle(engine, Nx, Ny, npml, pol, precision):
         engine.release(fid)
 
 
+@pytest.mark.parametrize("gemm", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("Nx,Ny,npml", [(96, 80, 8), (72, 100, 6)])
+def test_mixed_gemm_modes_match_oracle(monkeypatch, gemm, Nx, Ny, npml):
+    """Both GEMM kernels of the mixed path (complex64 factors with FP32 MFMA
+    products, or bf16x3 products) give the oracle's solution through the FP64
+    refinement, on grids deep enough for several dense reduction levels."""
+    monkeypatch.setenv("FDFD_MIXED_GEMM", gemm)  # read when the handle is created
+    eng = Engine(0)
+    try:
+        d = _device(Nx, Ny, npml, seed=3 + Nx)
+        _engine_grid(eng, d)
+        om = float(d.omega[0])
+        A = O.assemble(d.grid, d.eps_r, om, O.TM)
+        b = O.rhs(om, d.src)
+        xo, _ = O.linear_solve(A, b)
+        fid, _ = eng.factor(O.TM, om, "mixed")
+        x, st = eng.linsolve(fid, b, tol=1e-13)
+        eng.release(fid)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        print(f"mixed gemm={gemm} {Nx}x{Ny}: err={err:.2e} res={res:.2e} its={st['iterations']}",
+              flush=True)
+        assert res <= 1e-12
+        assert err <= FIELD_TOL
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("pol", [O.TM, O.TE])
 def test_config_c1_fields(engine, precision, pol):
