@@ -110,7 +110,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
     const Plan P = make_plan(m);
     // the QR work of a first-level line lives in its channel's panel buffer
-    if ((size_t)tri::work_doubles(n) * sizeof(double) > (size_t)n * PMAX * elem)
+    if ((size_t)tri::work_doubles(n) * sizeof(double) > (size_t)n * VMAX * elem)
         return fail(h, FDFD_ERR_INVALID, "internal: QR work does not fit the panel buffer");
     int rc;
     if ((rc = ensure_factor_work(h, n, std::max(1, P.max_gj * B)))) return rc;
@@ -189,7 +189,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.prow = F->prow + (long)s.line * n;
                     J.iprow = F->iprow + (long)s.line * n;
                     J.pivstat = F->pivstat + 3 * (long)s.line;
-                    J.work = (double*)((char*)h->Vbuf + (size_t)c * n * PMAX * elem);
+                    J.work = (double*)((char*)h->Vbuf + (size_t)c * n * VMAX * elem);
                     J.si = F->si;
                     tris.push_back(J);
                 } else if (st.kind == ST_FORM) {
@@ -216,10 +216,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.used = h->used + (long)c * n;
                     J.prow = F->prow + (long)j * n;
                     J.iprow = F->iprow + (long)j * n;
-                    J.Vbuf = (char*)h->Vbuf + (size_t)c * n * PMAX * elem;
+                    J.Vbuf = (char*)h->Vbuf + (size_t)c * n * VMAX * elem;
                     J.LU = (char*)h->LU + (size_t)c * 2 * PMAX * PMAX * elem;
-                    J.piv = h->piv + c * PMAX;
-                    J.Z = (char*)h->Zb + (size_t)c * PMAX * n * elem;
+                    J.piv = h->piv + c * VMAX;
+                    J.Z = (char*)h->Zb + (size_t)c * VMAX * n * elem;
                     J.win = h->win + c * WIN;
                     J.verr = h->verr;
                     J.pivstat = F->pivstat + 3 * (long)j;
@@ -301,39 +301,60 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             } else {
                 const LineJob* J = T.lj + ol;
                 launch_gjreset(st, J, nj, n);
+                // Panels of NB columns.  With the windowed search, two panels are
+                // factored before one trailing pass applies both (the second panel's
+                // columns and pivot rows are brought up to date by the thin update
+                // and the Z build); the full search runs single panels.
                 for (int k0 = 0; k0 < n;) {
-                    int nb;
+                    int nb1, nb2 = 0;
                     const int fresh = std::min(n, WIN + k0);
                     {
                         ProfScope* ps = scope(P_PANEL, 0, 0);
                         if (!full) {
-                            nb = std::min(NB, n - k0);
-                            if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb, fresh);
-                            else launch_wpanel<float>(st, J, nj, n, k0, nb, fresh);
+                            nb1 = std::min(NB, n - k0);
+                            if (fp64) launch_wpanel<double>(st, J, nj, n, k0, nb1, fresh, 0);
+                            else launch_wpanel<float>(st, J, nj, n, k0, nb1, fresh, 0);
                         } else {
                             // full-search panel width: powers of two (16 columns of
                             // complex128 per thread would not fit 1024 threads' registers)
-                            nb = fp64 ? 8 : 16;
-                            while (nb > n - k0) nb >>= 1;
-                            if (fp64) launch_panel<double>(st, J, nj, n, k0, nb);
-                            else launch_panel<float>(st, J, nj, n, k0, nb);
+                            nb1 = fp64 ? 8 : 16;
+                            while (nb1 > n - k0) nb1 >>= 1;
+                            if (fp64) launch_panel<double>(st, J, nj, n, k0, nb1);
+                            else launch_panel<float>(st, J, nj, n, k0, nb1);
                         }
                         delete ps;
                     }
                     {
                         ProfScope* ps = scope(P_ZBUILD, 0, 0);
-                        if (fp64) launch_zbuild<double>(st, J, nj, n, k0, nb, fresh, !full);
-                        else launch_zbuild<float>(st, J, nj, n, k0, nb, fresh, !full);
+                        if (fp64) launch_zbuild<double>(st, J, nj, n, k0, nb1, fresh, !full, 0, 0, 0);
+                        else launch_zbuild<float>(st, J, nj, n, k0, nb1, fresh, !full, 0, 0, 0);
+                        delete ps;
+                    }
+                    if (!full && nb1 == PMAX && k0 + nb1 < n && h->fuse_panels) {
+                        const int k2 = k0 + nb1;
+                        nb2 = std::min(NB, n - k2);
+                        const int fresh2 = std::min(n, WIN + k2);
+                        ProfScope* ps = scope(P_ZBUILD, 0, 0);
+                        if (fp64) launch_thin<double>(st, J, nj, n, k0, nb1, nb2);
+                        else launch_thin<float>(st, J, nj, n, k0, nb1, nb2);
+                        delete ps;
+                        ps = scope(P_PANEL, 0, 0);
+                        if (fp64) launch_wpanel<double>(st, J, nj, n, k2, nb2, fresh2, 1);
+                        else launch_wpanel<float>(st, J, nj, n, k2, nb2, fresh2, 1);
+                        delete ps;
+                        ps = scope(P_ZBUILD, 0, 0);
+                        if (fp64) launch_zbuild<double>(st, J, nj, n, k2, nb2, fresh2, 1, 1, k0, nb1);
+                        else launch_zbuild<float>(st, J, nj, n, k2, nb2, fresh2, 1, 1, k0, nb1);
                         delete ps;
                     }
                     {
-                        ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * nb * nj,
+                        ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * (nb1 + nb2) * nj,
                                               2.0 * n * (double)n * elem * nj);
-                        if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb);
-                        else launch_trailing<float>(st, J, nj, n, k0, nb);
+                        if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb1, nb2);
+                        else launch_trailing<float>(st, J, nj, n, k0, nb1, nb2);
                         delete ps;
                     }
-                    k0 += nb;
+                    k0 += nb1 + nb2;
                 }
                 ol += nj;
             }

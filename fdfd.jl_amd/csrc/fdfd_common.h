@@ -96,10 +96,10 @@ struct LineJob {
     int* used;        // n: row already chosen as a pivot
     int* prow;        // n: prow[c] = pivot row of column c
     int* iprow;       // n: inverse of prow (-1 until pivoted)
-    void* Vbuf;       // n x PMAX panel copy (original panel values)
-    void* LU;         // 2 x PMAX x PMAX: unit-lower L then upper U, V[piv, :] = L U
-    int* piv;         // PMAX: pivot rows of the current panel
-    void* Z;          // PMAX x n: (L U)^-1 * [A(piv, :) | I]
+    void* Vbuf;       // n x VMAX panel copies V (two panels of a fused pair)
+    void* LU;         // 2 x PMAX x PMAX: inverse of the pivot block, then its U factor
+    int* piv;         // VMAX: pivot rows of the two panels
+    void* Z;          // VMAX x n: V[piv, :]^-1 * [A(piv, :) | I] of the two panels
     int* win;         // 64: candidate-row window (first unpivoted rows by index)
     int* verr;        // set to 1 when a row outside the window beats a chosen pivot
     double* pivstat;  // [0] = min |pivot|, [1] = max |pivot|, [2] = zero-pivot flag
@@ -224,12 +224,15 @@ template <typename R>
 void launch_panel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
 template <typename R>
 void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
-                   int fresh);
+                   int fresh, int second);
 template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
-                   int fresh, int verify);
+                   int fresh, int verify, int second, int k1, int nb1);
 template <typename R>
-void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);
+void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1, int nb1, int nb2);
+template <typename R>
+void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
+                     int nb2);
 template <typename R>
 void launch_form(hipStream_t s, const FormJob* jobs_d, int njobs, int n);
 void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n);
@@ -248,8 +251,11 @@ void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2
 
 // largest line-block size the panel kernel supports (one row per thread)
 constexpr int MAX_LINE = 1024;
-// widest Gauss-Jordan panel; row stride of the panel copy and of L, U
+// widest Gauss-Jordan panel (L, U, pivot-block inverse are PMAX x PMAX)
 constexpr int PMAX = 32;
+// two panels are applied in one trailing pass: the panel copy V is n x VMAX,
+// Z is VMAX x n, pivots VMAX (first panel at 0, second at PMAX)
+constexpr int VMAX = 2 * PMAX;
 // rows in the pivot-candidate window of the fast panel (one wave)
 constexpr int WIN = 64;
 // threshold of the windowed pivot search: a window pivot is accepted when no

@@ -115,7 +115,8 @@ __device__ inline void finish_pivot_block(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
 // updated window list.
 // ---------------------------------------------------------------------------
 template <typename R, int NB>
-__global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh) {
+__global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh,
+                                                int second) {
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.x];
     const int n = J.n;
@@ -130,12 +131,16 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     const int row = (lane < nwin) ? gload_s(J.win + lane) : -1;
     bool act = row >= 0;
     int mystep = -1;  // panel step at which this lane's row became the pivot
-    const C* A = (const C*)J.Y;
+    // panel values: the block's columns, or (second panel of a pair) the
+    // columns brought up to date by the first panel, in the panel buffer
+    const C* src = second ? (const C*)J.Vbuf + PMAX : (const C*)J.Y + k0;
+    const long ld = second ? VMAX : n;
+    const int poff = second ? PMAX : 0;
     C W[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         W[c] = mk<C>(R(0), R(0));
-        if (act && c < nb) W[c] = gload(A + (long)row * n + k0 + c);
+        if (act && c < nb) W[c] = gload(src + (long)row * ld + c);
     }
     static_for<0, NB>([&](auto ic) {
         constexpr int cc = decltype(ic)::value;
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     finish_pivot_block<C>(Ls, Us, Xs, Ud, nb, lane, 64, (C*)J.LU);
     if (lane < nb) {
         const int p = prs[lane];
-        J.piv[lane] = p;
+        J.piv[poff + lane] = p;
         J.prow[k0 + lane] = p;
         J.iprow[p] = k0 + lane;
         J.used[p] = 1;
@@ -212,12 +217,12 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
 
 template <typename R>
 void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
-                   int fresh) {
+                   int fresh, int second) {
     (void)n;
-    k_wpanel<R, PMAX><<<njobs, 64, 0, s>>>(jobs_d, k0, nb, fresh);
+    k_wpanel<R, PMAX><<<njobs, 64, 0, s>>>(jobs_d, k0, nb, fresh, second);
 }
-template void launch_wpanel<double>(hipStream_t, const LineJob*, int, int, int, int, int);
-template void launch_wpanel<float>(hipStream_t, const LineJob*, int, int, int, int, int);
+template void launch_wpanel<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
+template void launch_wpanel<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Full panel (fallback, exact GEPP over every unpivoted row): one workgroup of
@@ -336,66 +341,93 @@ template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, in
 // Z build, panel copy and pivot check (one thread per column or row).
 //   blocks [0, nzb):  Z[:, col] = Vpinv T[:, col], T[k][col] = A[piv[k]][col]
 //                     (delta(k, col-k0) for col in K), Vpinv = V[piv, :]^-1.
-//   blocks [nzb, ..): copy V = A[r, K] to Vbuf (every row r) and, if verify,
-//                     check rows r >= fresh: the value GEPP would compare at
-//                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
-//                     y_c = a_c / U[c][c], must satisfy
+//   blocks [nzb, ..): copy V = A[r, K] to the panel buffer (every row r) and,
+//                     if verify, check rows r >= fresh: the value GEPP would
+//                     compare at step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c]
+//                     with y_c = a_c / U[c][c], must satisfy
 //                     PIV_U * |a_c| <= |U[c][c]| (threshold pivoting).
+// Second panel of a fused pair (columns K2 after K1 = [k1, k1 + nb1)): its
+// pivot rows are taken after the first panel's update,
+//   T[k][col] = [col not in K1] A[p2_k][col] - sum_q V1[p2_k][q] Z1[q][col],
+// Z goes to rows PMAX.. of the Z buffer, and the panel values (already in the
+// panel buffer, columns PMAX..) are only checked.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
-                                                int verify, int nzb) {
+                                                int verify, int nzb, int second, int k1,
+                                                int nb1) {
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
     __shared__ C Vi[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
+    __shared__ C V1p[PMAX][PMAX + 1];  // second panel: V1 rows of its pivots
     __shared__ C Ud[PMAX];
     __shared__ int pv[PMAX];
     const C* LUg = (const C*)J.LU;
+    const int poff = second ? PMAX : 0;
     for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
         Vi[e / PMAX][e % PMAX] = gload(LUg + e);
         Us[e / PMAX][e % PMAX] = gload(LUg + PMAX * PMAX + e);
     }
-    if ((int)threadIdx.x < nb) pv[threadIdx.x] = gload_s(J.piv + threadIdx.x);
+    if ((int)threadIdx.x < nb) pv[threadIdx.x] = gload_s(J.piv + poff + threadIdx.x);
     __syncthreads();
     if ((int)threadIdx.x < nb) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
+    const C* Vb = (const C*)J.Vbuf;
+    if (second)
+        for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
+            const int k = e / PMAX, q = e % PMAX;
+            C v = mk<C>(R(0), R(0));
+            if (q < nb1) v = gload(Vb + (long)pv[k] * VMAX + q);
+            V1p[k][q] = v;
+        }
     __syncthreads();
     const C* A = (const C*)J.Y;
+    C* Zg = (C*)J.Z;
     if ((int)blockIdx.x < nzb) {
         const int col = blockIdx.x * blockDim.x + threadIdx.x;
         if (col >= n) return;
         const bool inK = col >= k0 && col < k0 + nb;
+        const bool inK1 = second && col >= k1 && col < k1 + nb1;
         C t[PMAX];
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
             t[k] = mk<C>(R(0), R(0));
             if (k < nb) {
                 if (inK) t[k].x = (col - k0 == k) ? R(1) : R(0);
-                else t[k] = gload(A + (long)pv[k] * n + col);
+                else if (!inK1) t[k] = gload(A + (long)pv[k] * n + col);
             }
         }
-        C* Zg = (C*)J.Z;
+        if (second && !inK) {
+            for (int q = 0; q < nb1; ++q) {
+                const C z1 = gload(Zg + (long)q * n + col);
+#pragma unroll
+                for (int k = 0; k < PMAX; ++k)
+                    if (k < nb) t[k] = csub(t[k], cmul(V1p[k][q], z1));
+            }
+        }
         for (int k = 0; k < nb; ++k) {
             C z = mk<C>(R(0), R(0));
 #pragma unroll
             for (int q = 0; q < PMAX; ++q)
                 if (q < nb) z = cadd(z, cmul(Vi[k][q], t[q]));
-            gstore(Zg + (long)k * n + col, z);
+            gstore(Zg + (long)(poff + k) * n + col, z);
         }
     } else {
         const int r = ((int)blockIdx.x - nzb) * blockDim.x + threadIdx.x;
         if (r >= n) return;
         C v[PMAX];
+        C* Vr = (C*)J.Vbuf + (long)r * VMAX + poff;
 #pragma unroll
         for (int c = 0; c < PMAX; ++c) {
             v[c] = mk<C>(R(0), R(0));
-            if (c < nb) v[c] = gload(A + (long)r * n + k0 + c);
+            if (c < nb) v[c] = second ? gload(Vr + c) : gload(A + (long)r * n + k0 + c);
         }
-        C* Vb = (C*)J.Vbuf + (long)r * PMAX;
+        if (!second) {
 #pragma unroll
-        for (int c = 0; c < PMAX; ++c)
-            if (c < nb) gstore(Vb + c, v[c]);
+            for (int c = 0; c < PMAX; ++c)
+                if (c < nb) gstore(Vr + c, v[c]);
+        }
         if (verify && r >= fresh) {
             int bad = 0;
 #pragma unroll
@@ -416,13 +448,72 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
 
 template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
-                   int fresh, int verify) {
+                   int fresh, int verify, int second, int k1, int nb1) {
     const int nzb = (n + 255) / 256;
     const int nvb = (n + 255) / 256;
-    k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb);
+    k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb,
+                                                        second, k1, nb1);
 }
-template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
-template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);
+template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int, int, int,
+                                    int, int, int);
+template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int, int, int,
+                                   int, int, int);
+
+// ---------------------------------------------------------------------------
+// Thin update of a fused pair: the second panel's columns K2 = [k1 + nb1,
+// k1 + nb1 + nb2) after the first panel, one thread per row:
+//   V2[r] = Z1[slot, K2]                        for the first panel's pivots,
+//   V2[r] = A[r, K2] - V1[r] Z1[:, K2]           for every other row,
+// into columns PMAX.. of the panel buffer.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int nb1, int nb2) {
+    using C = cplx<R>;
+    const LineJob J = jobs[blockIdx.y];
+    const int n = J.n;
+    __shared__ C Z1k[PMAX][PMAX + 1];  // Z1[q][K2 column c]
+    const C* Zg = (const C*)J.Z;
+    const int k2 = k1 + nb1;
+    for (int e = threadIdx.x; e < PMAX * PMAX; e += blockDim.x) {
+        const int q = e / PMAX, c = e % PMAX;
+        C v = mk<C>(R(0), R(0));
+        if (q < nb1 && c < nb2) v = gload(Zg + (long)q * n + k2 + c);
+        Z1k[q][c] = v;
+    }
+    __syncthreads();
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    C* Vr = (C*)J.Vbuf + (long)r * VMAX;
+    const int slot = gload_s(J.iprow + r) - k1;
+    C w[PMAX];
+    if (slot >= 0 && slot < nb1) {
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c) w[c] = Z1k[slot][c];
+    } else {
+        const C* A = (const C*)J.Y + (long)r * n + k2;
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c) {
+            w[c] = mk<C>(R(0), R(0));
+            if (c < nb2) w[c] = gload(A + c);
+        }
+        for (int q = 0; q < nb1; ++q) {
+            const C v1 = gload(Vr + q);
+#pragma unroll
+            for (int c = 0; c < PMAX; ++c) w[c] = csub(w[c], cmul(v1, Z1k[q][c]));
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < PMAX; ++c)
+        if (c < nb2) gstore(Vr + PMAX + c, w[c]);
+}
+
+template <typename R>
+void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1, int nb1,
+                 int nb2) {
+    k_thin<R><<<dim3((n + 255) / 256, njobs), 256, 0, s>>>(jobs_d, k1, nb1, nb2);
+}
+template void launch_thin<double>(hipStream_t, const LineJob*, int, int, int, int, int);
+template void launch_thin<float>(hipStream_t, const LineJob*, int, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Trailing update on MFMA.  64x64 output tile per block, 4 waves of 32x32,
@@ -434,8 +525,17 @@ template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, i
 // overlaps the operand staging.
 // ---------------------------------------------------------------------------
 
+// Trailing update of one panel (nb2 = 0, columns K = [k0, k0 + nb1)):
+//   A[i, :] = A'[i, :] - V[i, :] Z  (A' = A with K zeroed),  A[piv, :] = Z;
+// or of a fused pair (K1 = [k0, k0 + PMAX), K2 = the next nb2 columns):
+//   rows of neither panel:  A'' - V1 Z1' - V2 Z2   (A'' zeroes K1 and K2,
+//                                                  Z1' = Z1 with K2 zeroed)
+//   first-panel pivots:     Z1'[slot] - V2 Z2
+//   second-panel pivots:    Z2[slot]
+// The rank-(nb1 + nb2) product runs in chunks of PMAX columns of V through
+// the same LDS tiles.
 template <typename R>
-__global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, int nb) {
+__global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, int nb1, int nb2) {
     using C = cplx<R>;
     using M = mfma_t<R>;
     using acc_t = typename M::acc;
@@ -446,12 +546,17 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     const int lr = lane & 15, lk = lane >> 4;
+    const bool fused = nb2 > 0;
+    const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
+    const int k2 = k0 + PMAX;                               // start of K2 (fused)
     __shared__ R Vr[64][NB + 1], Vi[64][NB + 1];
     __shared__ R Zr[NB][64 + 1], Zi[NB][64 + 1];
+    __shared__ int rslot[64];
     C* A = (C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
     const C* Zg = (const C*)J.Z;
-    // old tile values (zero for the panel columns) and the pivot-row slots
+    if (t < 64) rslot[t] = (i0 + t < n) ? gload_s(J.iprow + i0 + t) - k0 : -1;
+    // old tile values and the pivot-row slots
     C old[2][4][2];
     int slot[2][4];
 #pragma unroll
@@ -459,31 +564,23 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = i0 + wr * 32 + bi * 16 + M::row(lane, v);
-            slot[bi][v] = (i < n) ? J.iprow[i] - k0 : -1;
+            const int sl = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
+            slot[bi][v] = sl;
+            const bool p1 = fused && sl >= 0 && sl < PMAX;
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) {
                 const int j = j0 + wc * 32 + bj * 16 + lr;
-                const bool ld = i < n && j < n && !(j >= k0 && j < k0 + nb);
                 old[bi][v][bj].x = R(0);
                 old[bi][v][bj].y = R(0);
-                if (ld) old[bi][v][bj] = gload(A + (long)i * n + j);
+                if (i < n && j < n) {
+                    if (p1) {
+                        if (!(j >= k2 && j < kend)) old[bi][v][bj] = gload(Zg + (long)sl * n + j);
+                    } else if (!(j >= k0 && j < kend)) {
+                        old[bi][v][bj] = gload(A + (long)i * n + j);
+                    }
+                }
             }
         }
-    for (int e = t; e < 64 * NB; e += 256) {
-        const int ii = e / NB, k = e % NB, i = i0 + ii;
-        C v = mk<C>(R(0), R(0));
-        if (i < n && k < nb) v = gload(Vb + (long)i * PMAX + k);
-        Vr[ii][k] = v.x;
-        Vi[ii][k] = v.y;
-    }
-    for (int e = t; e < 64 * NB; e += 256) {
-        const int k = e / 64, jj = e % 64, j = j0 + jj;
-        C v = mk<C>(R(0), R(0));
-        if (j < n && k < nb) v = gload(Zg + (long)k * n + j);
-        Zr[k][jj] = v.x;
-        Zi[k][jj] = v.y;
-    }
-    __syncthreads();
     acc_t are[2][2], aim[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -492,25 +589,50 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
             are[bi][bj] = acc_t{0, 0, 0, 0};
             aim[bi][bj] = acc_t{0, 0, 0, 0};
         }
-    const int nbr = (nb + 3) & ~3;
-    for (int kk = 0; kk < nbr; kk += 4) {
-        R ar[2], ai[2], br[2], bi_[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            ar[q] = Vr[wr * 32 + q * 16 + lr][kk + lk];
-            ai[q] = Vi[wr * 32 + q * 16 + lr][kk + lk];
-            br[q] = Zr[kk + lk][wc * 32 + q * 16 + lr];
-            bi_[q] = Zi[kk + lk][wc * 32 + q * 16 + lr];
+    __syncthreads();  // rslot
+    const int nchunk = fused ? 2 : 1;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int koff = ch * PMAX, nbc = (ch == 0) ? nb1 : nb2;
+        if (ch > 0) __syncthreads();  // the previous chunk's tiles are consumed
+        for (int e = t; e < 64 * NB; e += 256) {
+            const int ii = e / NB, k = e % NB, i = i0 + ii;
+            C v = mk<C>(R(0), R(0));
+            // the first panel's pivot rows take no V1 part in a fused pair
+            const bool skip = fused && ch == 0 && rslot[ii] >= 0 && rslot[ii] < PMAX;
+            if (i < n && k < nbc && !skip) v = gload(Vb + (long)i * VMAX + koff + k);
+            Vr[ii][k] = v.x;
+            Vi[ii][k] = v.y;
         }
+        for (int e = t; e < 64 * NB; e += 256) {
+            const int k = e / 64, jj = e % 64, j = j0 + jj;
+            C v = mk<C>(R(0), R(0));
+            // Z1' : Z1 with the K2 columns zeroed
+            const bool skip = fused && ch == 0 && j >= k2 && j < kend;
+            if (j < n && k < nbc && !skip) v = gload(Zg + (long)(koff + k) * n + j);
+            Zr[k][jj] = v.x;
+            Zi[k][jj] = v.y;
+        }
+        __syncthreads();
+        const int nbr = (nbc + 3) & ~3;
+        for (int kk = 0; kk < nbr; kk += 4) {
+            R ar[2], ai[2], br[2], bi_[2];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
-                are[bi][bj] = M::run(ar[bi], br[bj], are[bi][bj]);
-                are[bi][bj] = M::run(-ai[bi], bi_[bj], are[bi][bj]);
-                aim[bi][bj] = M::run(ar[bi], bi_[bj], aim[bi][bj]);
-                aim[bi][bj] = M::run(ai[bi], br[bj], aim[bi][bj]);
+            for (int q = 0; q < 2; ++q) {
+                ar[q] = Vr[wr * 32 + q * 16 + lr][kk + lk];
+                ai[q] = Vi[wr * 32 + q * 16 + lr][kk + lk];
+                br[q] = Zr[kk + lk][wc * 32 + q * 16 + lr];
+                bi_[q] = Zi[kk + lk][wc * 32 + q * 16 + lr];
             }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    are[bi][bj] = M::run(ar[bi], br[bj], are[bi][bj]);
+                    are[bi][bj] = M::run(-ai[bi], bi_[bj], are[bi][bj]);
+                    aim[bi][bj] = M::run(ar[bi], bi_[bj], aim[bi][bj]);
+                    aim[bi][bj] = M::run(ai[bi], br[bj], aim[bi][bj]);
+                }
+        }
     }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -519,14 +641,16 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
             const int i = i0 + wr * 32 + bi * 16 + M::row(lane, v);
             if (i >= n) continue;
             const int sl = slot[bi][v];
-            const bool pivrow = (sl >= 0 && sl < nb);
+            // rows replaced by Z: the panel's pivots (single) or the second
+            // panel's pivots (fused)
+            const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj) {
                 const int jj = wc * 32 + bj * 16 + lr;
                 const int j = j0 + jj;
                 if (j >= n) continue;
                 C out;
-                if (pivrow) out = gload(Zg + (long)sl * n + j);
+                if (zrow) out = gload(Zg + (long)sl * n + j);
                 else out = mk<C>(old[bi][v][bj].x - are[bi][bj][v], old[bi][v][bj].y - aim[bi][bj][v]);
                 gstore(A + (long)i * n + j, out);
             }
@@ -534,12 +658,13 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
 }
 
 template <typename R>
-void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb) {
+void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
+                     int nb2) {
     dim3 grid((n + 63) / 64, (n + 63) / 64, njobs);
-    k_trailing<R><<<grid, 256, 0, s>>>(jobs_d, k0, nb);
+    k_trailing<R><<<grid, 256, 0, s>>>(jobs_d, k0, nb1, nb2);
 }
-template void launch_trailing<double>(hipStream_t, const LineJob*, int, int, int, int);
-template void launch_trailing<float>(hipStream_t, const LineJob*, int, int, int, int);
+template void launch_trailing<double>(hipStream_t, const LineJob*, int, int, int, int, int);
+template void launch_trailing<float>(hipStream_t, const LineJob*, int, int, int, int, int);
 
 
 }  // namespace fdfd

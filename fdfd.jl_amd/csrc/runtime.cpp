@@ -74,13 +74,13 @@ int ensure_factor_work(Handle* h, int n, int nch) {
     dfree(h->win); dfree(h->verr);
     h->used = h->piv = h->win = h->verr = nullptr;
     h->Vbuf = h->LU = h->Zb = nullptr;
-    const int NBmax = PMAX;
+    const int NBmax = VMAX;
     HIPCHK(h, hipMalloc(&h->used, sizeof(int) * nch * n));
     HIPCHK(h, hipMalloc(&h->piv, sizeof(int) * nch * NBmax));
     HIPCHK(h, hipMalloc(&h->win, sizeof(int) * nch * WIN));
     HIPCHK(h, hipMalloc(&h->verr, sizeof(int)));
     HIPCHK(h, hipMalloc(&h->Vbuf, sizeof(double2) * nch * (size_t)n * NBmax));
-    HIPCHK(h, hipMalloc(&h->LU, sizeof(double2) * nch * 2 * NBmax * NBmax));
+    HIPCHK(h, hipMalloc(&h->LU, sizeof(double2) * nch * 2 * PMAX * PMAX));
     HIPCHK(h, hipMalloc(&h->Zb, sizeof(double2) * nch * (size_t)NBmax * n));
     h->work_n = n;
     h->work_ch = nch;
@@ -379,6 +379,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.gemm_waves = (ew && ew[0] == '4') ? 4 : 8;
     const char* eb = std::getenv("FDFD_MIXED_GEMM");
     hh->impl.gemm_b3 = (eb && std::strcmp(eb, "fp32") == 0) ? 0 : 1;
+    const char* ef = std::getenv("FDFD_FUSE_PANELS");
+    hh->impl.fuse_panels = (ef && ef[0] == '0') ? 0 : 1;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;
