@@ -216,20 +216,32 @@ def main():
     if not kd["launches"]:  # --profile-steps 0: no per-kernel events were recorded
         kd = {"ms": 0.0, "launches": 1, "flops": 0.0, "bytes": 0.0}
     avg_s = kd["ms"] / kd["launches"] / 1e3
-    if kd["flops"] > 0 and dom == "gemm":
-        # algorithmic complex-GEMM flops (8 n^3 per job).  The default mixed
-        # path forms each real product from three bf16 MFMA products, so its
-        # ceiling is the bf16 dense peak / 3; FP32 / FP64 MFMA otherwise.
-        achieved = kd["flops"] / kd["launches"] / avg_s
-        if head == "mixed":
-            peak, pnote = PEAK["bf16"] / 3, "bf16 dense MFMA peak / 3 (bf16x3 products)"
+    # roofline of the dominant kernel: the tighter of its MFMA ceiling
+    # (algorithmic flops / matrix peak) and its HBM ceiling (algorithmic
+    # bytes / 8 TB/s) is its bound; frac = achieved / that peak
+    if head == "mixed":
+        if dom == "gemm":
+            # each real product of the default mixed GEMM is formed from
+            # three bf16 MFMA products, so its ceiling is the bf16 peak / 3
+            mpeak, pnote = PEAK["bf16"] / 3, "bf16 dense MFMA peak / 3 (bf16x3 products)"
         else:
-            peak, pnote = PEAK["fp64"], "FP64 dense MFMA peak"
-        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None, "kernel": dom,
+            mpeak, pnote = PEAK["fp32"], "FP32 dense MFMA peak"
+    else:
+        mpeak, pnote = PEAK["fp64"], "FP64 dense MFMA peak"
+    per_launch_f = kd["flops"] / kd["launches"]
+    per_launch_b = kd["bytes"] / kd["launches"]
+    t_m = per_launch_f / mpeak
+    t_b = per_launch_b / PEAK["hbm"]
+    if per_launch_f > 0 and t_m >= t_b:
+        achieved = per_launch_f / avg_s
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": mpeak / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / mpeak, "traffic": None, "kernel": dom,
                 "peak_note": pnote}
-    elif kd["bytes"] > 0:
-        achieved = kd["bytes"] / kd["launches"] / avg_s
+        if per_launch_b > 0:
+            roof["hbm_achieved_GBps"] = per_launch_b / avg_s / 1e9
+            roof["flop_per_byte"] = per_launch_f / per_launch_b
+    elif per_launch_b > 0:
+        achieved = per_launch_b / avg_s
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
                 "unit": "GB/s", "frac": achieved / PEAK["hbm"], "traffic": None, "kernel": dom}
     else:
@@ -243,10 +255,13 @@ def main():
     if os.path.exists(pmc):
         with open(pmc) as fh:
             P = json.load(fh)
-        prefix = {"gemm": "k_gemm<", "trailing": "k_trailing<", "solve": "k_mv<"}.get(dom)
         rt = "float" if head == "mixed" else "double"
+        # kernel-name prefix of the dominant slot (the mixed path's GEMM is
+        # the split-operand bf16x3 kernel)
+        prefix = {"gemm": "k_gemm_b3" if head == "mixed" else "k_gemm<" + rt,
+                  "trailing": "k_trailing<" + rt, "solve": "k_mv<" + rt}.get(dom)
         for name, v in P["kernels"].items():
-            if prefix and name.startswith(prefix + rt):
+            if prefix and name.startswith(prefix):
                 roof["traffic"] = v["read_bytes"] + v["write_bytes"]
                 roof["traffic_unit"] = "bytes/launch"
                 roof["traffic_source"] = os.path.relpath(pmc, ROOT)
