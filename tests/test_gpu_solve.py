@@ -115,6 +115,40 @@ def test_mixed_gemm_modes_match_oracle(monkeypatch, gemm, Nx, Ny, npml):
         eng.close()
 
 
+@pytest.mark.parametrize("switch", ["FDFD_FUSE_PANELS=0", "FDFD_GEMM3M=0", "FDFD_GEMM_WAVES=4",
+                                    "FDFD_GRAPHS=0"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_handle_switches_match_oracle(monkeypatch, switch, precision):
+    """Every kernel variant selectable at handle creation (one panel per
+    trailing pass, four-multiplication GEMM, four-wave GEMM workgroups, no
+    graph replay) gives the oracle's solution, plain and transposed."""
+    key, val = switch.split("=")
+    monkeypatch.setenv(key, val)  # read when the handle is created
+    eng = Engine(0)
+    try:
+        d = _device(72, 100, 6, seed=11)
+        _engine_grid(eng, d)
+        om = float(d.omega[0])
+        A = O.assemble(d.grid, d.eps_r, om, O.TM)
+        b = O.rhs(om, d.src)
+        fid, _ = eng.factor(O.TM, om, precision)
+        try:
+            for tr in (False, True):
+                ref, _ = O.linear_solve(A, b, trans="T" if tr else "N")
+                x, st = eng.linsolve(fid, b, transpose=tr, tol=1e-13)
+                Aop = A.T if tr else A
+                err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+                res = np.linalg.norm(b - Aop @ x) / np.linalg.norm(b)
+                print(f"{switch} {precision} tr={tr}: err={err:.2e} res={res:.2e} "
+                      f"its={st['iterations']}", flush=True)
+                assert res <= 1e-12
+                assert err <= FIELD_TOL
+        finally:
+            eng.release(fid)
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("pol", [O.TM, O.TE])
 def test_config_c1_fields(engine, precision, pol):
