@@ -257,6 +257,35 @@ TRI_HD inline void apply_q(int n, const double* w, int b, int kstart, Put put) {
     put(n - 1, tail);
 }
 
+// y = Q v for a general vector v (get(k) = v[k]); y written through
+// put(k, y[k]) in increasing k.  put may overwrite v[k] in place: v[k] is
+// read before y[k] is written, and never after.
+template <typename Get, typename Put>
+TRI_HD inline void apply_q_vec(int n, const double* w, Get get, Put put) {
+    if (n == 1) {
+        put(0, get(0));
+        return;
+    }
+    cz carry = get(0), tail = get(n - 1);
+    for (int k = 0; k + 2 < n; ++k) {
+        cz nxt = get(k + 1);
+        double c;
+        cz s;
+        get_rot(w, k, 0, c, s);
+        rot(c, s, carry, nxt);
+        get_rot(w, k, 1, c, s);
+        rot(c, s, carry, tail);
+        put(k, carry);
+        carry = nxt;
+    }
+    double c;
+    cz s;
+    get_rot(w, n - 2, 0, c, s);
+    rot(c, s, carry, tail);
+    put(n - 2, carry);
+    put(n - 1, tail);
+}
+
 template <typename Get, typename Put>
 TRI_HD inline void apply_rinv(int n, const double* w, int b, Get get, Put put) {
     const int k0 = first_nonzero(n, b);

@@ -26,6 +26,9 @@ see tests/test_oracle.py):
                                      npml=0 uniform eps; dense zgesv tiny case
   field recovery ................... closed-form differences of the solved field
   eigenfrequency ................... Kronecker-sum closed form (uniform eps, PML)
+  1-D eigenmode (mode sources) ..... closed-form spectrum (uniform eps, TM and TE);
+                                     lattice slab dispersion relations (TM and TE)
+                                     derived independently; hand-computed slices
   flux_surface_integral ............ lossless energy conservation between planes
   adjoint gradient ................. central finite differences
 """
@@ -235,6 +238,7 @@ class Device:
     omega: np.ndarray
     eps_r: np.ndarray = None
     src: np.ndarray = None
+    modes: list = dc_field(default_factory=list)  # src/device.jl:24
 
     def __post_init__(self):
         self.omega = np.atleast_1d(np.asarray(self.omega, dtype=np.float64))
@@ -355,13 +359,19 @@ def _stack(g: Grid, *vs):
 
 
 def solve(d: Device, pol: int = TM, return_system: bool = False):
-    """src/solver/driven.jl:4-59 (mode sources out of scope)."""
+    """src/solver/driven.jl:4-59.  With modes present the source is reset and
+    rebuilt from the TM modes at every frequency (driven.jl:15-19; the mode
+    polarization is always TM there)."""
     g = d.grid
     eps0, mu0, _ = normalize_parameters(g.L0)
     out = []
     systems = []
     for omega in d.omega:
         omega = float(omega)
+        if d.modes:
+            d.src = np.zeros(g.N, dtype=np.complex128)
+            for mode in d.modes:
+                setup_mode(d, TM, omega, mode.neff, mode.pt, mode.dir, mode.width)
         A = assemble(g, d.eps_r, omega, pol)
         b = rhs(omega, d.src)
         x, lu = linear_solve(A, b)
@@ -381,6 +391,100 @@ def solve(d: Device, pol: int = TM, return_system: bool = False):
         systems.append((A, b, lu))
     res = out[0] if len(out) == 1 else out
     return (res, systems) if return_system else res
+
+
+# ---------------------------------------------------------------------------
+# mode sources: src/device.jl:5-11,113-149 and the 1-D eigenmode of
+# src/solver/eigen.jl:6-29
+# ---------------------------------------------------------------------------
+@dataclass
+class Mode:
+    """src/device.jl:5-11.  pt is an (x, y) pair."""
+    pol: int
+    dir: int
+    neff: complex
+    pt: tuple
+    width: float
+
+
+def add_mode(d: Device, mode: Mode):
+    """src/device.jl:113-115."""
+    d.modes.append(mode)
+
+
+def mode_operator_1d(eps1d: np.ndarray, dh: float, omega: float, pol: int,
+                     L0: float = DEFAULT_L0):
+    """eigen.jl:7-21 on the 1-D device of get_modes (device.jl:144-146):
+    Grid(n, [0 n*dh]) has dx = (n*dh - 0)/n (grid.jl:53-58,68-70) and no PML;
+    periodic differences (grid.jl:138,141).
+      TM: A = w^2 mu0 Te + dxf dxb
+      TE: A = w^2 mu0 Te + Te dxf Texinv dxb
+    with Te = diag(eps0 eps), Texinv = diag(1/(eps0 grid_average(eps)))."""
+    eps1d = np.asarray(eps1d, dtype=np.complex128)
+    n = eps1d.size
+    eps0, mu0, _ = normalize_parameters(L0)
+    dx = (n * dh - 0.0) / n
+    dxf, dxb = _d1(n, dx, FORWARD), _d1(n, dx, BACKWARD)
+    Te = sp.diags(eps0 * eps1d)
+    if pol == TM:
+        return omega ** 2 * mu0 * Te + dxf @ dxb
+    Texinv = sp.diags((eps0 * grid_average(eps1d, XHAT)) ** -1)
+    return omega ** 2 * mu0 * Te + Te @ dxf @ Texinv @ dxb
+
+
+def eigenmode_1d(eps1d: np.ndarray, dh: float, omega: float, pol: int, neff, nev: int,
+                 L0: float = DEFAULT_L0):
+    """eigen.jl:6-29: (beta^2, vectors) = eigs(A, nev, sigma=(w/c0 neff)^2),
+    beta = sqrt(beta^2).  Shift-invert ARPACK with which=:LM returns the nev
+    eigenvalues nearest sigma; here they come from a dense eigendecomposition
+    (exact selection, the slices are small), nearest first.  Vectors have unit
+    2-norm with their largest-magnitude entry real and positive (ARPACK's
+    phase is arbitrary)."""
+    _, _, c0 = normalize_parameters(L0)
+    A = mode_operator_1d(eps1d, dh, omega, pol, L0).toarray()
+    sigma = (omega / c0 * complex(neff)) ** 2
+    w, V = np.linalg.eig(A)
+    order = np.argsort(np.abs(w - sigma), kind="stable")[:nev]
+    V = V[:, order]
+    for k in range(nev):
+        v = V[:, k] / np.linalg.norm(V[:, k])
+        i = int(np.argmax(np.abs(v)))
+        V[:, k] = v * (np.conj(v[i]) / abs(v[i]))
+    return np.sqrt(w[order].astype(np.complex128)), V, sigma
+
+
+def slice_indices(g: Grid, pt, slicenormal: int, slicewidth: float):
+    """device.jl:125-142: 1-based indices of the odd-length slice through pt
+    and its cell size.  x-normal slices run along y (width / dy points), y-normal
+    slices along x; an even count is made odd by adding one."""
+    indx, indy = g.coord2ind(*pt)
+    srcpoints = int(round(slicewidth / (g.dy if slicenormal == XHAT else g.dx)))
+    if srcpoints % 2 == 0:
+        srcpoints += 1
+    M = (srcpoints - 1) // 2
+    if slicenormal == XHAT:
+        return np.array([indx]), indy + np.arange(-M, M + 1), g.dy
+    return indx + np.arange(-M, M + 1), np.array([indy]), g.dx
+
+
+def get_modes(d: Device, pol: int, omega: float, neff, nmodes: int, pt, slicenormal: int,
+              slicewidth: float):
+    """device.jl:124-149.  Returns (beta, vectors, indx, indy) with 1-based
+    index arrays; a slice leaving the grid raises (Julia BoundsError)."""
+    ix, iy, dh = slice_indices(d.grid, pt, slicenormal, slicewidth)
+    if ix.min() < 1 or iy.min() < 1 or ix.max() > d.grid.N[0] or iy.max() > d.grid.N[1]:
+        raise IndexError("mode slice leaves the grid")
+    eps1d = d.eps_r[np.ix_(ix - 1, iy - 1)].ravel()
+    beta, vecs, _ = eigenmode_1d(eps1d, dh, omega, pol, neff, nmodes, d.grid.L0)
+    return beta, vecs, ix, iy
+
+
+def setup_mode(d: Device, pol: int, omega: float, neff, pt, srcnormal: int, srcwidth: float):
+    """device.jl:118-121: src[slice] += normalize(abs.(v)) of the mode nearest
+    the guess."""
+    _, v, ix, iy = get_modes(d, pol, omega, neff, 1, pt, srcnormal, srcwidth)
+    a = np.abs(v[:, 0])
+    d.src[np.ix_(ix - 1, iy - 1)] += (a / np.linalg.norm(a)).reshape(ix.size, iy.size)
 
 
 # ---------------------------------------------------------------------------

@@ -37,3 +37,18 @@ def test_tridiagonal_qr_inverse(tmp_path):
     lines = out.stdout.strip().splitlines()
     assert len(lines) == 34
     assert out.returncode == 0, "\n".join(l for l in lines if "FAIL" in l)
+
+
+def test_small_dense_eigensolver(tmp_path):
+    """tests/small_eig_check.cpp runs small_eig.h (the projected-matrix
+    eigensolver of both Arnoldi drivers, host and device) on random complex
+    matrices (eigenpair residual <= 1e-12) and on matrices of prescribed
+    spectrum, including a repeated eigenvalue (every eigenvalue to 1e-11)."""
+    exe = tmp_path / "small_eig_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "fdfd.jl_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "small_eig_check.cpp"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    lines = out.stdout.strip().splitlines()
+    assert len(lines) == 30
+    assert out.returncode == 0, "\n".join(l for l in lines if "FAIL" in l)

@@ -327,3 +327,116 @@ def test_adjoint_gradient_matches_fd():
         em[k] -= h
         fd = (J(ep)[0] - J(em)[0]) / (2 * h)
         assert abs(grad[kk] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(grad).max())
+
+
+# --- mode sources: 1-D eigenmode (eigen.jl:6-29), get_modes / setup_mode! ----
+L0 = O.DEFAULT_L0
+K0SQ = lambda om: om ** 2 * O.EPS0 * L0 * O.MU0 * L0  # noqa: E731
+
+
+@pytest.mark.parametrize("pol", [O.TM, O.TE])
+@pytest.mark.parametrize("n", [37, 40])
+def test_mode_operator_uniform_spectrum(pol, n):
+    """Uniform eps: A is circulant with eigenvalues
+    k0^2 eps - (4/dx^2) sin^2(pi p / n), p = 0..n-1, for TM and TE alike
+    (the TE factors eps/avg(eps) are 1).  Complex eps included."""
+    om = O.omega_of_wavelength(1.55)
+    eps = 4.0 + 0.3j
+    dh = 0.02
+    A = O.mode_operator_1d(np.full(n, eps), dh, om, pol).toarray()
+    dx = (n * dh) / n
+    p = np.arange(n)
+    ref = K0SQ(om) * eps - 4.0 / dx ** 2 * np.sin(np.pi * p / n) ** 2
+    got = np.linalg.eigvals(A)
+    err = np.abs(np.sort_complex(got) - np.sort_complex(ref)).max() / np.abs(ref).max()
+    assert err <= 1e-12, err
+
+
+def _slab(a: int, clad: int, ec: float, ecl: float):
+    """2a+1 core cells (eps ec) centred in 2*clad cladding cells (eps ecl)."""
+    return np.concatenate([np.full(clad, ecl), np.full(2 * a + 1, ec), np.full(clad, ecl)])
+
+
+def _slab_roots(g, lo, hi, nscan=4000):
+    xs = np.linspace(lo, hi, nscan + 2)[1:-1]
+    vals = np.array([g(x) for x in xs])
+    roots = []
+    for i in range(len(xs) - 1):
+        if np.sign(vals[i]) != np.sign(vals[i + 1]):
+            from scipy.optimize import brentq
+            roots.append(brentq(g, xs[i], xs[i + 1], xtol=1e-15 * hi, rtol=1e-15))
+    return roots
+
+
+@pytest.mark.parametrize("pol", [O.TM, O.TE])
+def test_mode_slab_lattice_dispersion(pol):
+    """Even guided modes of a symmetric slab on the lattice, derived without
+    the matrix: core f_i = cos(theta i) with 2cos(theta) - 2 = dx^2 (b2 - k0^2 ec),
+    cladding f_{a+k} = D r^k with r + 1/r - 2 = dx^2 (b2 - k0^2 ecl), |r| < 1.
+    The two rows at the interface fix b2:
+      TM: r cos(theta a) = cos(theta (a+1))
+      TE: with em = (ec+ecl)/2 (grid_average), rho = ecl/em,
+          D = rho f_a / (1 + rho r - r),
+          (ec/em)(D r - f_a) - (f_a - f_{a-1}) - (2cos(theta) - 2) f_a = 0.
+    Every root must be an eigenvalue of the restated operator, and
+    eigenmode_1d with a guess near it must return it.  The cladding is wide
+    enough that the periodic wrap is below round-off."""
+    om = O.omega_of_wavelength(1.55)
+    k2 = K0SQ(om)
+    a, clad, ec, ecl, dh = 7, 150, 12.25, 2.25, 0.02
+    dx = dh
+    em = 0.5 * (ec + ecl)
+    rho = ecl / em
+
+    def g(b2):
+        c = 1 + dx ** 2 * (b2 - k2 * ec) / 2
+        th = np.arccos(c)
+        q = 1 + dx ** 2 * (b2 - k2 * ecl) / 2
+        r = q - np.sqrt(q * q - 1)
+        fa, fa1, fp1 = np.cos(th * a), np.cos(th * (a - 1)), np.cos(th * (a + 1))
+        if pol == O.TM:
+            return r * fa - fp1
+        D = rho * fa / (1 + rho * r - r)
+        return (ec / em) * (D * r - fa) - (fa - fa1) - (2 * c - 2) * fa
+
+    roots = _slab_roots(g, k2 * ecl, k2 * ec)
+    assert len(roots) >= 1
+    eps1d = _slab(a, clad, ec, ecl)
+    lam = np.linalg.eigvals(O.mode_operator_1d(eps1d, dh, om, pol).toarray())
+    for b2 in roots:
+        assert np.abs(lam - b2).min() <= 1e-11 * b2
+        neff = np.sqrt(b2) / np.sqrt(k2) * (1 + 1e-4)
+        beta, v, _ = O.eigenmode_1d(eps1d, dh, om, pol, neff, 1)
+        assert abs(beta[0] ** 2 - b2) <= 1e-11 * b2
+        # the guided mode is even about the core centre
+        vv = np.abs(v[:, 0])
+        assert np.abs(vv - vv[::-1]).max() <= 1e-9
+
+
+def test_mode_slices_and_setup_mode():
+    """Slice indices of device.jl:125-142 by hand for the waveguide case of the
+    reference's tests (runtests.jl:23-27): Grid(0.02, [15 15], [0 10], [-1 1])
+    is 500 x 100; Point(1.0, 0) -> (51, 51); width 0.8 -> 40 points -> odd 41,
+    rows 31..71 of column 51.  y-normal: width 0.3 -> 15 points (odd), columns
+    44..58 of row 51.  setup_mode! adds a unit-norm, non-negative profile that
+    is even for a centred guide, and leaves every other cell alone."""
+    g = O.Grid.create(0.02, (15, 15), (0.0, 10.0), (-1.0, 1.0))
+    assert g.N == (500, 100)
+    ix, iy, dh = O.slice_indices(g, (1.0, 0.0), O.XHAT, 0.8)
+    assert list(ix) == [51] and list(iy) == list(range(31, 72)) and dh == g.dy
+    ix, iy, dh = O.slice_indices(g, (1.0, 0.0), O.YHAT, 0.3)
+    assert list(ix) == list(range(44, 59)) and list(iy) == [51] and dh == g.dx
+    d = O.Device(g, O.omega_of_wavelength(1.55))
+    d.eps_r[:, 43:58] = 12.0   # guide rows 44..58 (1-based), centred on row 51
+    d.src[3, 3] = 5.0
+    O.setup_mode(d, O.TM, float(d.omega[0]), 3.5, (1.0, 0.0), O.XHAT, 0.8)
+    prof = d.src[50, 30:71]
+    assert np.all(prof.real >= 0) and np.all(prof.imag == 0)
+    assert abs(np.linalg.norm(prof) - 1) <= 1e-14
+    assert np.abs(prof - prof[::-1]).max() <= 1e-9
+    rest = d.src.copy()
+    rest[50, 30:71] = 0
+    rest[3, 3] -= 5.0
+    assert not rest.any()
+    with pytest.raises(IndexError):
+        O.get_modes(d, O.TM, float(d.omega[0]), 3.5, 1, (1.0, -0.95), O.XHAT, 0.8)

@@ -2,7 +2,9 @@
 // (fdfd.jl_amd/csrc/tri_qr.h): for random complex blocks (indefinite
 // diagonals included) of every small size and a few large ones, the columns
 // formed exactly as the device kernel forms them must give T X = I to
-// round-off.  Prints one line per case; exit status 1 on failure.
+// round-off, and the solve of T x = v for a random v through apply_q_vec
+// must leave a residual at round-off.  Prints one line per case; exit
+// status 1 on failure.
 #include <complex>
 #include <cstdio>
 #include <random>
@@ -49,7 +51,21 @@ static double run(int n, double diag_scale, unsigned seed) {
             err = std::max(err, std::abs(s - (i == j ? 1.0 : 0.0)));
             xn = std::max(xn, std::abs(X[(size_t)i * n + j]));
         }
-    return err / std::max(1.0, xn);
+    // general right-hand side through apply_q_vec (in place) + apply_rinv:
+    // T x = v to round-off
+    std::vector<cz> v(n), x(n);
+    for (int i = 0; i < n; ++i) v[i] = mkz(U(rng), U(rng));
+    x = v;
+    apply_q_vec(n, w.data(), [&](int k) { return x[k]; }, [&](int k, cz y) { x[k] = y; });
+    apply_rinv(n, w.data(), 0, [&](int k) { return x[k]; }, [&](int k, cz y) { x[k] = y; });
+    double verr = 0, vn = 0;
+    for (int i = 0; i < n; ++i) {
+        cd s = 0;
+        for (int k = 0; k < n; ++k) s += T[(size_t)i * n + k] * cd(x[k].re, x[k].im);
+        verr = std::max(verr, std::abs(s - cd(v[i].re, v[i].im)));
+        vn = std::max(vn, std::abs(cd(v[i].re, v[i].im)));
+    }
+    return std::max(err / std::max(1.0, xn), verr / (vn * std::max(1.0, xn)));
 }
 
 int main() {
