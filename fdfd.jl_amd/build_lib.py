@@ -19,8 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfdfd.so")
 ARCH = "gfx950"
 
-SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "runtime.cpp", "bcr.cpp", "eigen.cpp",
-           "dense_eig.cpp", "batch.cpp"]
+SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "k_mode.hip", "runtime.cpp", "bcr.cpp",
+           "eigen.cpp", "dense_eig.cpp", "batch.cpp", "mode.cpp"]
 
 # All device code is built without packed FP32 instructions.
 HIPFLAGS = [

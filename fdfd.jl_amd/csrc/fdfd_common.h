@@ -192,6 +192,27 @@ struct DiagJob {
     long si;
 };
 
+// One 1-D eigenmode problem (src/solver/eigen.jl:6-29): the periodic
+// tridiagonal operator of the slice, shift sigma, Arnoldi storage and outputs.
+struct ModeJob {
+    const double2* eps;  // n relative permittivities along the slice
+    double2* V;          // (ncv + 1) * n Arnoldi basis
+    double2* W;          // ncv * n restart workspace
+    double* trw;         // factors of A - sigma I (tri::work_doubles(n))
+    double2* vec;        // out: nev eigenvectors, vector k at vec + k * n
+    double2* beta;       // out: nev propagation constants
+    int* info;           // out: status, converged pairs, restarts, OP applications
+    double2 sigma;       // (omega / c0' * neff)^2
+    double k0sq;         // omega^2 mu0' eps0'
+    double idx2;         // 1 / dx^2 of the 1-D grid
+    int n, pol;
+    int ncv;             // Arnoldi basis size (nev < ncv <= min(n, MODE_NCV))
+};
+// status values of ModeJob::info[0]
+enum { MODE_OK = 0, MODE_SINGULAR = 1, MODE_EIG_FAIL = 2 };
+// largest Arnoldi basis of the mode kernel (its projected matrices live in LDS)
+constexpr int MODE_NCV = 32;
+
 // ---------------------------------------------------------------------------
 // launch wrappers (implemented in the .hip files)
 // ---------------------------------------------------------------------------
@@ -248,6 +269,8 @@ void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);
 void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2* sxf,
                    const double2* syf, int Nx, int Ny, int inverse);
+void launch_mode(hipStream_t s, const ModeJob* jobs_d, int njobs, int nev, double tol,
+                 int maxiter);
 
 // largest line-block size the panel kernel supports (one row per thread)
 constexpr int MAX_LINE = 1024;

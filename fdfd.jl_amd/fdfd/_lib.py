@@ -37,6 +37,11 @@ class fdfd_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class fdfd_mode_problem(C.Structure):
+    _fields_ = [("n", C.c_int), ("pol", C.c_int), ("omega", C.c_double), ("dh", C.c_double),
+                ("neff", fdfd_complex), ("eps", C.c_void_p)]
+
+
 class FDFDError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
@@ -69,6 +74,8 @@ PROTOTYPES = {
     "fdfd_residual": (_I, [_P, _I, _P, _P, _PD]),
     "fdfd_adjoint_grad": (_I, [_P, _I, _P, _P, fdfd_complex, _P]),
     "fdfd_eigenfrequency": (_I, [_P, _I, _D, _I, _I, _I, _I, _D, _I, _P, _P, _PS]),
+    "fdfd_eigenmode": (_I, [_P, _I, C.POINTER(fdfd_mode_problem), _I, _I, _D, _D, _I, _P, _P,
+                            _PI, _PS]),
     "fdfd_batch_create": (_I, [_P, C.POINTER(_P)]),
     "fdfd_batch_add": (_I, [_P, _I, _I, _D, _D, _I, _I, _D, _I, _P, _I, _P, _I, _P, _PI]),
     "fdfd_batch_run": (_I, [_P, _I, _D, _PS]),

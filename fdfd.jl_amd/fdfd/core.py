@@ -107,6 +107,7 @@ class Device:
     omega: np.ndarray
     eps_r: np.ndarray = None
     src: np.ndarray = None
+    modes: list = field(default_factory=list)  # src/device.jl:24 (see modes.py)
 
     def __post_init__(self):
         self.omega = np.atleast_1d(np.asarray(self.omega, dtype=np.float64))

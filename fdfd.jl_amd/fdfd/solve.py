@@ -209,7 +209,9 @@ def solve(d: Device, pol: int = TM, precision="fp64", tol: float = 1e-10, eng: E
     """solve(d::Device, pol::Polarization=TM), src/solver/driven.jl:4-59.
 
     One factorization per frequency (driven.jl:11); returns a FieldTM/FieldTE,
-    or a list when d.omega has several entries (driven.jl:57-58)."""
+    or a list when d.omega has several entries (driven.jl:57-58).  With modes
+    present the source is rebuilt from them at every frequency
+    (driven.jl:15-19, modes.setup_modes_all)."""
     eng = eng or engine()
     eng.set_grid(d.grid)
     eng.set_eps(d.eps_r)
@@ -217,6 +219,9 @@ def solve(d: Device, pol: int = TM, precision="fp64", tol: float = 1e-10, eng: E
     for i, omega in enumerate(d.omega):
         log.info("Frequency: %d/%d", i + 1, len(d.omega))
         t0 = time.time()
+        if d.modes:
+            from .modes import setup_modes_all
+            setup_modes_all(d, float(omega), eng=eng)
         fid, fst = eng.factor(pol, float(omega), precision)
         try:
             f, sst = eng.solve_fields(fid, d.src, tol)

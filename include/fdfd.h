@@ -19,6 +19,8 @@
  *   solve(d::Device, pol)                 src/solver/driven.jl:4-59
  *   dolinearsolve(A, b, sym)  (lu(A)\b)   src/solver/solver.jl:4-41
  *   eigenfrequency(d, pol, nev; which)    src/solver/eigen.jl:69-115
+ *   eigenmode(d::Device{1}, pol, neff, n) src/solver/eigen.jl:6-29
+ *     (get_modes / setup_mode!)           src/device.jl:118-149
  *   S_create / create_sfactor             src/pml.jl:1-63
  *   delta / grid_average                  src/grid.jl:128-162
  *   FieldTM / FieldTE layout              src/data.jl:50-78
@@ -155,6 +157,44 @@ int fdfd_eigenfrequency(fdfd_handle* h, int pol, double omega0, int nev,
                         int maxiter, fdfd_complex* omegas, fdfd_complex* fields,
                         fdfd_stats* stats);
 
+/* ---- 1-D eigenmodes (mode sources) -------------------------------------- */
+/* One slice of eps, as get_modes builds it (src/device.jl:124-149): n cells of
+ * size dh along the slice, periodic, no PML. */
+typedef struct {
+    int n;                    /* cells of the slice, 3 .. 16384                  */
+    int pol;                  /* FDFD_TM or FDFD_TE                              */
+    double omega;             /* angular frequency (rad/s)                        */
+    double dh;                /* cell size along the slice, in units of L0        */
+    fdfd_complex neff;        /* effective-index guess                            */
+    const fdfd_complex* eps;  /* n relative permittivities along the slice (host) */
+} fdfd_mode_problem;
+
+/* eigenmode(d::Device{1}, pol, neff, nev) (src/solver/eigen.jl:6-29) for
+ * nprob slices at once, one workgroup each.  The 1-D grid is
+ * Grid(n, [0 n*dh]) (dx = (n*dh)/n); the operator is
+ *   TM: omega^2 mu0' eps0' diag(eps) + dxf dxb
+ *   TE: omega^2 mu0' eps0' diag(eps) + diag(eps) dxf diag(1/avg(eps)) dxb
+ * with avg = grid_average (src/grid.jl:157-158).  Shift-invert Arnoldi with
+ * sigma = (omega/c0' neff)^2 returns the nev eigenvalues beta^2 nearest
+ * sigma, nearest first, as beta = sqrt(beta^2) (principal branch).
+ *   ncv <= 0: max(20, 2 nev + 1) (the Arpack.jl default), at most 32, and
+ *             clamped to n per slice; nev < min(ncv, n) is required.
+ *   L0 <= 0: 1e-6.  tol <= 0: 1e-12 (Ritz residual relative to the Ritz
+ *             value of OP).  maxiter <= 0: 300 restarts.
+ *   beta:    nprob * nev, problem p at p * nev.
+ *   vectors: may be NULL; problem p's nev vectors start at nev * (n_0 + ..
+ *            + n_{p-1}), vector k at + k * n_p; unit 2-norm, largest entry
+ *            real and positive (ARPACK's phase is arbitrary; setup_mode! uses
+ *            abs.(v) only).
+ *   nconv:   may be NULL; nprob converged-pair counts.
+ * FDFD_ERR_SINGULAR if sigma is an eigenvalue of a slice's operator
+ * (stats.singular_line = that problem), FDFD_ERR_NOCONV if fewer than nev
+ * pairs converged (outputs still written). */
+int fdfd_eigenmode(fdfd_handle* h, int nprob, const fdfd_mode_problem* probs,
+                   int nev, int ncv, double L0, double tol, int maxiter,
+                   fdfd_complex* beta, fdfd_complex* vectors, int* nconv,
+                   fdfd_stats* stats);
+
 /* ---- batched driven solves (many grids x frequencies x sources) --------- */
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source
  * is solved at every frequency).  fdfd_batch_run factors all (job, omega)
@@ -175,7 +215,8 @@ int fdfd_batch_destroy(fdfd_batch* b);
 /* ---- tuning and profiling ----------------------------------------------- */
 /* Per-kernel device time accumulated with HIP events on the handle's stream
  * while profiling is enabled.  kernel: 0 = trailing GJE update, 1 = panel,
- * 2 = Schur formation, 3 = sweep GEMV, 4 = stencil/residual, 5 = Z build. */
+ * 2 = Schur formation, 3 = sweep GEMV, 4 = stencil/residual, 5 = Z build,
+ * 6 = complex GEMM. */
 int fdfd_profile_enable(fdfd_handle* h, int enable);
 int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
                       long long* launches, double* flops, double* bytes);

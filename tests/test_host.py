@@ -74,3 +74,26 @@ def test_poynting_tm_constant_fields():
     data[:, :, 2] = 3.0
     sx, sy = flux.poynting(core.FieldTM(g, 1.0, data))
     assert np.allclose(sx, -0.5 * 2.0 * 3.0) and np.allclose(sy, 0.0)
+
+
+@pytest.mark.parametrize("pt,normal,width", [
+    ((1.0, 0.0), core.XHAT, 0.8), ((1.0, 0.0), core.YHAT, 0.3), ((2.5, 0.31), core.XHAT, 0.61),
+    ((7.77, -0.5), core.YHAT, 1.0), ((1.0, 0.0), core.XHAT, 0.001),
+])
+def test_mode_slices_match_oracle(pt, normal, width):
+    """Slice indices of get_modes (src/device.jl:125-142): host mirror and
+    oracle agree, odd counts, the slice centred on coord2ind(pt)."""
+    from fdfd import modes
+    g = core.Grid(0.02, (15, 15), (0.0, 10.0), (-1.0, 1.0))
+    o = O.Grid.create(0.02, (15, 15), (0.0, 10.0), (-1.0, 1.0))
+    ix, iy, dh = modes.slice_indices(g, core.Point(*pt), normal, width)
+    oix, oiy, odh = O.slice_indices(o, pt, normal, width)
+    assert np.array_equal(ix, oix) and np.array_equal(iy, oiy) and dh == odh
+    assert (ix.size * iy.size) % 2 == 1
+
+
+def test_mode_slice_outside_grid_raises():
+    from fdfd import modes
+    g = core.Grid(0.02, (15, 15), (0.0, 10.0), (-1.0, 1.0))
+    with pytest.raises(IndexError):
+        modes.slice_indices(g, core.Point(1.0, -0.95), core.XHAT, 0.8)
