@@ -11,6 +11,7 @@ column-major array those constructors take, so no copy is made).
     using FDFD, FDFDAMD
     f = FDFDAMD.solve(dev, TM)               # replaces FDFD.solve (driven.jl:4)
     ω, fs = FDFDAMD.eigenfrequency(dev, TM, 8)  # replaces eigen.jl:69
+    β, v = FDFDAMD.eigenmode(dev1d, TM, 3.5, 1)  # replaces eigen.jl:6 (mode sources)
     FDFDAMD.install!()   # make FDFD.solve / FDFD.eigenfrequency use the GPU
 """
 module FDFDAMD
@@ -79,22 +80,141 @@ function set_problem!(h::Handle, d::FDFD.Device)
     return (Nx, Ny)
 end
 
+# mirrors fdfd_mode_problem in include/fdfd.h
+struct ModeProblem
+    n::Cint
+    pol::Cint
+    omega::Cdouble
+    dh::Cdouble
+    neff::ComplexF64
+    eps::Ptr{ComplexF64}
+end
+
+"""
+    eigenmode_slices(slices, dh, pol, ω, neff, nev; L₀, handle)
+
+1-D eigenmodes of several ε slices in one `fdfd_eigenmode` call (one GPU
+workgroup per slice).  Returns a vector of `(β, vectors)` per slice.
+"""
+function eigenmode_slices(slices::Vector{Vector{ComplexF64}}, dh::Vector{Float64},
+                          pol::Vector{Int}, ω::Vector{Float64}, neff::Vector{ComplexF64},
+                          nev::Int; L₀::Real=1e-6, ncv::Int=0, tol::Real=0.0, maxiter::Int=0,
+                          handle::Handle=default_handle())
+    h = handle
+    k = length(slices)
+    probs = [ModeProblem(length(slices[p]), pol[p], ω[p], dh[p], neff[p], pointer(slices[p]))
+             for p in 1:k]
+    ntot = sum(length, slices)
+    β = Vector{ComplexF64}(undef, k * nev)
+    vecs = Vector{ComplexF64}(undef, ntot * nev)
+    nconv = Vector{Cint}(undef, k)
+    st = Ref(Stats())
+    GC.@preserve slices probs β vecs nconv check(h, ccall((:fdfd_eigenmode, libfdfd), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{ModeProblem}, Cint, Cint, Cdouble, Cdouble, Cint,
+         Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{Cint}, Ref{Stats}),
+        h.ptr, k, probs, nev, ncv, L₀, tol, maxiter, β, vecs, nconv, st))
+    out = Tuple{Vector{ComplexF64},Matrix{ComplexF64}}[]
+    o = 0
+    for p in 1:k
+        n = length(slices[p])
+        push!(out, (β[(p - 1) * nev + 1:p * nev], reshape(vecs[o + 1:o + n * nev], n, nev)))
+        o += n * nev
+    end
+    return out
+end
+
+"""
+    eigenmode(d::AbstractDevice{1}, pol, neff, neigenvalues)
+
+Same contract as FDFD.eigenmode (src/solver/eigen.jl:6-29): the
+`neigenvalues` propagation constants β whose β² are nearest (ω/c₀·neff)², and
+their eigenvectors (columns; unit 2-norm, largest entry real and positive).
+"""
+function eigenmode(d::FDFD.AbstractDevice{1}, pol::FDFD.Polarization, neff::Number,
+                   neigenvalues::Int; handle::Handle=default_handle())
+    eps = Vector{ComplexF64}(d.ϵᵣ[:])
+    (β, v) = eigenmode_slices([eps], [Float64(FDFD.dx(d.grid))], [Int(pol)], [Float64(d.ω[1])],
+                              [ComplexF64(neff)], neigenvalues; L₀=d.grid.L₀, handle=handle)[1]
+    return (β, v)
+end
+
+# src/device.jl:125-142: indices of the odd-length slice through pt and its cell size
+function slice_indices(g::FDFD.Grid, pt, slicenormal, slicewidth::Number)
+    (indx, indy) = FDFD.coord2ind(g, pt)
+    srcpoints = slicenormal == FDFD.x̂ ? Int(round(slicewidth / FDFD.dy(g))) :
+                                        Int(round(slicewidth / FDFD.dx(g)))
+    srcpoints % 2 == 0 && (srcpoints += 1)
+    M = (srcpoints - 1) ÷ 2
+    if slicenormal == FDFD.x̂
+        return (indx:indx, indy .+ (-M:M), FDFD.dy(g))
+    else
+        return (indx .+ (-M:M), indy:indy, FDFD.dx(g))
+    end
+end
+
+"""
+    get_modes(d, pol, ω, neff, nmodes, pt, slicenormal, slicewidth)
+
+Same contract as FDFD.get_modes (src/device.jl:124-149): (β, vectors, indx, indy).
+"""
+function get_modes(d::FDFD.AbstractDevice, pol::FDFD.Polarization, ω::Real, neff::Number,
+                   nmodes::Int, pt, slicenormal, slicewidth::Number;
+                   handle::Handle=default_handle())
+    (ix, iy, dh) = slice_indices(d.grid, pt, slicenormal, slicewidth)
+    eps = Vector{ComplexF64}(d.ϵᵣ[ix, iy][:])
+    (β, v) = eigenmode_slices([eps], [Float64(dh)], [Int(pol)], [Float64(ω)],
+                              [ComplexF64(neff)], nmodes; L₀=d.grid.L₀, handle=handle)[1]
+    return (β, v, length(ix) == 1 ? ix[1] : ix, length(iy) == 1 ? iy[1] : iy)
+end
+
+"""
+    setup_mode!(d, pol, ω, neff, pt, srcnormal, srcwidth)
+
+Same contract as FDFD.setup_mode! (src/device.jl:118-121): adds
+normalize(abs.(v)) of the mode nearest the guess to `d.src` on the slice.
+"""
+function setup_mode!(d::FDFD.AbstractDevice, pol::FDFD.Polarization, ω::Real, neff::Number,
+                     pt, srcnormal, srcwidth::Number; handle::Handle=default_handle())
+    (_, v, indx, indy) = get_modes(d, pol, ω, neff, 1, pt, srcnormal, srcwidth; handle=handle)
+    a = abs.(v[:, 1])
+    d.src[indx, indy] += a ./ sqrt(sum(abs2, a))
+    return nothing
+end
+
+# driven.jl:15-19: the source of one frequency from the device's modes (TM,
+# as the reference passes), all slices in one batched call
+function mode_source!(d::FDFD.Device, ω::Real, handle::Handle)
+    d.src = zeros(Complex, size(d.grid))
+    isempty(d.modes) && return nothing
+    sl = [slice_indices(d.grid, m.pt, m.dir, m.width) for m in d.modes]
+    res = eigenmode_slices([Vector{ComplexF64}(d.ϵᵣ[ix, iy][:]) for (ix, iy, _) in sl],
+                           [Float64(dh) for (_, _, dh) in sl], fill(Int(FDFD.TM), length(sl)),
+                           fill(Float64(ω), length(sl)), [ComplexF64(m.neff) for m in d.modes],
+                           1; L₀=d.grid.L₀, handle=handle)
+    for ((ix, iy, _), (_, v)) in zip(sl, res)
+        a = abs.(v[:, 1])
+        d.src[ix, iy] .+= reshape(a ./ sqrt(sum(abs2, a)), length(ix), length(iy))
+    end
+    return nothing
+end
+
 """
     solve(d::Device, pol::Polarization=TM; precision=:fp64, tol=1e-10)
 
 Same contract as FDFD.solve (src/solver/driven.jl:4-59): one factorization
 per frequency in `d.ω`, returns a `FieldTM`/`FieldTE` or a vector of them.
-Mode sources (`d.modes`) are not supported by this backend.
+With mode sources (`d.modes`) the source is rebuilt from the TM modes at every
+frequency (driven.jl:15-19).
 """
 function solve(d::FDFD.Device, pol::FDFD.Polarization=FDFD.TM; precision::Symbol=:fp64,
                tol::Real=1e-10, handle::Handle=default_handle())
-    isempty(d.modes) || throw(ArgumentError("mode sources are not supported by FDFDAMD"))
     h = handle
     (Nx, Ny) = set_problem!(h, d)
-    src = Array{ComplexF64}(d.src)
     fields = pol == FDFD.TM ? FDFD.FieldTM[] : FDFD.FieldTE[]
     for (i, ω) in enumerate(d.ω)
         @info "Frequency: $i/$(length(d.ω))"
+        isempty(d.modes) || mode_source!(d, ω, h)
+        src = Array{ComplexF64}(d.src)
         fid = Ref{Cint}(0)
         st = Ref(Stats())
         check(h, ccall((:fdfd_factor, libfdfd), Cint,
@@ -141,11 +261,14 @@ end
 """
     install!()
 
-Route `FDFD.solve(::Device, ::Polarization)` and `FDFD.eigenfrequency` to
-this backend (method replacement in the FDFD module).
+Route `FDFD.solve(::Device, ::Polarization)`, `FDFD.eigenfrequency` and the
+1-D `FDFD.eigenmode` (so `setup_mode!` / `get_modes` as well) to this backend
+(method replacement in the FDFD module).
 """
 function install!()
     @eval FDFD solve(d::Device, pol::Polarization=TM) = $(solve)(d, pol)
+    @eval FDFD eigenmode(d::AbstractDevice{1}, pol::Polarization, neff::Number, n::Int) =
+        $(eigenmode)(d, pol, neff, n)
     @eval FDFD eigenfrequency(d::AbstractDevice, pol::Polarization, n::Int; which::Symbol=:LM) =
         $(eigenfrequency)(d, pol, n; which=which)
     return nothing
