@@ -468,13 +468,17 @@ static void launch_solve(Handle* h, Factor* F, bool prof) {
     }
 }
 
-// out = A'^-1 in, or A'^-T in = W^-1 A'^-1 W in (W = diag(sxf sxf)).  The
+// out = A'^-1 in, or A'^-T in = W^-1 A'^-1 W in (W below).  The
 // launch sequence of a factor is captured once in a HIP graph and replayed
 // (direct launches while per-kernel profiling is on).
 int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out) {
     const GridDesc& g = h->g;
     const long N = (long)g.Nx * g.Ny;
-    if (trans) launch_wscale(h->stream, h->x_out, in, F->sxf, F->syf, g.Nx, g.Ny, 0);
+    // A'^T = W^-1 A' W with W = sxf (x) syf (driven form) or sxb (x) syb
+    // (sideband form): the periodic differences satisfy dxf^T = -dxb
+    const double2* wx = F->form == 1 ? F->sxb : F->sxf;
+    const double2* wy = F->form == 1 ? F->syb : F->syf;
+    if (trans) launch_wscale(h->stream, h->x_out, in, wx, wy, g.Nx, g.Ny, 0);
     else
         HIPCHK(h, hipMemcpyAsync(h->x_out, in, sizeof(double2) * N, hipMemcpyDeviceToDevice,
                                  h->stream));
@@ -496,7 +500,7 @@ int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* o
         HIPCHK(h, hipGraphLaunch(F->solve_exec, h->stream));
     }
     HIPCHK(h, hipGetLastError());
-    if (trans) launch_wscale(h->stream, out, h->x_out, F->sxf, F->syf, g.Nx, g.Ny, 1);
+    if (trans) launch_wscale(h->stream, out, h->x_out, wx, wy, g.Nx, g.Ny, 1);
     else
         HIPCHK(h, hipMemcpyAsync(out, h->x_out, sizeof(double2) * N, hipMemcpyDeviceToDevice,
                                  h->stream));

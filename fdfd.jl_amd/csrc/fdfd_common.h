@@ -220,7 +220,11 @@ void launch_sfactor(hipStream_t s, double2* sxf, double2* sxb, double2* syf, dou
                     const GridDesc& g, double omega, double eps0n, double eta0);
 void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const double2* sxf,
                    const double2* sxb, const double2* syf, const double2* syb,
-                   const GridDesc& g, int pol, double kdiag);
+                   const GridDesc& g, int pol, double kdiag, int form);
+// y[j] += sign * c[j] * (conj(d) x[j+1] + d x[j-1]) for the nf sideband
+// blocks of N cells (modulation.jl:95-99); c has nf entries
+void launch_mf_couple(hipStream_t s, double2* y, const double2* x, const double2* d,
+                      const double* c, int nf, long N, double sign);
 void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef,
                   int Nx, int Ny, int transpose, const double2* b /*nullable: r=b-Ax*/);
 void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n);

@@ -72,7 +72,7 @@ void launch_sfactor(hipStream_t s, double2* sxf, double2* sxb, double2* syf, dou
 // ---------------------------------------------------------------------------
 __global__ void k_coeffs(double2* coef, const double2* eps, const double2* sxf,
                          const double2* sxb, const double2* syf, const double2* syb, int Nx,
-                         int Ny, double ry, int pol, double kdiag) {
+                         int Ny, double ry, int pol, double kdiag, int form) {
     long N = (long)Nx * Ny;
     long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (k >= N) return;
@@ -81,7 +81,15 @@ __global__ void k_coeffs(double2* coef, const double2* eps, const double2* sxf,
     int jp = (j + 1 == Ny) ? 0 : j + 1, jm = (j == 0) ? Ny - 1 : j - 1;
     double2 cE, cW, cN, cS, cC;
     double2 ryc = make_double2(ry, 0.0);
-    if (pol == 1) {
+    if (pol == 1 && form == 1) {
+        // sideband operator of the modulated solve, Sxb dxb Sxf dxf + Syb dyb
+        // Syf dyf (modulation.jl:83,88): sxb_i [sxf_i (e_E - e_C) - sxf_{i-1} (e_C - e_W)]
+        cE = cmul(sxb[i], sxf[i]);
+        cW = cmul(sxb[i], sxf[im]);
+        cN = cmul(ryc, cmul(syb[j], syf[j]));
+        cS = cmul(ryc, cmul(syb[j], syf[jm]));
+        cC = cadd(cneg(cadd(cadd(cE, cW), cadd(cN, cS))), cscale(eps[k], kdiag));
+    } else if (pol == 1) {
         cE = cmul(sxf[i], sxb[ip]);
         cW = cmul(sxf[i], sxb[i]);
         cN = cmul(ryc, cmul(syf[j], syb[jp]));
@@ -109,11 +117,34 @@ __global__ void k_coeffs(double2* coef, const double2* eps, const double2* sxf,
 
 void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const double2* sxf,
                    const double2* sxb, const double2* syf, const double2* syb,
-                   const GridDesc& g, int pol, double kdiag) {
+                   const GridDesc& g, int pol, double kdiag, int form) {
     long N = (long)g.Nx * g.Ny;
     double ry = (g.dx * g.dx) / (g.dy * g.dy);
     k_coeffs<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(coef, eps, sxf, sxb, syf, syb, g.Nx,
-                                                         g.Ny, ry, pol, kdiag);
+                                                         g.Ny, ry, pol, kdiag, form);
+}
+
+// Sideband coupling of the modulated operator (modulation.jl:95-99):
+// block (j, j+1) = 0.5 wn_j^2 conj(T_deps), block (j+1, j) = 0.5 wn_{j+1}^2 T_deps,
+// so row block j gets c_j (conj(d) x_{j+1} + d x_{j-1}), c_j = scale 0.5 wn_j^2 eps0'.
+__global__ void k_mf_couple(double2* y, const double2* x, const double2* d, const double* c,
+                            int nf, long N, double sign) {
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const double2 dk = d[k];
+    if (dk.x == 0.0 && dk.y == 0.0) return;
+    for (int j = 0; j < nf; ++j) {
+        double2 acc = make_double2(0.0, 0.0);
+        if (j + 1 < nf) acc = cadd(acc, cmul(make_double2(dk.x, -dk.y), x[(long)(j + 1) * N + k]));
+        if (j > 0) acc = cadd(acc, cmul(dk, x[(long)(j - 1) * N + k]));
+        const long o = (long)j * N + k;
+        y[o] = cadd(y[o], cscale(acc, sign * c[j]));
+    }
+}
+
+void launch_mf_couple(hipStream_t s, double2* y, const double2* x, const double2* d,
+                      const double* c, int nf, long N, double sign) {
+    k_mf_couple<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(y, x, d, c, nf, N, sign);
 }
 
 // ---------------------------------------------------------------------------

@@ -87,7 +87,7 @@ int ensure_factor_work(Handle* h, int n, int nch) {
     return FDFD_OK;
 }
 
-static const int KDIM = 30;
+static const int KDIM = GMRES_KDIM;
 
 int ensure_solve_work(Handle* h, long N) {
     if (h->work_N >= N) return FDFD_OK;
@@ -178,7 +178,10 @@ int rt_assemble(Handle* h, Factor* F) {
     HIPCHK(h, hipMalloc(&F->sxb, sizeof(double2) * g.Nx));
     HIPCHK(h, hipMalloc(&F->syf, sizeof(double2) * g.Ny));
     HIPCHK(h, hipMalloc(&F->syb, sizeof(double2) * g.Ny));
-    launch_sfactor(h->stream, F->sxf, F->sxb, F->syf, F->syb, g, F->omega, eps0n, eta0);
+    if (F->form == 1 && F->pol != FDFD_TM)
+        return fail(h, FDFD_ERR_INVALID, "the modulated (sideband) operator is TM only");
+    launch_sfactor(h->stream, F->sxf, F->sxb, F->syf, F->syb, g,
+                   F->omega_pml > 0 ? F->omega_pml : F->omega, eps0n, eta0);
     double kdiag;
     if (F->pol == FDFD_TM) {
         F->scale = mu0n * g.dx * g.dx;
@@ -190,7 +193,7 @@ int rt_assemble(Handle* h, Factor* F) {
     {
         ProfScope ps(h, P_STENCIL, 0, 0);
         launch_coeffs(h->stream, F->coef, h->eps, F->sxf, F->sxb, F->syf, F->syb, g, F->pol,
-                      kdiag);
+                      kdiag, F->form);
     }
     HIPCHK(h, hipGetLastError());
     return FDFD_OK;
@@ -241,36 +244,32 @@ struct SolveStats { int iters = 0, restarts = 0; double resid = 0; };
 
 // Right-preconditioned flexible GMRES(KDIM) on A' x = b (or A'^T x = b) in
 // FP64, preconditioned by the block factors.  x is the initial guess on entry.
-static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, double tol,
-                 int max_restarts, SolveStats& st) {
-    const GridDesc& g = h->g;
-    const long N = (long)g.Nx * g.Ny;
-    const int K = h->kdim;
-    double2* V = h->V;
-    double2* Zk = h->Zk;
+int rt_gmres_op(Handle* h, long N,
+                const std::function<int(const double2*, double2*, const double2*)>& apply,
+                const std::function<int(const double2*, double2*)>& prec, double2* V, double2* Zk,
+                const double2* b, double2* x, double tol, int max_restarts, int* iters,
+                int* restarts, double* resid_out) {
+    const int K = GMRES_KDIM;
     double bnorm;
     int rc;
     if ((rc = dev_norm(h, b, N, &bnorm))) return rc;
     if (bnorm == 0) {
         launch_zero(h->stream, x, N);
-        st.resid = 0;
+        *resid_out = 0;
         return FDFD_OK;
     }
     std::vector<cd> H((size_t)(K + 1) * K), gv(K + 1), sn(K), hc(K + 1), hc2(K + 1), y(K);
     std::vector<double> cs(K);
     double resid = 1;
     for (int rs = 0; rs <= max_restarts; ++rs) {
-        {
-            ProfScope ps(h, P_STENCIL, 0, 64.0 * N);
-            launch_apply(h->stream, V, x, F->coef, g.Nx, g.Ny, trans, b);
-        }
+        if ((rc = apply(x, V, b))) return rc;
         double beta;
         if ((rc = dev_norm(h, V, N, &beta))) return rc;
         resid = beta / bnorm;
-        st.resid = resid;
+        *resid_out = resid;
         if (resid <= tol) return FDFD_OK;
         if (rs == max_restarts) break;
-        if (rs > 0) st.restarts++;
+        if (rs > 0) ++*restarts;
         launch_scale_copy(h->stream, V, V, make_double2(1.0 / beta, 0), N);
         std::fill(gv.begin(), gv.end(), cd(0));
         gv[0] = beta;
@@ -279,11 +278,8 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
             double2* vj = V + (long)j * N;
             double2* zj = Zk + (long)j * N;
             double2* w = V + (long)(j + 1) * N;
-            if ((rc = rt_prec_apply(h, F, trans, vj, zj))) return rc;
-            {
-                ProfScope ps(h, P_STENCIL, 0, 64.0 * N);
-                launch_apply(h->stream, w, zj, F->coef, g.Nx, g.Ny, trans, nullptr);
-            }
+            if ((rc = prec(vj, zj))) return rc;
+            if ((rc = apply(zj, w, nullptr))) return rc;
             // CGS2
             if ((rc = dev_multidot(h, V, N, j + 1, w, hc.data()))) return rc;
             if ((rc = dev_update(h, w, V, N, j + 1, hc.data(), -1.0))) return rc;
@@ -315,7 +311,7 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
             }
             gv[j + 1] = -std::conj(sn[j]) * gv[j];
             gv[j] = cs[j] * gv[j];
-            st.iters++;
+            ++*iters;
             jdone = j + 1;
             double est = std::abs(gv[j + 1]) / bnorm;
             if (est <= 0.5 * tol || hn == 0) break;
@@ -328,11 +324,26 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
         }
         if ((rc = dev_update(h, x, Zk, N, jdone, y.data(), 1.0))) return rc;
     }
-    st.resid = resid;
+    *resid_out = resid;
     char buf[160];
     snprintf(buf, sizeof(buf),
              "refinement did not reach the tolerance %.1e (relative residual %.3e)", tol, resid);
     return fail(h, FDFD_ERR_NOCONV, buf);
+}
+
+// FGMRES on one factor's operator A' (or A'^T), preconditioned by the factor
+static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, double tol,
+                 int max_restarts, SolveStats& st) {
+    const GridDesc& g = h->g;
+    const long N = (long)g.Nx * g.Ny;
+    auto apply = [&](const double2* xx, double2* yy, const double2* bb) -> int {
+        ProfScope ps(h, P_STENCIL, 0, 64.0 * N);
+        launch_apply(h->stream, yy, xx, F->coef, g.Nx, g.Ny, trans, bb);
+        return FDFD_OK;
+    };
+    auto prec = [&](const double2* v, double2* z) -> int { return rt_prec_apply(h, F, trans, v, z); };
+    return rt_gmres_op(h, N, apply, prec, h->V, h->Zk, b, x, tol, max_restarts, &st.iters,
+                       &st.restarts, &st.resid);
 }
 
 }  // namespace fdfd

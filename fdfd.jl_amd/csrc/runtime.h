@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -36,6 +37,11 @@ struct Prof {
 struct Factor {
     int id = 0, pol = 1, precision = 0;
     double omega = 0;
+    // operator form: 0 = driven (Sxf dxf Sxb dxb + ..., driven.jl:35),
+    // 1 = sideband operator of the modulated solve (Sxb dxb Sxf dxf + ...,
+    // modulation.jl:83,88; TM only)
+    int form = 0;
+    double omega_pml = 0;          // frequency of the stretch factors (0: omega)
     int n = 0, m = 0, orient = 0;  // line length, number of lines, 0: lines along x
     long si = 1, sj = 1;           // stride of a line's cells / of consecutive lines
     double scale = 1;              // A' = scale * A
@@ -156,6 +162,17 @@ void rt_free_factor(Handle* h, Factor* F);
 // work buffers of the handle (grown on demand)
 int ensure_factor_work(Handle* h, int n, int nch);
 int ensure_solve_work(Handle* h, long N);
+
+// Right-preconditioned FGMRES(30) in FP64 on an operator given by callbacks:
+//   apply(x, y, b): y = b - A x if b != nullptr, else y = A x
+//   prec(v, z):     z = M^-1 v
+// V ((30 + 1) * N) and Zk (30 * N) are the caller's device workspace.  Stops
+// at ||b - A x|| <= tol ||b|| (FDFD_ERR_NOCONV after max_restarts restarts).
+constexpr int GMRES_KDIM = 30;
+int rt_gmres_op(Handle* h, long N, const std::function<int(const double2*, double2*, const double2*)>& apply,
+                const std::function<int(const double2*, double2*)>& prec, double2* V, double2* Zk,
+                const double2* b, double2* x, double tol, int max_restarts, int* iters,
+                int* restarts, double* resid);
 
 // block cyclic reduction (bcr.cpp)
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);

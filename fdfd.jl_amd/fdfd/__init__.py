@@ -10,11 +10,12 @@ from .core import (C0, DEFAULT_L0, EPS0, ETA0, MU0, TE, TM, XHAT, YHAT, ZHAT, De
 from .solve import Engine, eigenfrequency, engine, solve
 from .flux import flux_surface_integral, poynting, probe_field
 from .modes import Mode, add_mode, eigenmode, eigenmode_batch, get_modes, setup_mode
+from .modulated import ModulatedDevice, setup_deps, solve_modulated
 
 __all__ = [
     "C0", "DEFAULT_L0", "EPS0", "ETA0", "MU0", "TE", "TM", "XHAT", "YHAT", "ZHAT", "Device",
     "Field", "FieldTE", "FieldTM", "Grid", "Point", "normalize_parameters", "setup_eps",
     "setup_src", "Engine", "eigenfrequency", "engine", "solve", "flux_surface_integral",
     "poynting", "probe_field", "Mode", "add_mode", "eigenmode", "eigenmode_batch", "get_modes",
-    "setup_mode",
+    "setup_mode", "ModulatedDevice", "setup_deps", "solve_modulated",
 ]

@@ -76,6 +76,7 @@ PROTOTYPES = {
     "fdfd_eigenfrequency": (_I, [_P, _I, _D, _I, _I, _I, _I, _D, _I, _P, _P, _PS]),
     "fdfd_eigenmode": (_I, [_P, _I, C.POINTER(fdfd_mode_problem), _I, _I, _D, _D, _I, _P, _P,
                             _PI, _PS]),
+    "fdfd_modulated_solve": (_I, [_P, _D, _D, _I, _I, _P, _P, _I, _D, _P, _PS]),
     "fdfd_batch_create": (_I, [_P, C.POINTER(_P)]),
     "fdfd_batch_add": (_I, [_P, _I, _I, _D, _D, _I, _I, _D, _I, _P, _I, _P, _I, _P, _PI]),
     "fdfd_batch_run": (_I, [_P, _I, _D, _PS]),

@@ -206,6 +206,10 @@ def engine() -> Engine:
 
 def solve(d: Device, pol: int = TM, precision="fp64", tol: float = 1e-10, eng: Engine = None,
           return_stats: bool = False):
+    from .modulated import ModulatedDevice, solve_modulated
+    if isinstance(d, ModulatedDevice):  # solve(d::ModulatedDevice), modulation.jl:35
+        return solve_modulated(d, precision=precision, tol=tol, eng=eng,
+                               return_stats=return_stats)
     """solve(d::Device, pol::Polarization=TM), src/solver/driven.jl:4-59.
 
     One factorization per frequency (driven.jl:11); returns a FieldTM/FieldTE,

@@ -20,6 +20,7 @@
  *   dolinearsolve(A, b, sym)  (lu(A)\b)   src/solver/solver.jl:4-41
  *   eigenfrequency(d, pol, nev; which)    src/solver/eigen.jl:69-115
  *   eigenmode(d::Device{1}, pol, neff, n) src/solver/eigen.jl:6-29
+ *   solve(d::ModulatedDevice)             src/solver/modulation.jl:35-119
  *     (get_modes / setup_mode!)           src/device.jl:118-149
  *   S_create / create_sfactor             src/pml.jl:1-63
  *   delta / grid_average                  src/grid.jl:128-162
@@ -194,6 +195,28 @@ int fdfd_eigenmode(fdfd_handle* h, int nprob, const fdfd_mode_problem* probs,
                    int nev, int ncv, double L0, double tol, int maxiter,
                    fdfd_complex* beta, fdfd_complex* vectors, int* nconv,
                    fdfd_stats* stats);
+
+/* ---- modulated (MF-FDFD) driven solve ---------------------------------- */
+/* solve(d::ModulatedDevice) of src/solver/modulation.jl:35-119 at one
+ * carrier frequency omega (the reference loops over d.omega): TM only.  With
+ * K = nsidebands and wn_j = omega + j*Omega (j = -K..K) the coupled system is
+ *   A_j = Sxb dxb (1/mu0') Sxf dxf + Syb dyb (1/mu0') Syf dyf + wn_j^2 eps0' eps
+ *   row block j += 0.5 wn_j^2 eps0' (conj(deps) x_{j+1} + deps x_{j-1}),
+ *   b = i omega src in the centre block (j = 0),
+ * with the stretch factors at omega (sharedpml != 0) or at wn_j.  The handle's
+ * grid and eps (fdfd_set_grid, fdfd_set_eps) are used; deps and src are
+ * Nx*Ny.  The 2K+1 sideband operators are factored (precision as in
+ * fdfd_factor) and FP64 FGMRES on the coupled operator, preconditioned by
+ * them, reaches ||b - A x|| <= tol ||b||.
+ *   out: (2K+1) * Nx*Ny*3, sideband j = -K..K in order, each Ez, Hx, Hy with
+ *        Hx = -Syf dyf Ez / (i wn_j mu0'), Hy = Sxf dxf Ez / (i wn_j mu0')
+ *        (modulation.jl:108-109).
+ * Every wn_j must be > 0; K <= 16.  stats.t_factor_ms is the batched
+ * factorization, stats.t_solve_ms the whole call's device time. */
+int fdfd_modulated_solve(fdfd_handle* h, double omega, double Omega,
+                         int nsidebands, int sharedpml, const fdfd_complex* deps,
+                         const fdfd_complex* src, int precision, double tol,
+                         fdfd_complex* out, fdfd_stats* stats);
 
 /* ---- batched driven solves (many grids x frequencies x sources) --------- */
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source

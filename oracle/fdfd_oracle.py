@@ -26,6 +26,8 @@ see tests/test_oracle.py):
                                      npml=0 uniform eps; dense zgesv tiny case
   field recovery ................... closed-form differences of the solved field
   eigenfrequency ................... Kronecker-sum closed form (uniform eps, PML)
+  modulated (MF-FDFD) system ....... per-Fourier-mode closed form (uniform eps and
+                                     deps, npml=0); A^T = W^-1 A W with W = Sxb Syb
   1-D eigenmode (mode sources) ..... closed-form spectrum (uniform eps, TM and TE);
                                      lattice slab dispersion relations (TM and TE)
                                      derived independently; hand-computed slices
@@ -391,6 +393,101 @@ def solve(d: Device, pol: int = TM, return_system: bool = False):
         systems.append((A, b, lu))
     res = out[0] if len(out) == 1 else out
     return (res, systems) if return_system else res
+
+
+# ---------------------------------------------------------------------------
+# modulated device (MF-FDFD): src/solver/modulation.jl:4-119
+# ---------------------------------------------------------------------------
+@dataclass
+class ModulatedDevice:
+    """src/solver/modulation.jl:4-27: eps_r, deps_r (the modulation amplitude
+    Delta eps_r), src; omega (carrier list), Omega, nsidebands, sharedpml."""
+
+    grid: Grid
+    omega: np.ndarray
+    Omega: float
+    nsidebands: int
+    sharedpml: bool = True
+    eps_r: np.ndarray = None
+    deps_r: np.ndarray = None
+    src: np.ndarray = None
+    modes: list = dc_field(default_factory=list)
+
+    def __post_init__(self):
+        self.omega = np.atleast_1d(np.asarray(self.omega, dtype=np.float64))
+        if self.eps_r is None:
+            self.eps_r = np.ones(self.grid.N, dtype=np.complex128)
+        if self.deps_r is None:
+            self.deps_r = np.zeros(self.grid.N, dtype=np.complex128)
+        if self.src is None:
+            self.src = np.zeros(self.grid.N, dtype=np.complex128)
+
+
+def sideband_operator(g: Grid, eps_r: np.ndarray, omega: float, omega_pml: float):
+    """modulation.jl:80-89: Sxb dxb (1/mu0) Sxf dxf + Syb dyb (1/mu0) Syf dyf
+    + w^2 T_eps, stretch factors at omega_pml (note the order: backward
+    differences outside, forward inside, unlike driven.jl:35)."""
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    Sxf, Sxb, Syf, Syb = S_create(g, omega_pml)
+    dxf, dxb = delta(XHAT, FORWARD, g), delta(XHAT, BACKWARD, g)
+    dyf, dyb = delta(YHAT, FORWARD, g), delta(YHAT, BACKWARD, g)
+    Te = sp.diags(eps0 * eps_r.ravel(order="F"))
+    L = Sxb @ dxb * (1 / mu0) @ Sxf @ dxf + Syb @ dyb * (1 / mu0) @ Syf @ dyf
+    return (L + omega ** 2 * Te).tocsc(), (Sxf, Sxb, Syf, Syb)
+
+
+def modulated_system(d: ModulatedDevice, omega: float):
+    """modulation.jl:44-101 for one carrier: the (2K+1) N matrix, b, the
+    sideband frequencies and stretch factors."""
+    g = d.grid
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    K = d.nsidebands
+    nf = 2 * K + 1
+    wn = omega + d.Omega * np.arange(-K, K + 1)
+    N = len(g)
+    As, Ss = [], []
+    for j in range(nf):
+        A, S = sideband_operator(g, d.eps_r, wn[j], omega if d.sharedpml else wn[j])
+        As.append(A)
+        Ss.append(S)
+    A = sp.block_diag(As, format="csc")
+    if K > 0:
+        Td = sp.diags(eps0 * d.deps_r.ravel(order="F"))
+        Cp = sp.kron(sp.diags([0.5 * wn[:-1] ** 2], [1]), Td.conj())   # :95-96
+        Cm = sp.kron(sp.diags([0.5 * wn[1:] ** 2], [-1]), Td)          # :97-98
+        A = (A + Cp + Cm).tocsc()
+    b = np.zeros(nf * N, dtype=np.complex128)
+    b[K * N:(K + 1) * N] = 1j * omega * d.src.ravel(order="F")      # :67-69
+    return A, b, wn, Ss
+
+
+def solve_modulated(d: ModulatedDevice):
+    """solve(d::ModulatedDevice), modulation.jl:35-119: fields[i][j] is the
+    FieldTM of carrier i at sideband j (wn = w + (j-K) Omega), H from the
+    forward stretched differences (:107-109).  Modes rebuild src at every
+    carrier with TM modes (:52-56)."""
+    g = d.grid
+    _, mu0, _ = normalize_parameters(g.L0)
+    N = len(g)
+    out = []
+    for omega in d.omega:
+        omega = float(omega)
+        if d.modes:
+            d.src = np.zeros(g.N, dtype=np.complex128)
+            for mode in d.modes:
+                setup_mode(d, TM, omega, mode.neff, mode.pt, mode.dir, mode.width)
+        A, b, wn, Ss = modulated_system(d, omega)
+        x, _ = linear_solve(A, b)
+        dyf, dxf = delta(YHAT, FORWARD, g), delta(XHAT, FORWARD, g)
+        row = []
+        for j in range(len(wn)):
+            Sxf, _, Syf, _ = Ss[j]
+            ez = x[j * N:(j + 1) * N]
+            hx = -1 / 1j / wn[j] / mu0 * (Syf @ (dyf @ ez))
+            hy = 1 / 1j / wn[j] / mu0 * (Sxf @ (dxf @ ez))
+            row.append(FieldTM(g, complex(wn[j]), _stack(g, ez, hx, hy)))
+        out.append(row)
+    return out
 
 
 # ---------------------------------------------------------------------------

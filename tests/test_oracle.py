@@ -440,3 +440,90 @@ def test_mode_slices_and_setup_mode():
     assert not rest.any()
     with pytest.raises(IndexError):
         O.get_modes(d, O.TM, float(d.omega[0]), 3.5, 1, (1.0, -0.95), O.XHAT, 0.8)
+
+
+# --- modulated device, MF-FDFD (modulation.jl:35-119) ------------------------
+def test_modulated_fourier_closed_form():
+    """Uniform eps and deps, npml = 0: every 2-D Fourier mode (p, q) of the
+    coupled system is an independent (2K+1) x (2K+1) tridiagonal system
+      M[j][j]   = -(4/mu0)(sin^2(pi p/Nx)/dx^2 + sin^2(pi q/Ny)/dy^2) + wn_j^2 eps0 eps
+      M[j][j+1] = 0.5 wn_j^2 eps0 conj(deps),  M[j+1][j] = 0.5 wn_{j+1}^2 eps0 deps
+    (the backward-forward product has the same symbol as forward-backward).
+    Solving mode by mode through the FFT must reproduce the oracle's sparse
+    solve; a swapped conj, a wrong sideband index or a wrong omega in the
+    coupling fails by O(1)."""
+    Nx, Ny, K = 12, 10, 2
+    g = O.Grid.create(0.05, (0, 0), (0, Nx * 0.05), (0, Ny * 0.05))
+    om, Om = O.omega_of_wavelength(1.55), 1.7e14
+    d = O.ModulatedDevice(g, om, Om, K, sharedpml=True)
+    d.eps_r[:] = 3.0 + 0.1j
+    dep = 0.7 + 0.4j
+    d.deps_r[:] = dep
+    rng = np.random.default_rng(5)
+    d.src[:] = rng.standard_normal(g.N) + 1j * rng.standard_normal(g.N)
+    A, b, wn, _ = O.modulated_system(d, om)
+    x, _ = O.linear_solve(A, b)
+    eps0, mu0, _ = O.normalize_parameters(g.L0)
+    nf = 2 * K + 1
+    N = len(g)
+    bhat = np.fft.fft2(d.src * 1j * om)   # Nx x Ny (x fastest in the flat layout)
+    p = np.arange(Nx)[:, None]
+    q = np.arange(Ny)[None, :]
+    lap = -(4 / mu0) * (np.sin(np.pi * p / Nx) ** 2 / g.dx ** 2 + np.sin(np.pi * q / Ny) ** 2 / g.dy ** 2)
+    xhat = np.zeros((nf, Nx, Ny), dtype=complex)
+    for ip in range(Nx):
+        for iq in range(Ny):
+            M = np.zeros((nf, nf), dtype=complex)
+            for j in range(nf):
+                M[j, j] = lap[ip, iq] + wn[j] ** 2 * eps0 * d.eps_r[0, 0]
+                if j + 1 < nf:
+                    M[j, j + 1] = 0.5 * wn[j] ** 2 * eps0 * np.conj(dep)
+                    M[j + 1, j] = 0.5 * wn[j + 1] ** 2 * eps0 * dep
+            rhs = np.zeros(nf, dtype=complex)
+            rhs[K] = bhat[ip, iq]
+            xhat[:, ip, iq] = np.linalg.solve(M, rhs)
+    for j in range(nf):
+        ref = np.fft.ifft2(xhat[j]).ravel(order="F")
+        got = x[j * N:(j + 1) * N]
+        assert np.linalg.norm(got - ref) <= 1e-11 * np.linalg.norm(x)
+
+
+def test_sideband_operator_transpose_identity_and_cell_loop():
+    """With PML the sideband operator Sxb dxb Sxf dxf + Syb dyb Syf dyf + w^2 T
+    satisfies A^T = W^-1 A W for W = diag(sxb_x syb_y) (dxf^T = -dxb), which
+    the driven ordering (W = sxf syf) does not; and it equals a per-cell loop
+    built from the difference definitions."""
+    g = O.Grid.create(0.05, (4, 3), (0, 0.8), (0, 0.6))
+    om = O.omega_of_wavelength(1.55)
+    eps = 1 + 5 * np.random.default_rng(2).random(g.N)
+    A, (Sxf, Sxb, Syf, Syb) = O.sideband_operator(g, eps, om, om * 1.1)
+    sxb = np.diag(Sxb.toarray()).reshape(g.N, order="F")
+    syb = np.diag(Syb.toarray()).reshape(g.N, order="F")
+    W = sp_diag((sxb * syb).ravel(order="F"))
+    lhs = A.T.toarray()
+    rhs = (np.linalg.inv(W) @ A.toarray() @ W)
+    assert np.abs(lhs - rhs).max() <= 1e-12 * np.abs(lhs).max()
+    # the driven-form W does not work for this operator
+    sxf = np.diag(Sxf.toarray()).reshape(g.N, order="F")
+    syf = np.diag(Syf.toarray()).reshape(g.N, order="F")
+    Wf = sp_diag((sxf * syf).ravel(order="F"))
+    assert np.abs(lhs - np.linalg.inv(Wf) @ A.toarray() @ Wf).max() > 1e-6 * np.abs(lhs).max()
+    # per-cell loop
+    eps0, mu0, _ = O.normalize_parameters(g.L0)
+    sx_f, sx_b = sxf[:, 0], sxb[:, 0]
+    sy_f, sy_b = syf[0, :], syb[0, :]
+    Nx, Ny = g.N
+    e = np.random.default_rng(3).standard_normal(g.N) + 0j
+    y = np.zeros(g.N, dtype=complex)
+    for i in range(Nx):
+        for j in range(Ny):
+            ip, im, jp, jm = (i + 1) % Nx, (i - 1) % Nx, (j + 1) % Ny, (j - 1) % Ny
+            lx = sx_b[i] * (sx_f[i] * (e[ip, j] - e[i, j]) - sx_f[im] * (e[i, j] - e[im, j])) / g.dx ** 2
+            ly = sy_b[j] * (sy_f[j] * (e[i, jp] - e[i, j]) - sy_f[jm] * (e[i, j] - e[i, jm])) / g.dy ** 2
+            y[i, j] = (lx + ly) / mu0 + om ** 2 * eps0 * eps[i, j] * e[i, j]
+    got = (A @ e.ravel(order="F")).reshape(g.N, order="F")
+    assert np.abs(got - y).max() <= 1e-12 * np.abs(y).max()
+
+
+def sp_diag(v):
+    return np.diag(v)
