@@ -131,6 +131,53 @@ def mode_leg(eng, d, with_cpu: bool):
     return out
 
 
+def modulated_leg(eng, with_cpu: bool):
+    """MF-FDFD solve (fdfd_modulated_solve) of the modulated waveguide the
+    reference's tests use: 500 x 200 cells (dl = 0.01 um), one sideband each
+    side, a TM mode source, eps 12.25 guide, deps = exp(i q x) on half the
+    guide over 3 um.  GPU wall time per solve (factor + coupled FGMRES +
+    fields) for both precisions; the oracle's direct sparse solve of the same
+    coupled system is timed beside it."""
+    from fdfd import core, modes
+    from fdfd.modulated import ModulatedDevice, setup_deps, solve_modulated
+    a, q, L = 0.2202, 2.9263, 5.0
+    g = core.Grid(0.01, (15, 15), (0.0, L), (-1.0, 1.0))
+    d = ModulatedDevice(g, 2 * math.pi * 1.939e14, 4.541e14, 1)
+    core.setup_eps(d, lambda x, y: -a / 2 <= y <= a / 2, 12.25)
+    setup_deps(d, lambda x, y: 1 <= x <= L - 1 and -a / 2 <= y <= 0,
+               lambda x, y: np.exp(1j * q * x))
+    modes.add_mode(d, modes.Mode(1, core.XHAT, 3.5, core.Point(0.2, 0.0), 4 * a))
+    out = {"workload": "modulated waveguide 500x200, K=1 (3 sidebands), mode source, tol 1e-10"}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        solve_modulated(d, precision=prec, tol=1e-10, eng=eng)  # warm
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            f, st = solve_modulated(d, precision=prec, tol=1e-10, eng=eng, return_stats=True)
+        wall = (time.perf_counter() - t0) / reps
+        res[prec] = f[0]
+        out[prec] = {"wall_ms": wall * 1e3, "factor_ms": st[0]["t_factor_ms"],
+                     "solve_ms": st[0]["t_solve_ms"], "fgmres_its": st[0]["iterations"],
+                     "residual": st[0]["residual"]}
+    if with_cpu:
+        import fdfd_oracle as O
+        og = O.Grid.create(0.01, (15, 15), (0.0, L), (-1.0, 1.0))
+        o = O.ModulatedDevice(og, float(d.omega[0]), d.Omega, 1)
+        o.eps_r[:] = d.eps_r
+        o.deps_r[:] = d.deps_r
+        O.add_mode(o, O.Mode(1, O.XHAT, 3.5, (0.2, 0.0), 4 * a))
+        t0 = time.perf_counter()
+        fo = O.solve_modulated(o)[0]
+        out["cpu_oracle_s"] = time.perf_counter() - t0
+        out["cpu_kind"] = "port (scipy spsolve/SuperLU of the coupled system + refinement, 1 thread)"
+        sc = max(np.linalg.norm(x.data) for x in fo)
+        out["max_rel_diff_vs_oracle"] = {p: float(max(np.linalg.norm(a_.data - b_.data) / sc
+                                                       for a_, b_ in zip(res[p], fo)))
+                                         for p in res}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +382,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "modes": mode_leg(eng, d, not a.no_cpu),
+        "modulated": modulated_leg(eng, not a.no_cpu),
     }
     for p in (eps_d, src_d, out_d):
         hip.free(p)

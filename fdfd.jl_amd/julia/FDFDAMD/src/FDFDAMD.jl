@@ -259,14 +259,62 @@ function eigenfrequency(d::FDFD.AbstractDevice, pol::FDFD.Polarization, nev::Int
 end
 
 """
+    solve(d::ModulatedDevice; precision=:fp64, tol=1e-10)
+
+Same contract as FDFD.solve(d::ModulatedDevice) (src/solver/modulation.jl:35-119):
+a `length(d.ω) × (2K+1)` array of `FieldTM`, sideband j at ωₙ = ω + (j-K-1)Ω.
+Each carrier is one `fdfd_modulated_solve` call: the 2K+1 sideband operators
+are factored on the GPU and FP64 FGMRES on the coupled operator, preconditioned
+by them, reaches `tol`.  Mode sources are rebuilt per carrier (TM).
+"""
+function solve(d::FDFD.ModulatedDevice; precision::Symbol=:fp64, tol::Real=1e-10,
+               handle::Handle=default_handle())
+    h = handle
+    g = d.grid
+    (Nx, Ny) = size(g)
+    check(h, ccall((:fdfd_set_grid, libfdfd), Cint,
+                   (Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Cint, Cint, Cdouble),
+                   h.ptr, Nx, Ny, FDFD.dx(g), FDFD.dy(g), g.Npml[1], g.Npml[2], g.L₀))
+    eps = Array{ComplexF64}(d.ϵᵣ)
+    GC.@preserve eps check(h, ccall((:fdfd_set_eps, libfdfd), Cint,
+                                    (Ptr{Cvoid}, Ptr{ComplexF64}), h.ptr, eps))
+    K = d.nsidebands
+    nf = 2K + 1
+    deps = Array{ComplexF64}(d.Δϵᵣ)
+    fields = Array{FDFD.FieldTM}(undef, length(d.ω), nf)
+    for (i, ω) in enumerate(d.ω)
+        @info "Frequency: $i/$(length(d.ω))"
+        if !isempty(d.modes)
+            d.src = zeros(Complex, size(g))
+            for m in d.modes
+                setup_mode!(d, FDFD.TM, ω, m.neff, m.pt, m.dir, m.width; handle=h)
+            end
+        end
+        src = Array{ComplexF64}(d.src)
+        out = Array{ComplexF64}(undef, Nx, Ny, 3, nf)
+        st = Ref(Stats())
+        GC.@preserve deps src out check(h, ccall((:fdfd_modulated_solve, libfdfd), Cint,
+            (Ptr{Cvoid}, Cdouble, Cdouble, Cint, Cint, Ptr{ComplexF64}, Ptr{ComplexF64}, Cint,
+             Cdouble, Ptr{ComplexF64}, Ref{Stats}),
+            h.ptr, ω, d.Ω, K, d.sharedpml ? 1 : 0, deps, src, precision_code(precision), tol,
+            out, st))
+        for j in 1:nf
+            fields[i, j] = FDFD.FieldTM(g, ω + d.Ω * (j - K - 1), out[:, :, :, j])
+        end
+    end
+    return fields
+end
+
+"""
     install!()
 
-Route `FDFD.solve(::Device, ::Polarization)`, `FDFD.eigenfrequency` and the
-1-D `FDFD.eigenmode` (so `setup_mode!` / `get_modes` as well) to this backend
-(method replacement in the FDFD module).
+Route `FDFD.solve(::Device, ::Polarization)`, `FDFD.solve(::ModulatedDevice)`,
+`FDFD.eigenfrequency` and the 1-D `FDFD.eigenmode` (so `setup_mode!` /
+`get_modes` as well) to this backend (method replacement in the FDFD module).
 """
 function install!()
     @eval FDFD solve(d::Device, pol::Polarization=TM) = $(solve)(d, pol)
+    @eval FDFD solve(d::ModulatedDevice) = $(solve)(d)
     @eval FDFD eigenmode(d::AbstractDevice{1}, pol::Polarization, neff::Number, n::Int) =
         $(eigenmode)(d, pol, neff, n)
     @eval FDFD eigenfrequency(d::AbstractDevice, pol::Polarization, n::Int; which::Symbol=:LM) =
