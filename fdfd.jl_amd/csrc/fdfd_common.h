@@ -225,6 +225,11 @@ void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const doubl
 // blocks of N cells (modulation.jl:95-99); c has nf entries
 void launch_mf_couple(hipStream_t s, double2* y, const double2* x, const double2* d,
                       const double* c, int nf, long N, double sign);
+// Poynting maps (S: Sx then Sy, N doubles each) and plane fluxes (int4 =
+// normal, index, lo, hi) of a 3 x N field array
+void launch_poynting(hipStream_t s, double* S, const double2* f, int Nx, int Ny, int pol);
+void launch_flux(hipStream_t s, double* out, const double2* f, const int4* planes, int nplanes,
+                 int Nx, int Ny, int pol, double dx, double dy);
 void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef,
                   int Nx, int Ny, int transpose, const double2* b /*nullable: r=b-Ax*/);
 void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n);

@@ -375,6 +375,71 @@ void launch_fields(hipStream_t s, double2* out, const double2* x, const double2*
                                                          g.Ny, g.dx, g.dy, pol, w, eigen_mode);
 }
 
+// ---------------------------------------------------------------------------
+// Poynting vector and flux through planes (src/flux.jl:15-66).  Fields are
+// the 3 x N layout of fdfd_solve (Ez, Hx, Hy or Hz, Ex, Ey), x fastest.
+//   TM: Sx = -1/2 Re(avg_x(Ez) conj(Hy)),  Sy = 1/2 Re(avg_y(Ez) conj(Hx))
+//   TE: Sx =  1/2 Re(Ey conj(avg_x(Hz))),  Sy = -1/2 Re(Ex conj(avg_y(Hz)))
+// avg = grid_average (periodic, the cell and its lower neighbour).
+// ---------------------------------------------------------------------------
+__global__ void k_poynting(double* S, const double2* f, int Nx, int Ny, int pol) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    const long kw = (i == 0) ? k + Nx - 1 : k - 1;
+    const long ks = (j == 0) ? k + N - Nx : k - Nx;
+    const double2 u = f[k];
+    const double2 ax = cscale(cadd(u, f[kw]), 0.5), ay = cscale(cadd(u, f[ks]), 0.5);
+    const double2 a = f[N + k], b = f[2 * N + k];  // Hx, Hy (TM) or Ex, Ey (TE)
+    if (pol == 1) {
+        S[k] = -0.5 * cmulc(ax, b).x;      // Re(ax conj(Hy))
+        S[N + k] = 0.5 * cmulc(ay, a).x;   // Re(ay conj(Hx))
+    } else {
+        S[k] = 0.5 * cmulc(b, ax).x;       // Re(Ey conj(ax))
+        S[N + k] = -0.5 * cmulc(a, ay).x;  // Re(Ex conj(ay))
+    }
+}
+
+void launch_poynting(hipStream_t s, double* S, const double2* f, int Nx, int Ny, int pol) {
+    const long N = (long)Nx * Ny;
+    k_poynting<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(S, f, Nx, Ny, pol);
+}
+
+// One workgroup per plane {normal, index, lo, hi} (0-based, lo..hi inclusive
+// along the plane, index + 1 < the normal extent): the normal component with
+// the first field averaged over the plane's line and the next one (the
+// reference's view xind:xind+1, flux.jl:42-44), summed, times the cell size
+// along the plane.
+//   x normal, TM: -1/2 Re(avg Ez conj(Hy))   TE: 1/2 Re(Ey conj(avg Hz))
+//   y normal, TM:  1/2 Re(avg Ez conj(Hx))   TE: -1/2 Re(Ex conj(avg Hz))
+__global__ __launch_bounds__(256) void k_flux(double* out, const double2* f, const int4* planes,
+                                              int Nx, int Ny, int pol, double dx, double dy) {
+    const long N = (long)Nx * Ny;
+    const int4 P = planes[blockIdx.x];
+    const bool xn = P.x == 1;
+    double acc = 0;
+    for (int t = P.z + (int)threadIdx.x; t <= P.w; t += 256) {
+        const long k = xn ? (long)t * Nx + P.y : (long)P.y * Nx + t;
+        const long k1 = xn ? k + 1 : k + Nx;
+        const double2 u = cscale(cadd(f[k], f[k1]), 0.5);
+        double v;
+        if (xn) v = (pol == 1) ? -0.5 * cmulc(u, f[2 * N + k]).x : 0.5 * cmulc(f[2 * N + k], u).x;
+        else v = (pol == 1) ? 0.5 * cmulc(u, f[N + k]).x : -0.5 * cmulc(f[N + k], u).x;
+        acc += v;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (part[0] + part[1] + part[2] + part[3]) * (xn ? dy : dx);
+}
+
+void launch_flux(hipStream_t s, double* out, const double2* f, const int4* planes, int nplanes,
+                 int Nx, int Ny, int pol, double dx, double dy) {
+    k_flux<<<nplanes, 256, 0, s>>>(out, f, planes, Nx, Ny, pol, dx, dy);
+}
+
 // grad[k] = Re(coef * lam[k] * e[k])
 __global__ void k_grad(double* grad, const double2* e, const double2* lam, double2 coef,
                        long n) {

@@ -42,6 +42,10 @@ class fdfd_mode_problem(C.Structure):
                 ("neff", fdfd_complex), ("eps", C.c_void_p)]
 
 
+class fdfd_flux_plane(C.Structure):
+    _fields_ = [("normal", C.c_int), ("index", C.c_int), ("lo", C.c_int), ("hi", C.c_int)]
+
+
 class FDFDError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
@@ -77,6 +81,10 @@ PROTOTYPES = {
     "fdfd_eigenmode": (_I, [_P, _I, C.POINTER(fdfd_mode_problem), _I, _I, _D, _D, _I, _P, _P,
                             _PI, _PS]),
     "fdfd_modulated_solve": (_I, [_P, _D, _D, _I, _I, _P, _P, _I, _D, _P, _PS]),
+    "fdfd_poynting": (_I, [_P, _I, _I, _I, _P, _P]),
+    "fdfd_poynting_d": (_I, [_P, _I, _I, _I, _P, _P]),
+    "fdfd_flux": (_I, [_P, _I, _I, _I, _D, _D, _P, _I, C.POINTER(fdfd_flux_plane), _PD]),
+    "fdfd_flux_d": (_I, [_P, _I, _I, _I, _D, _D, _P, _I, C.POINTER(fdfd_flux_plane), _PD]),
     "fdfd_batch_create": (_I, [_P, C.POINTER(_P)]),
     "fdfd_batch_add": (_I, [_P, _I, _I, _D, _D, _I, _I, _D, _I, _P, _I, _P, _I, _P, _PI]),
     "fdfd_batch_run": (_I, [_P, _I, _D, _PS]),

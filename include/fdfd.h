@@ -21,6 +21,7 @@
  *   eigenfrequency(d, pol, nev; which)    src/solver/eigen.jl:69-115
  *   eigenmode(d::Device{1}, pol, neff, n) src/solver/eigen.jl:6-29
  *   solve(d::ModulatedDevice)             src/solver/modulation.jl:35-119
+ *   poynting / flux_surface_integral      src/flux.jl:15-66
  *     (get_modes / setup_mode!)           src/device.jl:118-149
  *   S_create / create_sfactor             src/pml.jl:1-63
  *   delta / grid_average                  src/grid.jl:128-162
@@ -217,6 +218,42 @@ int fdfd_modulated_solve(fdfd_handle* h, double omega, double Omega,
                          int nsidebands, int sharedpml, const fdfd_complex* deps,
                          const fdfd_complex* src, int precision, double tol,
                          fdfd_complex* out, fdfd_stats* stats);
+
+/* ---- Poynting vector and flux (post-processing of fields) --------------- */
+/* poynting(field) (src/flux.jl:15-30) on a Nx*Ny*3 field array (Ez, Hx, Hy
+ * for TM; Hz, Ex, Ey for TE; the fdfd_solve layout).  S: Sx then Sy, Nx*Ny
+ * doubles each.
+ *   TM: Sx = -1/2 Re(avg_x(Ez) conj(Hy)), Sy = 1/2 Re(avg_y(Ez) conj(Hx))
+ *   TE: Sx = 1/2 Re(Ey conj(avg_x(Hz))), Sy = -1/2 Re(Ex conj(avg_y(Hz)))
+ * avg = grid_average (src/grid.jl:157-162).  The reference's TE branch reads
+ * an undefined field (flux.jl:26-27); these are its TE analogue.  The grid is
+ * explicit: these calls never touch the handle's grid or factors. */
+int fdfd_poynting(fdfd_handle* h, int pol, int Nx, int Ny,
+                  const fdfd_complex* fields, double* S);
+int fdfd_poynting_d(fdfd_handle* h, int pol, int Nx, int Ny,
+                    const void* d_fields, void* d_S);
+
+/* A flux plane: normal 1 (x) or 2 (y); index = the 0-based column (x normal)
+ * or row (y normal), averaged with index + 1 as in flux.jl:42-44; cells lo..hi
+ * (0-based, inclusive) along the plane. */
+typedef struct {
+    int normal;
+    int index;
+    int lo;
+    int hi;
+} fdfd_flux_plane;
+
+/* flux_surface_integral(field, center, width, normal) (src/flux.jl:37-66)
+ * for nplanes planes at once, one value each: the normal Poynting component
+ * summed along the plane times the cell size along it.  x-normal TM is the
+ * reference's formula; x-normal TE (broken in the reference, :49-54) and the
+ * y-normal cases (unimplemented there, :58-65) are completed the same way. */
+int fdfd_flux(fdfd_handle* h, int pol, int Nx, int Ny, double dx, double dy,
+              const fdfd_complex* fields, int nplanes,
+              const fdfd_flux_plane* planes, double* flux);
+int fdfd_flux_d(fdfd_handle* h, int pol, int Nx, int Ny, double dx, double dy,
+                const void* d_fields, int nplanes,
+                const fdfd_flux_plane* planes, double* flux);
 
 /* ---- batched driven solves (many grids x frequencies x sources) --------- */
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source

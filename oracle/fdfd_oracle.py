@@ -32,6 +32,10 @@ see tests/test_oracle.py):
                                      lattice slab dispersion relations (TM and TE)
                                      derived independently; hand-computed slices
   flux_surface_integral ............ lossless energy conservation between planes
+                                     (TM and TE, x and y normals)
+  scattering_parameters ............ modal propagation in a straight guide: |S|
+                                     constant and phase advancing by the lattice
+                                     wavenumber arccos(1 - b^2 dx^2/2)/dx
   adjoint gradient ................. central finite differences
 """
 from __future__ import annotations
@@ -660,6 +664,71 @@ def flux_surface_integral_tm_x(f: FieldTM, xcen: float, ycen: float, width: floa
     Ezx = (Ez[0, :] + Ez[1, :]) / 2
     Hy = f.data[xi, yi, 2]
     return float(np.sum(-0.5 * np.real(Ezx * np.conj(Hy))) * g.dy)
+
+
+def poynting(f):
+    """flux.jl:15-30.  TM: Sx = -1/2 Re(avg_x(Ez) conj(Hy)),
+    Sy = 1/2 Re(avg_y(Ez) conj(Hx)), avg = grid_average (periodic, i and
+    i-1).  The TE branch of the reference reads an undefined field (:26-27);
+    here it is the TE analogue, Sx = 1/2 Re(Ey conj(avg_x Hz)),
+    Sy = -1/2 Re(Ex conj(avg_y Hz)) (the x, y parts of 1/2 Re(E x H*))."""
+    u = f.data[:, :, 0]
+    if isinstance(f, FieldTM):
+        return (-0.5 * np.real(grid_average(u, XHAT) * np.conj(f.data[:, :, 2])),
+                0.5 * np.real(grid_average(u, YHAT) * np.conj(f.data[:, :, 1])))
+    return (0.5 * np.real(f.data[:, :, 2] * np.conj(grid_average(u, XHAT))),
+            -0.5 * np.real(f.data[:, :, 1] * np.conj(grid_average(u, YHAT))))
+
+
+def flux_surface_integral(f, xcen: float, ycen: float, width: float, normal: int) -> float:
+    """flux.jl:37-66.  x-normal TM is the reference's formula
+    (flux_surface_integral_tm_x).  The reference's x-normal TE branch is
+    broken (:49-54: undefined pt, :Hy for TE) and its y-normal branches are
+    unimplemented (:58-65); here they are completed the same way: the line
+    nearest the centre (tolerance h/2), cells within [c - w, c + w] along it,
+    the first field averaged with the next line (index + 1), times the cell
+    size along the line.
+      TE x: 1/2 Re(Ey conj(avg Hz))     TM y: 1/2 Re(avg Ez conj(Hx))
+      TE y: -1/2 Re(Ex conj(avg Hz))"""
+    g = f.grid
+    xs, ys = g.xc(), g.yc()
+    if normal == XHAT:
+        ci = np.nonzero(np.abs(xs - xcen) <= g.dx / 2)[0]
+        if ci.size == 0:
+            raise ValueError("no column within dx/2")
+        i = int(ci[0])
+        line = np.nonzero((ys >= ycen - width) & (ys <= ycen + width))[0]
+        u = (f.data[i, line, 0] + f.data[i + 1, line, 0]) / 2
+        if isinstance(f, FieldTM):
+            return float(np.sum(-0.5 * np.real(u * np.conj(f.data[i, line, 2]))) * g.dy)
+        return float(np.sum(0.5 * np.real(f.data[i, line, 2] * np.conj(u))) * g.dy)
+    cj = np.nonzero(np.abs(ys - ycen) <= g.dy / 2)[0]
+    if cj.size == 0:
+        raise ValueError("no row within dy/2")
+    j = int(cj[0])
+    line = np.nonzero((xs >= xcen - width) & (xs <= xcen + width))[0]
+    u = (f.data[line, j, 0] + f.data[line, j + 1, 0]) / 2
+    if isinstance(f, FieldTM):
+        return float(np.sum(0.5 * np.real(u * np.conj(f.data[line, j, 1]))) * g.dx)
+    return float(np.sum(-0.5 * np.real(f.data[line, j, 1] * np.conj(u))) * g.dx)
+
+
+def scattering_parameters(fields, d):
+    """flux.jl:68-91 (commented out in the reference, with `mode.coor` for
+    `mode.pt`): S[i, j] = sum((Ec - Ej) conj(Ej)) / sum(|Ej|^2) for the first
+    mode, sum(Ec conj(Ej)) / sum(|Ej|^2) for the others, Ec the computed
+    first field component on mode j's slice and Ej the mode (unit norm,
+    largest entry real positive)."""
+    S = np.zeros((len(d.omega), len(d.modes)), dtype=np.complex128)
+    for i, omega in enumerate(d.omega):
+        for j, mode in enumerate(d.modes):
+            _, Ej, ix, iy = get_modes(d, mode.pol, float(omega), mode.neff, 1, mode.pt, mode.dir,
+                                      mode.width)
+            Ej = Ej[:, 0]
+            Ec = fields[i].data[np.ix_(ix - 1, iy - 1)][:, :, 0].ravel()
+            numer = (Ec - Ej) * np.conj(Ej) if j == 0 else Ec * np.conj(Ej)
+            S[i, j] = np.sum(numer) / np.sum(Ej * np.conj(Ej))
+    return S
 
 
 # ---------------------------------------------------------------------------

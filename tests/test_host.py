@@ -52,28 +52,29 @@ def test_field_indexing_layout():
     assert f.components == ("Ez", "Hx", "Hy")
 
 
-def test_flux_matches_oracle_restatement():
+@pytest.mark.parametrize("center,width,normal", [
+    ((0.31, 0.4), 0.2, core.XHAT), ((0.55, 0.21), 0.1, core.XHAT), ((0.5, 0.41), 0.3, core.YHAT),
+    ((0.011, 0.75), 0.05, core.YHAT),
+])
+def test_flux_plane_matches_oracle_sum(center, width, normal):
+    """flux_plane resolves the reference's line and interval (flux.jl:40-42):
+    summing the oracle's flux formula over the resolved cells gives the
+    oracle's flux_surface_integral."""
     rng = np.random.default_rng(2)
     g = core.Grid(0.02, (0, 0), (0, 1.0), (0, 0.8))
     og = O.Grid.create(0.02, (0, 0), (0, 1.0), (0, 0.8))
     data = rng.standard_normal((50, 40, 3)) + 1j * rng.standard_normal((50, 40, 3))
-    f = core.FieldTM(g, 1.0, data)
+    nrm, idx, lo, hi = flux.flux_plane(g, core.Point(*center), width, normal)
     fo = O.FieldTM(og, 1.0, data)
-    for xc, yc, w in [(0.31, 0.4, 0.2), (0.55, 0.21, 0.1)]:
-        assert flux.flux_surface_integral(f, core.Point(xc, yc), w, core.XHAT) == pytest.approx(
-            O.flux_surface_integral_tm_x(fo, xc, yc, w), rel=1e-13)
-    with pytest.raises(NotImplementedError):
-        flux.flux_surface_integral(f, core.Point(0.31, 0.4), 0.2, core.YHAT)
-
-
-def test_poynting_tm_constant_fields():
-    g = core.Grid(1.0, (0, 0), (0, 4.0), (0, 4.0))
-    data = np.zeros((4, 4, 3), dtype=complex)
-    data[:, :, 0] = 2.0
-    data[:, :, 1] = 1j
-    data[:, :, 2] = 3.0
-    sx, sy = flux.poynting(core.FieldTM(g, 1.0, data))
-    assert np.allclose(sx, -0.5 * 2.0 * 3.0) and np.allclose(sy, 0.0)
+    if normal == core.XHAT:
+        u = (data[idx, lo:hi + 1, 0] + data[idx + 1, lo:hi + 1, 0]) / 2
+        ref = np.sum(-0.5 * np.real(u * np.conj(data[idx, lo:hi + 1, 2]))) * g.dy
+    else:
+        u = (data[lo:hi + 1, idx, 0] + data[lo:hi + 1, idx + 1, 0]) / 2
+        ref = np.sum(0.5 * np.real(u * np.conj(data[lo:hi + 1, idx, 1]))) * g.dx
+    assert O.flux_surface_integral(fo, center[0], center[1], width, normal) == pytest.approx(ref, rel=1e-13)
+    with pytest.raises(ValueError):
+        flux.flux_plane(g, core.Point(5.0, 0.4), 0.2, core.XHAT)
 
 
 @pytest.mark.parametrize("pt,normal,width", [

@@ -527,3 +527,93 @@ def test_sideband_operator_transpose_identity_and_cell_loop():
 
 def sp_diag(v):
     return np.diag(v)
+
+
+# --- flux, Poynting, S-parameters (flux.jl:15-91), TE and y-normal completed --
+def test_flux_hand_cases_te_and_y_normal():
+    """Hand values: a 4 x 6 grid of unit cells, constant fields except one
+    line, so the averaged neighbour (index + 1) and the conjugated factor are
+    visible in the result."""
+    g = O.Grid.create(1.0, (0, 0), (0, 4.0), (0, 6.0))
+    # TE, x normal: 1/2 Re(Ey conj(avg Hz)); column 1 averaged with column 2
+    d = np.zeros((4, 6, 3), dtype=complex)
+    d[:, :, 0] = 2.0          # Hz
+    d[2, :, 0] = 4.0 + 2.0j
+    d[:, :, 2] = 1.0 + 1.0j   # Ey
+    v = O.flux_surface_integral(O.FieldTE(g, 1.0, d), 1.5, 3.0, 1.0, O.XHAT)
+    # avg Hz = 3 + 1j; Re((1+1j)(3-1j)) = 4; two rows (y = 2.5, 3.5); dy = 1
+    assert v == pytest.approx(2 * 0.5 * 4.0)
+    # TM, y normal: 1/2 Re(avg Ez conj(Hx)); row 2 averaged with row 3
+    d = np.zeros((4, 6, 3), dtype=complex)
+    d[:, :, 0] = 1.0j         # Ez
+    d[:, 3, 0] = 3.0j
+    d[:, :, 1] = 2.0j         # Hx
+    v = O.flux_surface_integral(O.FieldTM(g, 1.0, d), 2.0, 2.5, 1.0, O.YHAT)
+    # avg Ez = 2j; Re(2j * conj(2j)) = 4; columns x = 1.5, 2.5; dx = 1
+    assert v == pytest.approx(2 * 0.5 * 4.0)
+    # TE, y normal: -1/2 Re(Ex conj(avg Hz))
+    d = np.zeros((4, 6, 3), dtype=complex)
+    d[:, :, 0] = 1.0
+    d[:, 3, 0] = -3.0         # avg Hz over rows 2, 3 = -1
+    d[:, :, 1] = 5.0          # Ex
+    v = O.flux_surface_integral(O.FieldTE(g, 1.0, d), 2.0, 2.5, 1.0, O.YHAT)
+    assert v == pytest.approx(2 * (-0.5) * (5.0 * -1.0))
+    # Poynting maps: TE with constant fields, Sx = 1/2 Re(Ey conj Hz)
+    d = np.zeros((4, 6, 3), dtype=complex)
+    d[:, :, 0], d[:, :, 1], d[:, :, 2] = 2.0, 1.0j, 3.0
+    sx, sy = O.poynting(O.FieldTE(g, 1.0, d))
+    assert np.allclose(sx, 3.0) and np.allclose(sy, -0.5 * np.real(1j * 2.0))
+
+
+@pytest.mark.parametrize("pol", [O.TM, O.TE])
+def test_flux_conservation_te_and_y_normal(pol):
+    """Lossless, source-free region between two planes: the same power
+    crosses both (up to the half-cell staggering of the flux formula), for
+    x-normal planes and, on the transposed device, y-normal planes."""
+    g = O.Grid.create(0.02, (20, 0), (0, 4.0), (0, 1.0))
+    d = O.Device(g, O.omega_of_wavelength(1.55))
+    d.eps_r[:, 20:31] = 12.25
+    d.src[50, :] = 1.0
+    f = O.solve(d, pol)
+    p1 = O.flux_surface_integral(f, 2.01, 0.5, 0.6, O.XHAT)
+    p2 = O.flux_surface_integral(f, 3.01, 0.5, 0.6, O.XHAT)
+    assert p1 > 0 and abs(p1 - p2) / abs(p1) < 2e-2
+    gt = O.Grid.create(0.02, (0, 20), (0, 1.0), (0, 4.0))
+    dt = O.Device(gt, d.omega)
+    dt.eps_r[:] = d.eps_r.T
+    dt.src[:] = d.src.T
+    ft = O.solve(dt, pol)
+    q1 = O.flux_surface_integral(ft, 0.5, 2.01, 0.6, O.YHAT)
+    q2 = O.flux_surface_integral(ft, 0.5, 3.01, 0.6, O.YHAT)
+    assert q1 > 0 and abs(q1 - q2) / abs(q1) < 2e-2
+    # x <-> y transposition maps Ez, Hy to Ez, -Hx (TM) and Hz, Ey to Hz, -Ex
+    # (TE): the y-normal power is the x-normal power of the original device
+    assert abs(q1 - p1) / abs(p1) < 1e-10
+
+
+def test_scattering_parameters_modal_propagation():
+    """A straight lossless guide excited by its own mode: the overlap of the
+    computed field with the mode keeps its magnitude along the guide, and its
+    phase advances by the lattice wavenumber k = arccos(1 - b^2 dx^2 / 2)/dx
+    of the mode's beta (the exact dispersion of the x differences), to 1e-5
+    (PML reflections and the slice's periodic wrap are below that)."""
+    g = O.Grid.create(0.02, (15, 15), (0.0, 6.0), (-1.0, 1.0))
+    d = O.Device(g, O.omega_of_wavelength(1.55))
+    d.eps_r[:, 43:58] = 12.25   # rows 44..58 (1-based), centred on row 51
+    w = 1.6
+    O.add_mode(d, O.Mode(O.TM, O.XHAT, 3.2, (1.0, 0.0), w))
+    f = O.solve(d, O.TM)          # driven by the input mode only
+    # ports: the input and two output planes (modes are sources in solve, so
+    # the output ports are added after it)
+    O.add_mode(d, O.Mode(O.TM, O.XHAT, 3.2, (3.0, 0.0), w))
+    O.add_mode(d, O.Mode(O.TM, O.XHAT, 3.2, (4.0, 0.0), w))
+    S = O.scattering_parameters([f], d)
+    beta, _, ix, _ = O.get_modes(d, O.TM, float(d.omega[0]), 3.2, 1, (3.0, 0.0), O.XHAT, w)
+    k = np.arccos(1 - (beta[0].real * g.dx) ** 2 / 2) / g.dx
+    _, _, ix2, _ = O.get_modes(d, O.TM, float(d.omega[0]), 3.2, 1, (4.0, 0.0), O.XHAT, w)
+    D = (ix2[0] - ix[0]) * g.dx
+    s1, s2 = S[0, 1], S[0, 2]
+    assert abs(abs(s2) / abs(s1) - 1) < 1e-5
+    # the solved fields go as exp(-i k x) downstream of the source
+    dphi = np.angle(s2 / s1)
+    assert abs(np.angle(np.exp(1j * (dphi + k * D)))) < 1e-5
