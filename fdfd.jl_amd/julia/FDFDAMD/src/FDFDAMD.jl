@@ -305,6 +305,64 @@ function solve(d::FDFD.ModulatedDevice; precision::Symbol=:fp64, tol::Real=1e-10
     return fields
 end
 
+# mirrors fdfd_flux_plane in include/fdfd.h (0-based indices)
+struct FluxPlane
+    normal::Cint
+    index::Cint
+    lo::Cint
+    hi::Cint
+end
+
+_flat(f::FDFD.Field) = Array{ComplexF64}(f.data)   # Nx x Ny x 3, column-major
+
+"""
+    poynting(field)
+
+Same contract as FDFD.poynting (src/flux.jl:15-30), with the TE branch
+completed (the reference's reads an undefined field): returns the
+reference's `Flux(grid, ω, Sx, Sy)`.
+"""
+function poynting(f::FDFD.Field; handle::Handle=default_handle())
+    (Nx, Ny) = size(f.grid)
+    data = _flat(f)
+    S = Array{Float64}(undef, Nx, Ny, 2)
+    pol = isa(f, FDFD.FieldTM) ? 1 : 2
+    GC.@preserve data S check(handle, ccall((:fdfd_poynting, libfdfd), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{ComplexF64}, Ptr{Float64}),
+        handle.ptr, pol, Nx, Ny, data, S))
+    return FDFD.Flux(f.grid, f.ω, S[:, :, 1], S[:, :, 2])
+end
+
+"""
+    flux_surface_integral(field, center, width, normal)
+
+Same contract as FDFD.flux_surface_integral (src/flux.jl:37-66); the TE and
+ŷ-normal branches (broken or unimplemented in the reference) are completed.
+"""
+function flux_surface_integral(f::FDFD.Field, center, width::Real, normal;
+                               handle::Handle=default_handle())
+    g = f.grid
+    (Nx, Ny) = size(g)
+    xc, yc = FDFD.xc(g), FDFD.yc(g)
+    if normal == FDFD.x̂
+        i = findfirst(x -> abs(x - center.x) <= FDFD.dx(g) / 2, xc)
+        line = findall(y -> center.y - width <= y <= center.y + width, yc)
+    else
+        i = findfirst(y -> abs(y - center.y) <= FDFD.dy(g) / 2, yc)
+        line = findall(x -> center.x - width <= x <= center.x + width, xc)
+    end
+    (i === nothing || isempty(line)) && error("flux plane outside the grid")
+    planes = [FluxPlane(Int(normal), i - 1, first(line) - 1, last(line) - 1)]
+    data = _flat(f)
+    out = Vector{Float64}(undef, 1)
+    pol = isa(f, FDFD.FieldTM) ? 1 : 2
+    GC.@preserve data planes out check(handle, ccall((:fdfd_flux, libfdfd), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{ComplexF64}, Cint,
+         Ptr{FluxPlane}, Ptr{Float64}),
+        handle.ptr, pol, Nx, Ny, FDFD.dx(g), FDFD.dy(g), data, 1, planes, out))
+    return out[1]
+end
+
 """
     install!()
 
@@ -315,6 +373,9 @@ Route `FDFD.solve(::Device, ::Polarization)`, `FDFD.solve(::ModulatedDevice)`,
 function install!()
     @eval FDFD solve(d::Device, pol::Polarization=TM) = $(solve)(d, pol)
     @eval FDFD solve(d::ModulatedDevice) = $(solve)(d)
+    @eval FDFD poynting(f::Field) = $(poynting)(f)
+    @eval FDFD flux_surface_integral(f::Field, c::Point, w::Real, n::Direction) =
+        $(flux_surface_integral)(f, c, w, n)
     @eval FDFD eigenmode(d::AbstractDevice{1}, pol::Polarization, neff::Number, n::Int) =
         $(eigenmode)(d, pol, neff, n)
     @eval FDFD eigenfrequency(d::AbstractDevice, pol::Polarization, n::Int; which::Symbol=:LM) =
