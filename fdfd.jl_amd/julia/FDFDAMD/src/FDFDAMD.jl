@@ -330,7 +330,8 @@ function poynting(f::FDFD.Field; handle::Handle=default_handle())
     GC.@preserve data S check(handle, ccall((:fdfd_poynting, libfdfd), Cint,
         (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{ComplexF64}, Ptr{Float64}),
         handle.ptr, pol, Nx, Ny, data, S))
-    return FDFD.Flux(f.grid, f.ω, S[:, :, 1], S[:, :, 2])
+    # the 3-D constructor: the (Sx, Sy) one uses the pre-1.0 cat(3, ...) (data.jl:108)
+    return FDFD.Flux(f.grid, f.ω, S)
 end
 
 """
