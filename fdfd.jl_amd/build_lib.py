@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "k_mode.hip", "runtime.cpp", "bcr.cpp",
            "eigen.cpp", "dense_eig.cpp", "batch.cpp", "mode.cpp", "modulated.cpp",
-           "flux.cpp"]
+           "flux.cpp", "arnoldi.cpp", "vmode.cpp"]
 
 # All device code is built without packed FP32 instructions.
 HIPFLAGS = [

@@ -225,6 +225,14 @@ void launch_coeffs(hipStream_t s, double2* coef, const double2* eps, const doubl
 // blocks of N cells (modulation.jl:95-99); c has nf entries
 void launch_mf_couple(hipStream_t s, double2* y, const double2* x, const double2* d,
                       const double* c, int nf, long N, double sign);
+// full-vector mode operator (unknowns Hx, Hy): planes of s (A_bb - sigma),
+// the off-diagonal coupling y += cs (A_xy x_y ; A_yx x_x), and the six fields
+void launch_vmode_coeffs(hipStream_t st, double2* coef, const double2* eps, const GridDesc& g,
+                         double k0sq, double2 sig, double s, int block);
+void launch_vmode_offdiag(hipStream_t st, double2* y, const double2* x, const double2* eps,
+                          const GridDesc& g, double cs);
+void launch_vmode_fields(hipStream_t st, double2* out, const double2* h, const double2* eps,
+                         const GridDesc& g, double2 beta, double omega, double eps0n);
 // Poynting maps (S: Sx then Sy, N doubles each) and plane fluxes (int4 =
 // normal, index, lo, hi) of a 3 x N field array
 void launch_poynting(hipStream_t s, double* S, const double2* f, int Nx, int Ny, int pol);

@@ -440,6 +440,162 @@ void launch_flux(hipStream_t s, double* out, const double2* f, const int4* plane
     k_flux<<<nplanes, 256, 0, s>>>(out, f, planes, Nx, Ny, pol, dx, dy);
 }
 
+// ---------------------------------------------------------------------------
+// Full-vector mode operator of a waveguide cross-section (eigen.jl:31-47),
+// unknowns (Hx, Hy), periodic differences, no PML:
+//   A_xx = k0^2 ey + ey dyf tz dyb + dxb dxf
+//   A_xy = -ey dyf tz dxb + dxb dyf
+//   A_yx = -ex dxf tz dyb + dyb dxf
+//   A_yy = k0^2 ex + ex dxf tz dxb + dyb dyf
+// with relative permittivity averages ex = avg_x eps, ey = avg_y eps,
+// tz = 1 / avg_y(avg_x eps) (grid_average, grid.jl:157-162; eps0 cancels).
+// Everything is scaled by s = dx^2 so the factors see O(1) entries.
+// ---------------------------------------------------------------------------
+namespace {
+__device__ inline double2 vm_eps(const double2* e, int i, int j, int Nx, int Ny) {
+    i = (i + Nx) % Nx;
+    j = (j + Ny) % Ny;
+    return e[(long)j * Nx + i];
+}
+__device__ inline double2 vm_ex(const double2* e, int i, int j, int Nx, int Ny) {
+    return cscale(cadd(vm_eps(e, i, j, Nx, Ny), vm_eps(e, i - 1, j, Nx, Ny)), 0.5);
+}
+__device__ inline double2 vm_ey(const double2* e, int i, int j, int Nx, int Ny) {
+    return cscale(cadd(vm_eps(e, i, j, Nx, Ny), vm_eps(e, i, j - 1, Nx, Ny)), 0.5);
+}
+__device__ inline double2 vm_tz(const double2* e, int i, int j, int Nx, int Ny) {
+    const double2 a = cscale(cadd(vm_ex(e, i, j, Nx, Ny), vm_ex(e, i, j - 1, Nx, Ny)), 0.5);
+    return crcp(a);
+}
+}  // namespace
+
+// 5-point planes of s (A_bb - sigma I), block b = 0 (xx) or 1 (yy)
+__global__ void k_vmode_coeffs(double2* coef, const double2* eps, int Nx, int Ny, double idx2,
+                               double idy2, double k0sq, double2 sig, double s, int block) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    double2 cE, cW, cN, cS, diag;
+    if (block == 0) {
+        const double2 ey = vm_ey(eps, i, j, Nx, Ny);
+        cE = make_double2(idx2, 0);
+        cW = cE;
+        cN = cscale(cmul(ey, vm_tz(eps, i, j + 1, Nx, Ny)), idy2);
+        cS = cscale(cmul(ey, vm_tz(eps, i, j, Nx, Ny)), idy2);
+        diag = cscale(ey, k0sq);
+    } else {
+        const double2 ex = vm_ex(eps, i, j, Nx, Ny);
+        cE = cscale(cmul(ex, vm_tz(eps, i + 1, j, Nx, Ny)), idx2);
+        cW = cscale(cmul(ex, vm_tz(eps, i, j, Nx, Ny)), idx2);
+        cN = make_double2(idy2, 0);
+        cS = cN;
+        diag = cscale(ex, k0sq);
+    }
+    const double2 cC = csub(csub(diag, cadd(cadd(cE, cW), cadd(cN, cS))), sig);
+    coef[CW * N + k] = cscale(cW, s);
+    coef[CE * N + k] = cscale(cE, s);
+    coef[CS * N + k] = cscale(cS, s);
+    coef[CN * N + k] = cscale(cN, s);
+    coef[CC * N + k] = cscale(cC, s);
+}
+
+void launch_vmode_coeffs(hipStream_t st, double2* coef, const double2* eps, const GridDesc& g,
+                         double k0sq, double2 sig, double s, int block) {
+    const long N = (long)g.Nx * g.Ny;
+    k_vmode_coeffs<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(
+        coef, eps, g.Nx, g.Ny, 1.0 / (g.dx * g.dx), 1.0 / (g.dy * g.dy), k0sq, sig, s, block);
+}
+
+// y[0:N] += c s A_xy x[N:2N],  y[N:2N] += c s A_yx x[0:N]
+__global__ void k_vmode_offdiag(double2* y, const double2* x, const double2* eps, int Nx, int Ny,
+                                double idxdy, double cs) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    const double2* hx = x;
+    const double2* hy = x + N;
+    auto at = [&](const double2* v, int a, int b) { return v[(long)((b + Ny) % Ny) * Nx + (a + Nx) % Nx]; };
+    const double2 tz = vm_tz(eps, i, j, Nx, Ny);
+    // A_xy hy
+    {
+        const double2 ey = vm_ey(eps, i, j, Nx, Ny);
+        const double2 tzn = vm_tz(eps, i, j + 1, Nx, Ny);
+        const double2 dn = csub(at(hy, i, j + 1), at(hy, i - 1, j + 1));
+        const double2 d0 = csub(at(hy, i, j), at(hy, i - 1, j));
+        const double2 t1 = cmul(ey, csub(cmul(tzn, dn), cmul(tz, d0)));
+        const double2 v = cscale(csub(csub(dn, d0), t1), idxdy * cs);
+        y[k] = cadd(y[k], v);
+    }
+    // A_yx hx
+    {
+        const double2 ex = vm_ex(eps, i, j, Nx, Ny);
+        const double2 tze = vm_tz(eps, i + 1, j, Nx, Ny);
+        const double2 de = csub(at(hx, i + 1, j), at(hx, i + 1, j - 1));
+        const double2 d0 = csub(at(hx, i, j), at(hx, i, j - 1));
+        const double2 t1 = cmul(ex, csub(cmul(tze, de), cmul(tz, d0)));
+        const double2 v = cscale(csub(csub(de, d0), t1), idxdy * cs);
+        y[N + k] = cadd(y[N + k], v);
+    }
+}
+
+void launch_vmode_offdiag(hipStream_t st, double2* y, const double2* x, const double2* eps,
+                          const GridDesc& g, double cs) {
+    const long N = (long)g.Nx * g.Ny;
+    k_vmode_offdiag<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(y, x, eps, g.Nx, g.Ny,
+                                                                 1.0 / (g.dx * g.dy), cs);
+}
+
+// fields of a mode (eigen.jl:56-61); out = Ex, Ey, Ez, Hx, Hy, Hz (6 N),
+// h = (Hx, Hy) (2 N).  Two passes: Hz, then E from Hz's neighbours.
+//   Hz = (dxf Hx + dyf Hy) / (i b)
+//   Ex = (dyb Hz + i b Hy) / (i w eps0 ex),  Ey = (-i b Hx - dxb Hz) / (i w eps0 ey)
+//   Ez = (dxb Hy - dyb Hx) tz / (i w eps0)
+__global__ void k_vmode_hz(double2* out, const double2* h, int Nx, int Ny, double idx, double idy,
+                           double2 inv_ib) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    const long ke = (i + 1 == Nx) ? k - (Nx - 1) : k + 1;
+    const long kn = (j + 1 == Ny) ? k - (N - Nx) : k + Nx;
+    const double2 dv = cadd(cscale(csub(h[ke], h[k]), idx), cscale(csub(h[N + kn], h[N + k]), idy));
+    out[5 * N + k] = cmul(dv, inv_ib);
+    out[3 * N + k] = h[k];
+    out[4 * N + k] = h[N + k];
+}
+
+__global__ void k_vmode_e(double2* out, const double2* eps, int Nx, int Ny, double idx, double idy,
+                          double2 ib, double2 inv_iweps0) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    const long kw = (i == 0) ? k + Nx - 1 : k - 1;
+    const long ks = (j == 0) ? k + N - Nx : k - Nx;
+    const double2 hx = out[3 * N + k], hy = out[4 * N + k], hz = out[5 * N + k];
+    const double2 ex = cdiv(cadd(cscale(csub(hz, out[5 * N + ks]), idy), cmul(ib, hy)),
+                            vm_ex(eps, i, j, Nx, Ny));
+    const double2 ey = cdiv(csub(cneg(cmul(ib, hx)), cscale(csub(hz, out[5 * N + kw]), idx)),
+                            vm_ey(eps, i, j, Nx, Ny));
+    const double2 curl = csub(cscale(csub(hy, out[4 * N + kw]), idx), cscale(csub(hx, out[3 * N + ks]), idy));
+    out[k] = cmul(ex, inv_iweps0);
+    out[N + k] = cmul(ey, inv_iweps0);
+    out[2 * N + k] = cmul(cmul(curl, vm_tz(eps, i, j, Nx, Ny)), inv_iweps0);
+}
+
+void launch_vmode_fields(hipStream_t st, double2* out, const double2* h, const double2* eps,
+                         const GridDesc& g, double2 beta, double omega, double eps0n) {
+    const long N = (long)g.Nx * g.Ny;
+    const unsigned nb = (unsigned)((N + 255) / 256);
+    const double2 ib = make_double2(-beta.y, beta.x);           // i beta
+    const double2 inv_ib = crcp(ib);
+    const double2 inv_iweps0 = make_double2(0.0, -1.0 / (omega * eps0n));
+    k_vmode_hz<<<nb, 256, 0, st>>>(out, h, g.Nx, g.Ny, 1.0 / g.dx, 1.0 / g.dy, inv_ib);
+    k_vmode_e<<<nb, 256, 0, st>>>(out, eps, g.Nx, g.Ny, 1.0 / g.dx, 1.0 / g.dy, ib, inv_iweps0);
+}
+
 // grad[k] = Re(coef * lam[k] * e[k])
 __global__ void k_grad(double* grad, const double2* e, const double2* lam, double2 coef,
                        long n) {

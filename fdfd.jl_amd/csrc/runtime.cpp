@@ -180,6 +180,18 @@ int rt_assemble(Handle* h, Factor* F) {
     HIPCHK(h, hipMalloc(&F->syb, sizeof(double2) * g.Ny));
     if (F->form == 1 && F->pol != FDFD_TM)
         return fail(h, FDFD_ERR_INVALID, "the modulated (sideband) operator is TM only");
+    if (F->form == 2 || F->form == 3) {
+        // a block of the full-vector mode operator: no stretching (unit
+        // factors, so transposed solves and field helpers stay defined)
+        const GridDesc g0{g.Nx, g.Ny, g.dx, g.dy, 0, 0, g.L0};
+        launch_sfactor(h->stream, F->sxf, F->sxb, F->syf, F->syb, g0, F->omega, eps0n, eta0);
+        F->scale = g.dx * g.dx;
+        const double k0sq = F->omega * F->omega * mu0n * eps0n;
+        ProfScope ps(h, P_STENCIL, 0, 0);
+        launch_vmode_coeffs(h->stream, F->coef, h->eps, g, k0sq, F->shift, F->scale, F->form - 2);
+        HIPCHK(h, hipGetLastError());
+        return FDFD_OK;
+    }
     launch_sfactor(h->stream, F->sxf, F->sxb, F->syf, F->syb, g,
                    F->omega_pml > 0 ? F->omega_pml : F->omega, eps0n, eta0);
     double kdiag;
@@ -260,7 +272,7 @@ int rt_gmres_op(Handle* h, long N,
     }
     std::vector<cd> H((size_t)(K + 1) * K), gv(K + 1), sn(K), hc(K + 1), hc2(K + 1), y(K);
     std::vector<double> cs(K);
-    double resid = 1;
+    double resid = 1, prev = 2;
     for (int rs = 0; rs <= max_restarts; ++rs) {
         if ((rc = apply(x, V, b))) return rc;
         double beta;
@@ -269,6 +281,10 @@ int rt_gmres_op(Handle* h, long N,
         *resid_out = resid;
         if (resid <= tol) return FDFD_OK;
         if (rs == max_restarts) break;
+        // a whole restart cycle that gains less than 10% has reached the
+        // rounding floor of the operator: stop instead of cycling
+        if (rs > 0 && resid > 0.9 * prev) break;
+        prev = resid;
         if (rs > 0) ++*restarts;
         launch_scale_copy(h->stream, V, V, make_double2(1.0 / beta, 0), N);
         std::fill(gv.begin(), gv.end(), cd(0));

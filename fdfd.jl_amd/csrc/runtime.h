@@ -40,8 +40,11 @@ struct Factor {
     // operator form: 0 = driven (Sxf dxf Sxb dxb + ..., driven.jl:35),
     // 1 = sideband operator of the modulated solve (Sxb dxb Sxf dxf + ...,
     // modulation.jl:83,88; TM only)
+    // 2, 3 = the Hx-Hx / Hy-Hy blocks of the full-vector mode operator minus
+    // `shift` (eigen.jl:45; no PML, scale dx^2)
     int form = 0;
     double omega_pml = 0;          // frequency of the stretch factors (0: omega)
+    double2 shift = {0, 0};        // sigma of forms 2, 3 (unscaled)
     int n = 0, m = 0, orient = 0;  // line length, number of lines, 0: lines along x
     long si = 1, sj = 1;           // stride of a line's cells / of consecutive lines
     double scale = 1;              // A' = scale * A

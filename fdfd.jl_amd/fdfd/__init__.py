@@ -6,11 +6,13 @@ flux.
 All numerics run in libfdfd on the GPU.
 """
 from .core import (C0, DEFAULT_L0, EPS0, ETA0, MU0, TE, TM, XHAT, YHAT, ZHAT, Device, Field,
-                   FieldTE, FieldTM, Grid, Point, normalize_parameters, setup_eps, setup_src)
+                   FieldAll, FieldTE, FieldTM, Grid, Point, normalize_parameters, setup_eps,
+                   setup_src)
 from .solve import Engine, eigenfrequency, engine, solve
 from .flux import (flux_plane, flux_surface_integral, flux_surface_integrals, poynting,
                    probe_field, scattering_parameters)
-from .modes import Mode, add_mode, eigenmode, eigenmode_batch, get_modes, setup_mode
+from .modes import (Mode, add_mode, eigenmode, eigenmode_batch, get_modes, setup_mode,
+                    vector_eigenmode)
 from .modulated import ModulatedDevice, setup_deps, solve_modulated
 
 __all__ = [
@@ -19,5 +21,5 @@ __all__ = [
     "setup_src", "Engine", "eigenfrequency", "engine", "solve", "flux_surface_integral",
     "poynting", "probe_field", "flux_plane", "flux_surface_integrals", "scattering_parameters",
     "Mode", "add_mode", "eigenmode", "eigenmode_batch", "get_modes",
-    "setup_mode", "ModulatedDevice", "setup_deps", "solve_modulated",
+    "setup_mode", "vector_eigenmode", "FieldAll", "ModulatedDevice", "setup_deps", "solve_modulated",
 ]

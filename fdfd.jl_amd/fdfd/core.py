@@ -183,3 +183,10 @@ class FieldTM(Field):
 class FieldTE(Field):
     def __init__(self, grid, omega, data):
         super().__init__(grid, complex(omega), data, ("Hz", "Ex", "Ey"))
+
+
+class FieldAll(Field):
+    """src/data.jl:80-92: all six components (full-vector modes)."""
+
+    def __init__(self, grid, omega, data):
+        super().__init__(grid, complex(omega), data, ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))

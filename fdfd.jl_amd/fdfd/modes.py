@@ -80,6 +80,31 @@ def eigenmode(eps1d: np.ndarray, dh: float, pol: int, omega: float, neff, nev: i
     return beta[0], vecs[0]
 
 
+def vector_eigenmode(d: Device, neff, nev: int = 1, precision="fp64", tol: float = 0.0,
+                     ncv: int = 0, maxiter: int = 0, eng=None):
+    """eigenmode(d::AbstractDevice{2}, neff, nev) (src/solver/eigen.jl:31-66):
+    full-vector modes of the cross-section d.eps_r at d.omega[0] (periodic,
+    no PML).  Returns (beta[nev], [FieldAll]) nearest (omega/c0 neff)^2 first."""
+    from .core import FieldAll
+    from .solve import PRECISIONS, engine
+    eng = eng or engine()
+    eng.set_grid(d.grid)
+    eng.set_eps(d.eps_r)
+    g = d.grid
+    N = len(g)
+    beta = np.empty(nev, dtype=np.complex128)
+    fl = np.empty((nev, 6, N), dtype=np.complex128)
+    st = _lib.fdfd_stats()
+    ne = complex(neff)
+    eng._chk(eng.lib.fdfd_vector_eigenmode(
+        eng.h, float(d.omega[0]), _lib.fdfd_complex(ne.real, ne.imag), int(nev), int(ncv),
+        PRECISIONS[precision], float(tol), int(maxiter), beta.ctypes.data_as(C.c_void_p),
+        fl.ctypes.data_as(C.c_void_p), C.byref(st)))
+    data = fl.reshape(nev, 6, g.N[1], g.N[0]).transpose(0, 3, 2, 1)
+    omega = float(d.omega[0])
+    return beta, [FieldAll(g, omega, data[i]) for i in range(nev)]
+
+
 def slice_indices(g: Grid, pt: Point, slicenormal: int, slicewidth: float):
     """src/device.jl:125-142: 1-based index arrays of the odd-length slice
     through pt and its cell size.  An x-normal slice runs along y

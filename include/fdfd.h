@@ -20,6 +20,7 @@
  *   dolinearsolve(A, b, sym)  (lu(A)\b)   src/solver/solver.jl:4-41
  *   eigenfrequency(d, pol, nev; which)    src/solver/eigen.jl:69-115
  *   eigenmode(d::Device{1}, pol, neff, n) src/solver/eigen.jl:6-29
+ *   eigenmode(d::AbstractDevice{2}, ...)  src/solver/eigen.jl:31-66
  *   solve(d::ModulatedDevice)             src/solver/modulation.jl:35-119
  *   poynting / flux_surface_integral      src/flux.jl:15-66
  *     (get_modes / setup_mode!)           src/device.jl:118-149
@@ -196,6 +197,22 @@ int fdfd_eigenmode(fdfd_handle* h, int nprob, const fdfd_mode_problem* probs,
                    int nev, int ncv, double L0, double tol, int maxiter,
                    fdfd_complex* beta, fdfd_complex* vectors, int* nconv,
                    fdfd_stats* stats);
+
+/* ---- full-vector waveguide modes --------------------------------------- */
+/* eigenmode(d::AbstractDevice{2}, neff, nev) (src/solver/eigen.jl:31-66) on
+ * the handle's grid and eps (periodic differences, no PML, as the reference):
+ * the nev eigenvalues beta^2 of the (Hx, Hy) operator of eigen.jl:45 nearest
+ * (omega/c0' neff)^2, nearest first, as beta = sqrt(beta^2).
+ *   fields: may be NULL; nev * Nx*Ny*6, each Ex, Ey, Ez, Hx, Hy, Hz
+ *           (the FieldAll components of data.jl:86-88), from eigen.jl:56-61,
+ *           with (Hx, Hy) of unit 2-norm and its largest entry real positive.
+ *   ncv <= 0: max(20, 2 nev + 1); at most 64.  tol, maxiter as for
+ *   fdfd_eigenfrequency.  precision: FDFD_FP64 or FDFD_FP32_IR (the factors
+ *   of the two diagonal blocks; FP64 FGMRES refines every application). */
+int fdfd_vector_eigenmode(fdfd_handle* h, double omega, fdfd_complex neff,
+                          int nev, int ncv, int precision, double tol,
+                          int maxiter, fdfd_complex* beta, fdfd_complex* fields,
+                          fdfd_stats* stats);
 
 /* ---- modulated (MF-FDFD) driven solve ---------------------------------- */
 /* solve(d::ModulatedDevice) of src/solver/modulation.jl:35-119 at one

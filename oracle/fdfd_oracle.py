@@ -28,6 +28,8 @@ see tests/test_oracle.py):
   eigenfrequency ................... Kronecker-sum closed form (uniform eps, PML)
   modulated (MF-FDFD) system ....... per-Fourier-mode closed form (uniform eps and
                                      deps, npml=0); A^T = W^-1 A W with W = Sxb Syb
+  2-D full-vector eigenmode ........ closed-form spectrum (uniform eps); slabs uniform
+                                     along x or y reduce exactly to the 1-D TM mode
   1-D eigenmode (mode sources) ..... closed-form spectrum (uniform eps, TM and TE);
                                      lattice slab dispersion relations (TM and TE)
                                      derived independently; hand-computed slices
@@ -586,6 +588,68 @@ def setup_mode(d: Device, pol: int, omega: float, neff, pt, srcnormal: int, srcw
     _, v, ix, iy = get_modes(d, pol, omega, neff, 1, pt, srcnormal, srcwidth)
     a = np.abs(v[:, 0])
     d.src[np.ix_(ix - 1, iy - 1)] += (a / np.linalg.norm(a)).reshape(ix.size, iy.size)
+
+
+# ---------------------------------------------------------------------------
+# full-vector waveguide modes: src/solver/eigen.jl:31-66
+# ---------------------------------------------------------------------------
+@dataclass
+class FieldAll:
+    """src/data.jl:80-92: data is Nx x Ny x 6 (Ex, Ey, Ez, Hx, Hy, Hz)."""
+    grid: Grid
+    omega: complex
+    data: np.ndarray
+
+
+def vector_mode_operator(g: Grid, eps_r: np.ndarray, omega: float):
+    """eigen.jl:35-45: the (Hx, Hy) operator
+      w^2 mu0 blockdiag(Tey, Tex) + blockdiag(Tey, Tex) [-dyf; dxf] Tezinv [-dyb, dxb]
+      + [dxb; dyb] [dxf, dyf]
+    with Tex = eps0 avg_x eps, Tey = eps0 avg_y eps, Tezinv = 1/(eps0 avg_y avg_x eps);
+    plain periodic differences (no PML)."""
+    eps0, mu0, _ = normalize_parameters(g.L0)
+    Tex = sp.diags(eps0 * grid_average(eps_r, XHAT).ravel(order="F"))
+    Tey = sp.diags(eps0 * grid_average(eps_r, YHAT).ravel(order="F"))
+    Tezi = sp.diags((eps0 * grid_average(grid_average(eps_r, XHAT), YHAT).ravel(order="F")) ** -1)
+    dxb, dxf = delta(XHAT, BACKWARD, g), delta(XHAT, FORWARD, g)
+    dyb, dyf = delta(YHAT, BACKWARD, g), delta(YHAT, FORWARD, g)
+    B = sp.block_diag([Tey, Tex])
+    return (omega ** 2 * mu0 * B + B @ sp.vstack([-dyf, dxf]) @ Tezi @ sp.hstack([-dyb, dxb])
+            + sp.vstack([dxb, dyb]) @ sp.hstack([dxf, dyf])).tocsc()
+
+
+def vector_eigenmode(g: Grid, eps_r: np.ndarray, omega: float, neff, nev: int):
+    """eigenmode(d::AbstractDevice{2}, neff, nev), eigen.jl:31-66: ARPACK
+    shift-invert (eigs with sigma = (w/c0 neff)^2), nearest first; fields of
+    eigen.jl:54-61 from (Hx, Hy) of unit norm with the largest entry real
+    positive (ARPACK's phase is arbitrary)."""
+    eps0, mu0, c0 = normalize_parameters(g.L0)
+    A = vector_mode_operator(g, eps_r, omega)
+    sigma = (omega / c0 * complex(neff)) ** 2
+    N = len(g)
+    lam, vec = spla.eigs(A, k=nev, sigma=sigma, which="LM", tol=1e-14)
+    order = np.argsort(np.abs(lam - sigma), kind="stable")
+    lam, vec = lam[order], vec[:, order]
+    beta = np.sqrt(lam.astype(np.complex128))
+    Texi = sp.diags((eps0 * grid_average(eps_r, XHAT).ravel(order="F")) ** -1)
+    Teyi = sp.diags((eps0 * grid_average(eps_r, YHAT).ravel(order="F")) ** -1)
+    Tezi = sp.diags((eps0 * grid_average(grid_average(eps_r, XHAT), YHAT).ravel(order="F")) ** -1)
+    dxb, dxf = delta(XHAT, BACKWARD, g), delta(XHAT, FORWARD, g)
+    dyb, dyf = delta(YHAT, BACKWARD, g), delta(YHAT, FORWARD, g)
+    fields = []
+    for i in range(nev):
+        v = vec[:, i] / np.linalg.norm(vec[:, i])
+        k = int(np.argmax(np.abs(v)))
+        v = v * (np.conj(v[k]) / abs(v[k]))
+        hx, hy = v[:N], v[N:]
+        b = beta[i]
+        hz = 1 / (1j * b) * (dxf @ hx + dyf @ hy)
+        ex = 1 / (1j * omega) * (Texi @ (dyb @ hz + 1j * b * hy))
+        ey = 1 / (1j * omega) * (Teyi @ (-1j * b * hx - dxb @ hz))
+        ez = 1 / (1j * omega) * (Tezi @ (dxb @ hy - dyb @ hx))
+        data = np.stack([c.reshape(g.N, order="F") for c in (ex, ey, ez, hx, hy, hz)], axis=2)
+        fields.append(FieldAll(g, complex(omega), data))
+    return beta, fields, lam, sigma
 
 
 # ---------------------------------------------------------------------------

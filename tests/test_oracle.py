@@ -617,3 +617,48 @@ def test_scattering_parameters_modal_propagation():
     # the solved fields go as exp(-i k x) downstream of the source
     dphi = np.angle(s2 / s1)
     assert abs(np.angle(np.exp(1j * (dphi + k * D)))) < 1e-5
+
+
+# --- full-vector waveguide modes (eigen.jl:31-66) -----------------------------
+def test_vector_mode_uniform_spectrum():
+    """Uniform eps: the polarization coupling vanishes (the periodic
+    differences commute) and both blocks are Laplacians, so the spectrum is
+    k0^2 eps - (4/dx^2) sin^2(pi p/Nx) - (4/dy^2) sin^2(pi q/Ny), each twice."""
+    g = O.Grid.create(0.05, (0, 0), (0, 0.6), (0, 0.5))
+    om = O.omega_of_wavelength(1.55)
+    eps = 2.5 + 0.1j
+    A = O.vector_mode_operator(g, np.full(g.N, eps), om).toarray()
+    Nx, Ny = g.N
+    P, Q = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    base = (K0SQ(om) * eps - 4 / g.dx ** 2 * np.sin(np.pi * P / Nx) ** 2
+            - 4 / g.dy ** 2 * np.sin(np.pi * Q / Ny) ** 2).ravel()
+    ref = np.sort_complex(np.concatenate([base, base]))
+    got = np.sort_complex(np.linalg.eigvals(A))
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("axis", ["y", "x"])
+def test_vector_mode_slab_reduces_to_1d_tm(axis):
+    """A slab uniform along x (eps varies with y only): (Hx, Hy) = (0, h(y))
+    uniform in x is an exact eigenvector, with h the 1-D TM mode of the eps(y)
+    slice (A_yy = k0^2 avg_x eps + dyb dyf, and avg_x eps = eps; A_xy kills
+    x-uniform fields).  Likewise (h(x), 0) for a slab uniform along y.  The
+    fundamental vector mode's beta must equal the 1-D TM beta."""
+    om = O.omega_of_wavelength(1.55)
+    prof = np.concatenate([np.full(20, 2.25), np.full(9, 12.25), np.full(20, 2.25)]) + 0j
+    if axis == "y":
+        g = O.Grid.create(0.02, (0, 0), (0, 0.12), (0, 0.98))
+        eps = np.tile(prof, (g.N[0], 1))
+    else:
+        g = O.Grid.create(0.02, (0, 0), (0, 0.98), (0, 0.12))
+        eps = np.tile(prof[:, None], (1, g.N[1]))
+    b1, v1, _ = O.eigenmode_1d(prof, 0.02, om, O.TM, 3.0, 1)
+    beta, fields, _, _ = O.vector_eigenmode(g, eps, om, b1[0] / np.sqrt(K0SQ(om)) * (1 + 1e-3), 1)
+    assert abs(beta[0] - b1[0]) <= 1e-10 * abs(b1[0])
+    f = fields[0].data
+    h = f[:, :, 4] if axis == "y" else f[:, :, 3]   # Hy or Hx carries the mode
+    other = f[:, :, 3] if axis == "y" else f[:, :, 4]
+    assert np.abs(other).max() <= 1e-8 * np.abs(h).max()
+    line = h[0, :] if axis == "y" else h[:, 0]
+    line = line / np.linalg.norm(line)
+    assert np.abs(np.abs(line) - np.abs(v1[:, 0])).max() <= 1e-8
