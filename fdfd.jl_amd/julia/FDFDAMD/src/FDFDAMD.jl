@@ -138,6 +138,29 @@ function eigenmode(d::FDFD.AbstractDevice{1}, pol::FDFD.Polarization, neff::Numb
     return (β, v)
 end
 
+"""
+    eigenmode(d::AbstractDevice{2}, neff, neigenvalues; precision=:fp64)
+
+Same contract as FDFD.eigenmode for a 2-D cross-section (src/solver/eigen.jl:31-66):
+the propagation constants β nearest ω/c₀·neff and their `FieldAll` fields.
+"""
+function eigenmode(d::FDFD.AbstractDevice{2}, neff::Number, neigenvalues::Int;
+                   precision::Symbol=:fp64, tol::Real=1e-12, ncv::Int=0, maxiter::Int=300,
+                   handle::Handle=default_handle())
+    h = handle
+    (Nx, Ny) = set_problem!(h, d)
+    β = Vector{ComplexF64}(undef, neigenvalues)
+    data = Array{ComplexF64}(undef, Nx, Ny, 6, neigenvalues)
+    st = Ref(Stats())
+    GC.@preserve β data check(h, ccall((:fdfd_vector_eigenmode, libfdfd), Cint,
+        (Ptr{Cvoid}, Cdouble, ComplexF64, Cint, Cint, Cint, Cdouble, Cint, Ptr{ComplexF64},
+         Ptr{ComplexF64}, Ref{Stats}),
+        h.ptr, d.ω[1], ComplexF64(neff), neigenvalues, ncv, precision_code(precision), tol,
+        maxiter, β, data, st))
+    fields = [FDFD.FieldAll(d.grid, d.ω[1], data[:, :, :, i]) for i in 1:neigenvalues]
+    return (β, fields)
+end
+
 # src/device.jl:125-142: indices of the odd-length slice through pt and its cell size
 function slice_indices(g::FDFD.Grid, pt, slicenormal, slicewidth::Number)
     (indx, indy) = FDFD.coord2ind(g, pt)
@@ -379,6 +402,7 @@ function install!()
         $(flux_surface_integral)(f, c, w, n)
     @eval FDFD eigenmode(d::AbstractDevice{1}, pol::Polarization, neff::Number, n::Int) =
         $(eigenmode)(d, pol, neff, n)
+    @eval FDFD eigenmode(d::AbstractDevice{2}, neff::Number, n::Int) = $(eigenmode)(d, neff, n)
     @eval FDFD eigenfrequency(d::AbstractDevice, pol::Polarization, n::Int; which::Symbol=:LM) =
         $(eigenfrequency)(d, pol, n; which=which)
     return nothing
