@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the mode-source and modulated-solve legs (headline only)")
     ap.add_argument("--precisions", default="mixed,fp64,mixed-fp32gemm",
                     help="mixed: complex64 factors with bf16x3 GEMM products (default mixed "
                          "path); mixed-fp32gemm: complex64 factors with FP32 MFMA products; "
@@ -381,9 +383,10 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
-        "modes": mode_leg(eng, d, not a.no_cpu),
-        "modulated": modulated_leg(eng, not a.no_cpu),
     }
+    if not a.no_legs:
+        line["modes"] = mode_leg(eng, d, not a.no_cpu)
+        line["modulated"] = modulated_leg(eng, not a.no_cpu)
     for p in (eps_d, src_d, out_d):
         hip.free(p)
     eng.close()

@@ -5,9 +5,9 @@
 // pivoting.  Pivot rows are not swapped; they are tracked by index.  For a
 // panel of nb columns K with original values V = A[:, K] and pivot rows p:
 //   * k_panel    : GEPP pivot selection on the not-yet-pivoted rows of V
-//                  (one workgroup, one row per thread in registers), then the
-//                  inverse Vpinv of the pivot block V[p, :] = L U;
-//   * k_zbuild   : Z = Vpinv * [A[p, C] | I_K]              (nb x n);
+//                  (one workgroup, one row per thread in registers), giving
+//                  the factors of the pivot block V[p, :] = L U;
+//   * k_zbuild   : Z = U^-1 L^-1 [A[p, C] | I_K]            (nb x n);
 //   * k_trailing : A[i, :] = A'[i, :] - V[i, :] Z  (i not in p), A[p, :] = Z,
 //                  a rank-nb complex update on MFMA (A' = A with K zeroed).
 // After the last panel the stored Y holds the inverse X = D^-1 as
@@ -65,43 +65,32 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long k) {
 __device__ inline unsigned long long mag_key(double m) { return (unsigned long long)__double_as_longlong(m); }
 __device__ inline unsigned long long mag_key(float m) { return (unsigned long long)(unsigned)__float_as_int(m); }
 
-// Inverse of the pivot block V[piv, :] = L U (unit lower L, upper U, nb x nb,
-// in LDS).  Thread c < nb computes column c, Vpinv[:, c] = U^-1 L^-1 e_c, in
-// Xs.  Then every thread helps write Vpinv (first PMAX x PMAX block of LU)
-// and U (second block) to the job's LU buffer, rows k < nb.
+// Factors of the pivot block V[piv, :] = L U (unit lower L, upper U, nb x nb,
+// in LDS) to the job's LU buffer: L in the first PMAX x PMAX block, U in the
+// second, rows k < nb (columns >= nb zero).  Z is built from them by two
+// triangular solves (k_zbuild).
 template <typename C>
-__device__ inline void finish_pivot_block(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
-                                          C (*Xs)[PMAX + 1], C* Ud, int nb, int t, int nthr,
-                                          C* LUg) {
+__device__ inline void store_pivot_factors(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1], int nb, int t,
+                                           int nthr, C* LUg) {
     using R = decltype(C{}.x);
-    if (t < nb) Ud[t] = crcp(Us[t][t]);
-    __syncthreads();
-    if (t < nb) {
-        const int c = t;
-        for (int k = 0; k < nb; ++k) {  // y = L^-1 e_c (zero above row c)
-            C y = mk<C>(k == c ? R(1) : R(0), R(0));
-            for (int q = c; q < k; ++q) y = csub(y, cmul(Ls[k][q], Xs[q][c]));
-            Xs[k][c] = y;
-        }
-        for (int k = nb - 1; k >= 0; --k) {  // x = U^-1 y
-            C x = Xs[k][c];
-            for (int q = k + 1; q < nb; ++q) x = csub(x, cmul(Us[k][q], Xs[q][c]));
-            Xs[k][c] = cmul(x, Ud[k]);
-        }
-    }
-    __syncthreads();
-    C* Vg = LUg;
+    C* Lg = LUg;
     C* Ug = LUg + PMAX * PMAX;
     for (int e = t; e < nb * PMAX; e += nthr) {
         const int k = e / PMAX, c = e % PMAX;
-        C v = mk<C>(R(0), R(0)), u = mk<C>(R(0), R(0));
+        C l = mk<C>(R(0), R(0)), u = mk<C>(R(0), R(0));
         if (c < nb) {
-            v = Xs[k][c];
+            l = Ls[k][c];
             u = Us[k][c];
         }
-        gstore(Vg + e, v);
+        gstore(Lg + e, l);
         gstore(Ug + e, u);
     }
+}
+
+// lane k of this lane's half-wave (lanes 0-31 or 32-63)
+template <typename C> __device__ inline C shfl_half(C v, int k) {
+    const int src = (int)(threadIdx.x & 32) + k;
+    return mk<C>(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
 // ---------------------------------------------------------------------------
@@ -111,8 +100,9 @@ __device__ inline void finish_pivot_block(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
 // threshold partial pivoting, |pivot| >= PIV_U * max |candidate| over every
 // unpivoted row, or verr is raised and the factorization is redone with the
 // full-column search.
-// Outputs: pivots, the inverse of the pivot block V[piv, :] and its U factor,
-// updated window list.
+// Outputs: pivots, the L and U factors of the pivot block V[piv, :], updated
+// window list.  The window's rows are read through LDS (lanes across columns,
+// coalesced) before each lane takes its row into registers.
 // ---------------------------------------------------------------------------
 template <typename R, int NB>
 __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh,
@@ -126,8 +116,8 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     __shared__ R pmags[NB];
     __shared__ C Ls[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
-    __shared__ C Xs[PMAX][PMAX + 1];
-    __shared__ C Ud[PMAX];
+    __shared__ C Ws[WIN][NB + 1];
+    __shared__ int wrow[WIN];
     const int row = (lane < nwin) ? gload_s(J.win + lane) : -1;
     bool act = row >= 0;
     int mystep = -1;  // panel step at which this lane's row became the pivot
@@ -136,11 +126,22 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     const C* src = second ? (const C*)J.Vbuf + PMAX : (const C*)J.Y + k0;
     const long ld = second ? VMAX : n;
     const int poff = second ? PMAX : 0;
+    static_assert(NB == 32 && WIN == 64, "the staged window load pairs two rows per wave");
+    wrow[lane] = row;
+    __syncthreads();
+    for (int rr = 0; rr < WIN; rr += 2) {
+        const int ri = rr + (lane >> 5), c = lane & 31;
+        const int rw = wrow[ri];
+        C v = mk<C>(R(0), R(0));
+        if (rw >= 0 && c < nb) v = gload(src + (long)rw * ld + c);
+        Ws[ri][c] = v;
+    }
+    __syncthreads();
     C W[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
         W[c] = mk<C>(R(0), R(0));
-        if (act && c < nb) W[c] = gload(src + (long)row * ld + c);
+        if (act && c < nb) W[c] = Ws[lane][c];
     }
     static_for<0, NB>([&](auto ic) {
         constexpr int cc = decltype(ic)::value;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
         }
     }
     __syncthreads();
-    finish_pivot_block<C>(Ls, Us, Xs, Ud, nb, lane, 64, (C*)J.LU);
+    store_pivot_factors<C>(Ls, Us, nb, lane, 64, (C*)J.LU);
     if (lane < nb) {
         const int p = prs[lane];
         J.piv[poff + lane] = p;
@@ -242,8 +243,6 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
     __shared__ R pmags[NB];
     __shared__ C Ls[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
-    __shared__ C Xs[PMAX][PMAX + 1];
-    __shared__ C Ud[PMAX];
     const C* A = (const C*)J.Y;
     const int r = t;
     const bool valid = r < n;
@@ -302,7 +301,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    finish_pivot_block<C>(Ls, Us, Xs, Ud, NB, t, 1024, (C*)J.LU);
+    store_pivot_factors<C>(Ls, Us, NB, t, 1024, (C*)J.LU);
     if (t < NB) {
         const int p = prs[t];
         J.piv[t] = p;
@@ -338,14 +337,16 @@ template void launch_panel<double>(hipStream_t, const LineJob*, int, int, int, i
 template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// Z build, panel copy and pivot check (one thread per column or row).
-//   blocks [0, nzb):  Z[:, col] = Vpinv T[:, col], T[k][col] = A[piv[k]][col]
-//                     (delta(k, col-k0) for col in K), Vpinv = V[piv, :]^-1.
-//   blocks [nzb, ..): copy V = A[r, K] to the panel buffer (every row r) and,
-//                     if verify, check rows r >= fresh: the value GEPP would
-//                     compare at step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c]
-//                     with y_c = a_c / U[c][c], must satisfy
-//                     PIV_U * |a_c| <= |U[c][c]| (threshold pivoting).
+// Z build, panel copy and pivot check.
+//   blocks [0, nzb):  one thread per column:
+//                     Z[:, col] = U^-1 L^-1 T[:, col], T[k][col] = A[piv[k]][col]
+//                     (delta(k, col-k0) for col in K), V[piv, :] = L U;
+//   blocks [nzb, ..): 32 rows per workgroup, 32 lanes per row (coalesced):
+//                     copy V = A[r, K] to the panel buffer and, if verify,
+//                     check rows r >= fresh: the value GEPP would compare at
+//                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
+//                     y_c = a_c / U[c][c], must satisfy PIV_U |a_c| <= |U[c][c]|
+//                     (threshold pivoting).
 // Second panel of a fused pair (columns K2 after K1 = [k1, k1 + nb1)): its
 // pivot rows are taken after the first panel's update,
 //   T[k][col] = [col not in K1] A[p2_k][col] - sum_q V1[p2_k][q] Z1[q][col],
@@ -359,22 +360,23 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ C Vi[PMAX][PMAX + 1];
+    __shared__ C Ls[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
     __shared__ C V1p[PMAX][PMAX + 1];  // second panel: V1 rows of its pivots
     __shared__ C Ud[PMAX];
     __shared__ int pv[PMAX];
     const C* LUg = (const C*)J.LU;
     const int poff = second ? PMAX : 0;
+    const bool zpart = (int)blockIdx.x < nzb;
     for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
-        Vi[e / PMAX][e % PMAX] = gload(LUg + e);
+        if (zpart) Ls[e / PMAX][e % PMAX] = gload(LUg + e);
         Us[e / PMAX][e % PMAX] = gload(LUg + PMAX * PMAX + e);
     }
     if ((int)threadIdx.x < nb) pv[threadIdx.x] = gload_s(J.piv + poff + threadIdx.x);
     __syncthreads();
     if ((int)threadIdx.x < nb) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
     const C* Vb = (const C*)J.Vbuf;
-    if (second)
+    if (second && zpart)
         for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
             const int k = e / PMAX, q = e % PMAX;
             C v = mk<C>(R(0), R(0));
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     __syncthreads();
     const C* A = (const C*)J.Y;
     C* Zg = (C*)J.Z;
-    if ((int)blockIdx.x < nzb) {
+    if (zpart) {
         const int col = blockIdx.x * blockDim.x + threadIdx.x;
         if (col >= n) return;
         const bool inK = col >= k0 && col < k0 + nb;
@@ -406,42 +408,53 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
                     if (k < nb) t[k] = csub(t[k], cmul(V1p[k][q], z1));
             }
         }
-        for (int k = 0; k < nb; ++k) {
-            C z = mk<C>(R(0), R(0));
-#pragma unroll
-            for (int q = 0; q < PMAX; ++q)
-                if (q < nb) z = cadd(z, cmul(Vi[k][q], t[q]));
-            gstore(Zg + (long)(poff + k) * n + col, z);
-        }
-    } else {
-        const int r = ((int)blockIdx.x - nzb) * blockDim.x + threadIdx.x;
-        if (r >= n) return;
-        C v[PMAX];
-        C* Vr = (C*)J.Vbuf + (long)r * VMAX + poff;
-#pragma unroll
-        for (int c = 0; c < PMAX; ++c) {
-            v[c] = mk<C>(R(0), R(0));
-            if (c < nb) v[c] = second ? gload(Vr + c) : gload(A + (long)r * n + k0 + c);
-        }
-        if (!second) {
-#pragma unroll
-            for (int c = 0; c < PMAX; ++c)
-                if (c < nb) gstore(Vr + c, v[c]);
-        }
-        if (verify && r >= fresh) {
-            int bad = 0;
-#pragma unroll
-            for (int k = 0; k < PMAX; ++k) {
-                if (k < nb) {
-                    const C ak = v[k];
-                    if (R(PIV_U) * cabs1(ak) > cabs1(Us[k][k])) bad = 1;
-                    const C yk = cmul(ak, Ud[k]);
-#pragma unroll
-                    for (int c = k + 1; c < PMAX; ++c)
-                        if (c < nb) v[c] = csub(v[c], cmul(yk, Us[k][c]));
-                }
+        // t = L^-1 t (unit lower), then z = U^-1 t
+        static_for<1, PMAX>([&](auto ik) {
+            constexpr int k = decltype(ik)::value;
+            if (k < nb) {
+                C sacc = t[k];
+                static_for<0, k>([&](auto iq) {
+                    constexpr int q = decltype(iq)::value;
+                    sacc = csub(sacc, cmul(Ls[k][q], t[q]));
+                });
+                t[k] = sacc;
             }
-            if (bad) *(GLOBAL_AS int*)J.verr = 1;
+        });
+        static_for<0, PMAX>([&](auto ik) {
+            constexpr int k = PMAX - 1 - decltype(ik)::value;
+            if (k < nb) {
+                C sacc = t[k];
+                static_for<k + 1, PMAX>([&](auto iq) {
+                    constexpr int q = decltype(iq)::value;
+                    if (q < nb) sacc = csub(sacc, cmul(Us[k][q], t[q]));
+                });
+                t[k] = cmul(sacc, Ud[k]);
+            }
+        });
+        static_for<0, PMAX>([&](auto ik) {
+            constexpr int k = decltype(ik)::value;
+            if (k < nb) gstore(Zg + (long)(poff + k) * n + col, t[k]);
+        });
+    } else {
+        const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
+        const int rbase = ((int)blockIdx.x - nzb) * 32;
+        for (int rr = half; rr < 32; rr += 8) {
+            const int r = rbase + rr;  // the same for the 32 lanes of a half-wave
+            if (r >= n) break;
+            C* Vr = (C*)J.Vbuf + (long)r * VMAX + poff;
+            C v = mk<C>(R(0), R(0));
+            if (c < nb) v = second ? gload(Vr + c) : gload(A + (long)r * n + k0 + c);
+            if (!second && c < nb) gstore(Vr + c, v);
+            if (verify && r >= fresh) {
+                bool bad = false;
+                for (int k = 0; k < nb; ++k) {
+                    const C ak = shfl_half(v, k);
+                    if (R(PIV_U) * cabs1(ak) > cabs1(Us[k][k])) bad = true;
+                    const C yk = cmul(ak, Ud[k]);
+                    if (c > k && c < nb) v = csub(v, cmul(yk, Us[k][c]));
+                }
+                if (bad && c == 0) *(GLOBAL_AS int*)J.verr = 1;
+            }
         }
     }
 }
@@ -450,7 +463,7 @@ template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int verify, int second, int k1, int nb1) {
     const int nzb = (n + 255) / 256;
-    const int nvb = (n + 255) / 256;
+    const int nvb = (n + 31) / 32;
     k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb,
                                                         second, k1, nb1);
 }
@@ -461,7 +474,8 @@ template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, i
 
 // ---------------------------------------------------------------------------
 // Thin update of a fused pair: the second panel's columns K2 = [k1 + nb1,
-// k1 + nb1 + nb2) after the first panel, one thread per row:
+// k1 + nb1 + nb2) after the first panel, 32 lanes per row (lane c = column),
+// 32 rows per workgroup:
 //   V2[r] = Z1[slot, K2]                        for the first panel's pivots,
 //   V2[r] = A[r, K2] - V1[r] Z1[:, K2]           for every other row,
 // into columns PMAX.. of the panel buffer.
@@ -481,36 +495,30 @@ __global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int n
         Z1k[q][c] = v;
     }
     __syncthreads();
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    C* Vr = (C*)J.Vbuf + (long)r * VMAX;
-    const int slot = gload_s(J.iprow + r) - k1;
-    C w[PMAX];
-    if (slot >= 0 && slot < nb1) {
-#pragma unroll
-        for (int c = 0; c < PMAX; ++c) w[c] = Z1k[slot][c];
-    } else {
-        const C* A = (const C*)J.Y + (long)r * n + k2;
-#pragma unroll
-        for (int c = 0; c < PMAX; ++c) {
-            w[c] = mk<C>(R(0), R(0));
-            if (c < nb2) w[c] = gload(A + c);
+    const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
+    const int rbase = blockIdx.x * 32;
+    for (int rr = half; rr < 32; rr += 8) {
+        const int r = rbase + rr;  // the same for the 32 lanes of a half-wave
+        if (r >= n) break;
+        C* Vr = (C*)J.Vbuf + (long)r * VMAX;
+        const int slot = gload_s(J.iprow + r) - k1;
+        C w = mk<C>(R(0), R(0));
+        if (slot >= 0 && slot < nb1) {
+            w = Z1k[slot][c];
+        } else {
+            if (c < nb2) w = gload((const C*)J.Y + (long)r * n + k2 + c);
+            C v1 = mk<C>(R(0), R(0));
+            if (c < nb1) v1 = gload(Vr + c);
+            for (int q = 0; q < nb1; ++q) w = csub(w, cmul(shfl_half(v1, q), Z1k[q][c]));
         }
-        for (int q = 0; q < nb1; ++q) {
-            const C v1 = gload(Vr + q);
-#pragma unroll
-            for (int c = 0; c < PMAX; ++c) w[c] = csub(w[c], cmul(v1, Z1k[q][c]));
-        }
+        if (c < nb2) gstore(Vr + PMAX + c, w);
     }
-#pragma unroll
-    for (int c = 0; c < PMAX; ++c)
-        if (c < nb2) gstore(Vr + PMAX + c, w[c]);
 }
 
 template <typename R>
 void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1, int nb1,
                  int nb2) {
-    k_thin<R><<<dim3((n + 255) / 256, njobs), 256, 0, s>>>(jobs_d, k1, nb1, nb2);
+    k_thin<R><<<dim3((n + 31) / 32, njobs), 256, 0, s>>>(jobs_d, k1, nb1, nb2);
 }
 template void launch_thin<double>(hipStream_t, const LineJob*, int, int, int, int, int);
 template void launch_thin<float>(hipStream_t, const LineJob*, int, int, int, int, int);
