@@ -36,6 +36,9 @@ template <typename C> __device__ inline void gstore(C* p, C c) {
     *(GLOBAL_AS V*)p = v;
 }
 template <typename T> __device__ inline T gload_s(const T* p) { return *(const GLOBAL_AS T*)p; }
+// two adjacent complex floats as one 16-byte access (p 16-byte aligned)
+__device__ inline f4v gload2c(const float2* p) { return *(const GLOBAL_AS f4v*)p; }
+__device__ inline void gstore2c(float2* p, f4v v) { *(GLOBAL_AS f4v*)p = v; }
 
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {

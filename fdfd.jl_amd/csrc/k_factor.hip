@@ -665,10 +665,200 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
         }
 }
 
+// FP32 trailing update for lines with n % 4 == 0: the same product as
+// k_trailing<float> with the operands of the MFMA swapped (A = Z^T, B = V^T),
+// so each lane holds one row i and four consecutive columns j = 4 (lane / 16)
+// + v of the result and reads and writes them as two 16-byte accesses.
+//   * V and Z tiles are fetched as 16-byte pairs into registers without
+//     branches (out-of-range lanes read offset 0; a bit mask applied when
+//     the pair is stored to LDS zeroes them), so a fused pair's second chunk
+//     is in flight while the first chunk's MFMAs run.
+//   * The old tile values are fetched after chunk 0 is staged; the row slots
+//     come from LDS, so no global load forces an early wait.
+//   * The flat grid is ordered so consecutive tiles of a line land on one
+//     XCD (workgroups are dealt to the 8 XCDs round-robin) and share its L2.
+// n % 4 == 0 keeps every 16-byte access aligned and inside its row.
+__global__ __launch_bounds__(256) void k_trailing_f32(const LineJob* jobs, int k0, int nb1, int nb2,
+                                                      int ntx, int total) {
+    using R = float;
+    using C = cplx<R>;
+    using M = mfma_t<R>;
+    using acc_t = typename M::acc;
+    constexpr int NB = PMAX;
+    // XCD-aware tile order: the first 8 * per workgroups are dealt so that
+    // XCD x (= L % 8) runs the contiguous tile range [x per, (x + 1) per)
+    const int L = blockIdx.x, per = total / 8;
+    const int lin = (L < 8 * per) ? (L % 8) * per + L / 8 : L;
+    const int job = lin / (ntx * ntx), rem = lin % (ntx * ntx);
+    const LineJob J = jobs[job];
+    const int n = J.n;
+    const int i0 = (rem / ntx) * 64, j0 = (rem % ntx) * 64;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    const bool fused = nb2 > 0;
+    const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
+    const int k2 = k0 + PMAX;                               // start of K2 (fused)
+    __shared__ R Vr[64][NB + 1], Vi[64][NB + 1];
+    __shared__ R Zr[NB][64 + 1], Zi[NB][64 + 1];
+    __shared__ int rslot[64];
+    C* A = (C*)J.Y;
+    const C* Vb = (const C*)J.Vbuf;
+    const C* Zg = (const C*)J.Z;
+    if (t < 64) rslot[t] = (i0 + t < n) ? gload_s(J.iprow + i0 + t) - k0 : -1;
+    acc_t are[2][2], aim[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            are[bi][bj] = acc_t{0, 0, 0, 0};
+            aim[bi][bj] = acc_t{0, 0, 0, 0};
+        }
+    __syncthreads();  // rslot
+    // element pairs: V[ii][kp, kp + 1] with ii = e / 16, kp = 2 (e % 16);
+    // Z[k][jp, jp + 1] with k = e / 32, jp = 2 (e % 32); e = t + 256 q
+    f4v pv[4], pz[4];
+    unsigned msk = 0;  // bits 2q, 2q+1: V pair q halves; bits 8+2q, 9+2q: Z pair q halves
+    auto fetch = [&](int ch) {
+        const int koff = ch * PMAX, nbc = (ch == 0) ? nb1 : nb2;
+        msk = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const int ii = e >> 4, kp = (e & 15) * 2;
+            const int i = i0 + ii;
+            // the first panel's pivot rows take no V1 part in a fused pair
+            const bool skip = fused && ch == 0 && rslot[ii] >= 0 && rslot[ii] < PMAX;
+            const bool okv = i < n && !skip;
+            msk |= (unsigned)(okv && kp < nbc) << (2 * q);
+            msk |= (unsigned)(okv && kp + 1 < nbc) << (2 * q + 1);
+            pv[q] = gload2c(Vb + (okv ? (long)i * VMAX + koff + kp : 0));
+            const int k = e >> 5, j = j0 + (e & 31) * 2;
+            const bool okz = k < nbc && j < n;
+            // Z1' : Z1 with the K2 columns zeroed
+            const bool ex0 = fused && ch == 0 && j >= k2 && j < kend;
+            const bool ex1 = fused && ch == 0 && j + 1 >= k2 && j + 1 < kend;
+            msk |= (unsigned)(okz && !ex0) << (8 + 2 * q);
+            msk |= (unsigned)(okz && !ex1) << (9 + 2 * q);
+            pz[q] = gload2c(Zg + (okz ? (long)(koff + k) * n + j : 0));
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const int ii = e >> 4, kp = (e & 15) * 2;
+            const bool a0 = (msk >> (2 * q)) & 1, a1 = (msk >> (2 * q + 1)) & 1;
+            Vr[ii][kp] = a0 ? pv[q][0] : R(0);
+            Vi[ii][kp] = a0 ? pv[q][1] : R(0);
+            Vr[ii][kp + 1] = a1 ? pv[q][2] : R(0);
+            Vi[ii][kp + 1] = a1 ? pv[q][3] : R(0);
+            const int k = e >> 5, jp = (e & 31) * 2;
+            const bool b0 = (msk >> (8 + 2 * q)) & 1, b1 = (msk >> (9 + 2 * q)) & 1;
+            Zr[k][jp] = b0 ? pz[q][0] : R(0);
+            Zi[k][jp] = b0 ? pz[q][1] : R(0);
+            Zr[k][jp + 1] = b1 ? pz[q][2] : R(0);
+            Zi[k][jp + 1] = b1 ? pz[q][3] : R(0);
+        }
+    };
+    // old values of this lane's row i, columns jb .. jb + 3 (two 16-byte
+    // pairs); exclusions are applied in the epilogue
+    f4v raw[2][2][2];
+    int slot[2];
+    auto load_old = [&]() {
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int ii = wr * 32 + bi * 16 + lr, i = i0 + ii;
+            const int sl = rslot[ii];
+            slot[bi] = sl;
+            const bool p1 = fused && sl >= 0 && sl < PMAX;
+            const C* src = p1 ? Zg : A;
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
+                const long off = (i < n && jb < n) ? (p1 ? (long)sl * n + jb : (long)i * n + jb) : 0;
+                raw[bi][bj][0] = gload2c(src + off);
+                raw[bi][bj][1] = gload2c(src + off + 2);
+            }
+        }
+    };
+    fetch(0);
+    const int nchunk = fused ? 2 : 1;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int nbc = (ch == 0) ? nb1 : nb2;
+        if (ch > 0) __syncthreads();  // the previous chunk's tiles are consumed
+        stash();
+        if (ch == 0) {
+            if (fused) fetch(1);
+            load_old();
+        }
+        __syncthreads();
+        const int nbr = (nbc + 3) & ~3;
+        for (int kk = 0; kk < nbr; kk += 4) {
+            R ar[2], ai[2], br[2], bi_[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ar[q] = Vr[wr * 32 + q * 16 + lr][kk + lk];
+                ai[q] = Vi[wr * 32 + q * 16 + lr][kk + lk];
+                br[q] = Zr[kk + lk][wc * 32 + q * 16 + lr];
+                bi_[q] = Zi[kk + lk][wc * 32 + q * 16 + lr];
+            }
+            // D[j][i] += sum_k Z[k][j] V[i][k]: result row = column j of the tile
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    are[bi][bj] = M::run(br[bj], ar[bi], are[bi][bj]);
+                    are[bi][bj] = M::run(-bi_[bj], ai[bi], are[bi][bj]);
+                    aim[bi][bj] = M::run(bi_[bj], ar[bi], aim[bi][bj]);
+                    aim[bi][bj] = M::run(br[bj], ai[bi], aim[bi][bj]);
+                }
+        }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int i = i0 + wr * 32 + bi * 16 + lr;
+        if (i >= n) continue;
+        const int sl = slot[bi];
+        const bool p1 = fused && sl >= 0 && sl < PMAX;
+        // rows replaced by Z: the panel's pivots (single) or the second
+        // panel's pivots (fused)
+        const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
+            if (jb >= n) continue;
+            R o[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = jb + v;
+                const bool ex = p1 ? (j >= k2 && j < kend) : (j >= k0 && j < kend);
+                const f4v& rw = raw[bi][bj][v >> 1];
+                const R orr = ex ? R(0) : rw[(v & 1) * 2], oii = ex ? R(0) : rw[(v & 1) * 2 + 1];
+                o[2 * v] = orr - are[bi][bj][v];
+                o[2 * v + 1] = oii - aim[bi][bj][v];
+            }
+            f4v o0 = {o[0], o[1], o[2], o[3]}, o1 = {o[4], o[5], o[6], o[7]};
+            if (zrow) {
+                o0 = gload2c(Zg + (long)sl * n + jb);
+                o1 = gload2c(Zg + (long)sl * n + jb + 2);
+            }
+            gstore2c(A + (long)i * n + jb, o0);
+            gstore2c(A + (long)i * n + jb + 2, o1);
+        }
+    }
+}
+
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
                      int nb2) {
-    dim3 grid((n + 63) / 64, (n + 63) / 64, njobs);
+    const int ntx = (n + 63) / 64;
+    if (std::is_same<R, float>::value && n % 4 == 0) {
+        const int total = ntx * ntx * njobs;
+        k_trailing_f32<<<total, 256, 0, s>>>(jobs_d, k0, nb1, nb2, ntx, total);
+        return;
+    }
+    dim3 grid(ntx, ntx, njobs);
     k_trailing<R><<<grid, 256, 0, s>>>(jobs_d, k0, nb1, nb2);
 }
 template void launch_trailing<double>(hipStream_t, const LineJob*, int, int, int, int, int);
