@@ -339,8 +339,10 @@ def main():
         rt = "float" if head == "mixed" else "double"
         # kernel-name prefix of the dominant slot (the mixed path's GEMM is
         # the split-operand bf16x3 kernel)
+        # (FP32 lines with n % 4 == 0 use the 16-byte-access trailing kernel)
+        trail = "k_trailing_f32" if head == "mixed" and n % 4 == 0 else "k_trailing<" + rt
         prefix = {"gemm": "k_gemm_b3" if head == "mixed" else "k_gemm<" + rt,
-                  "trailing": "k_trailing<" + rt, "solve": "k_mv<" + rt}.get(dom)
+                  "trailing": trail, "solve": "k_mv<" + rt}.get(dom)
         for name, v in P["kernels"].items():
             if prefix and name.startswith(prefix):
                 roof["traffic"] = v["read_bytes"] + v["write_bytes"]

@@ -116,7 +116,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         }
         // frequencies per group: what the free HBM holds (factors + operator)
         const int n = std::min(g.Nx, g.Ny), m = std::max(g.Nx, g.Ny);
-        const double per = (double)rt_factor_bytes(n, m, precision) + 96.0 * N;
+        const double per = (double)rt_factor_bytes(n, m, precision, h->gemm_b3 != 0) + 96.0 * N;
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
         int B = (int)std::max(1.0, std::floor(0.8 * (double)freeb / per - 1.0));

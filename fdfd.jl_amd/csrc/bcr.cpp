@@ -34,10 +34,13 @@ using namespace bcr;
 
 int rt_assemble(Handle* h, Factor* F);
 
-size_t rt_factor_bytes(int n, int m, int precision) {
+size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
     const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
     const Plan P = make_plan(m);
-    return (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
+    size_t bytes = (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
+    // bf16x3 GEMM operands: four tiled bf16 planes per block
+    if (b3 && precision != FDFD_FP64) bytes += (size_t)P.nblocks * 4 * b3_plane(n) * 2;
+    return bytes;
 }
 
 // ---------------------------------------------------------------------------
@@ -84,9 +87,9 @@ static void own_planes(const Factor* F, long N, int line, const double2** sub,
 static void* blk(const Factor* F, int id) {
     return (char*)F->blocks + (size_t)id * F->n * F->n * F->elem;
 }
-// bf16 planes of block id (bf16x3 mode): 4 x n x n bf16
+// bf16 planes of block id (bf16x3 mode): 4 tiled planes of bf16
 static void* blk_b(const Factor* F, int id) {
-    return (char*)F->blocks_b + (size_t)id * F->n * F->n * 8;
+    return (char*)F->blocks_b + (size_t)id * 4 * b3_plane(F->n) * 2;
 }
 
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
@@ -125,7 +128,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                                              " MiB) does not fit in device memory");
         F->blocks_bytes = bytes;
         if (b3) {
-            const size_t bb = (size_t)P.nblocks * n * n * 8;
+            const size_t bb = (size_t)P.nblocks * 4 * b3_plane(n) * 2;
             F->blocks_b = arena_take(h, bb);
             if (!F->blocks_b)
                 return fail(h, FDFD_ERR_OOM, "bf16 operand storage does not fit in device memory");
@@ -441,7 +444,12 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         }
         if ((rc = upload(&F->mv_tab, mvs)) || (rc = upload(&F->dg_tab, dgs))) return rc;
         F->ssteps.clear();
-        for (const Step& s : P.ssteps) F->ssteps.push_back({s.kind == ST_MV ? 0 : 1, s.first, s.count});
+        for (const Step& s : P.ssteps) {
+            int mats = 0;
+            if (s.kind == ST_MV)
+                for (int i = 0; i < s.count; ++i) mats += P.mvs[s.first + i].nt;
+            F->ssteps.push_back({s.kind == ST_MV ? 0 : 1, s.first, s.count, mats});
+        }
     }
     return FDFD_OK;
 }
@@ -455,8 +463,8 @@ static void launch_solve(Handle* h, Factor* F, bool prof) {
     for (const auto& s : F->ssteps) {
         ProfScope* ps = nullptr;
         if (prof) {
-            const double bytes = s.kind == 0 ? (double)n * n * F->elem * s.count : 64.0 * n * s.count;
-            ps = new ProfScope(h, P_SWEEP, s.kind == 0 ? 8.0 * n * n * s.count : 0, bytes);
+            const double bytes = s.kind == 0 ? (double)n * n * F->elem * s.mats : 64.0 * n * s.count;
+            ps = new ProfScope(h, P_SWEEP, s.kind == 0 ? 8.0 * n * n * s.mats : 0, bytes);
         }
         if (s.kind == 0) {
             if (fp64) launch_mv<double>(h->stream, F->mv_tab + s.first, s.count, n);

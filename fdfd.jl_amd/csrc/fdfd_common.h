@@ -145,8 +145,15 @@ struct TriJob {
     long si;
 };
 
-// bf16 planes (re hi, re lo, im hi, im lo; [4][n][n]) of an n x n complex64
-// block: as is, or transposed with a row gather, dst[c][k] = src[map[k]][c].
+// bf16 planes (re hi, re lo, im hi, im lo) of an n x n complex64 block, as
+// is (row r, k = column) or transposed with a row gather (row c, k:
+// src[map[k]][c]), in the tiled operand layout of the bf16x3 GEMM: tiles of
+// 16 rows x 32 k, tile (rb, kb) at (rb * nkb + kb) * 512, entry (r, k) of a
+// tile at (k / 8) * 128 + r * 8 + k % 8, so that lane l of an MFMA fragment
+// reads the 16 bytes at 16 l.  Rows and k past n are zero.
+inline __host__ __device__ int b3_nrb(int n) { return (n + 15) / 16; }
+inline __host__ __device__ int b3_nkb(int n) { return (n + 31) / 32; }
+inline __host__ __device__ long b3_plane(int n) { return (long)b3_nrb(n) * b3_nkb(n) * 512; }
 struct SplitJob {
     const void* src;
     void* dst;

@@ -368,16 +368,18 @@ template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int, in
 template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// bf16x3 products for the mixed path (opt-in).  Each FP32 value x is split
+// bf16x3 products for the mixed path (its default).  Each FP32 value x is split
 // once into bf16 hi = bf16(x), lo = bf16(x - hi); a real product is
 // a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32 by
 // v_mfma_f32_16x16x32_bf16 (relative error ~2^-16 per product, removed by the
 // FP64 refinement).
 //
 // k_split writes the four bf16 planes (re hi, re lo, im hi, im lo) of an
-// n x n complex block, either as is ([row][k], an A operand) or transposed
-// with the row gather of the B operand applied ([col][k] = B[map[k]][col]),
-// so that the GEMM reads every MFMA operand (8 consecutive k) as 16 bytes.
+// n x n complex block in the tiled operand layout (b3_plane): as is (an A
+// operand, row = block row, k = block column) or transposed with the row
+// gather of the B operand applied (row = block column c, k: B[map[k]][c]).
+// A workgroup converts a 64 x 64 piece: 4 x 2 tiles per plane, each written
+// by one wave as 1 KB contiguous.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
@@ -388,50 +390,72 @@ __device__ inline void split_bf16(float x, __bf16& h, __bf16& l) {
 
 __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     const SplitJob J = jobs[blockIdx.z];
-    __shared__ float2 tile[64][65];
-    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    __shared__ float2 tile[64][65];  // [row][k] of the piece
+    const int r0 = blockIdx.y * 64, q0 = blockIdx.x * 64;  // rows r0.., k q0..
     const float2* src = (const float2*)J.src;
-    __bf16* dst = (__bf16*)J.dst;
-    const long plane = (long)n * n;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // all 16 loads of a thread in flight before the LDS stores; entries past n
+    // read src[0] and are stored as zero
+    float2 v[16];
     if (!J.trans) {
-        for (int rr = ty; rr < 64; rr += 4) {
-            const int r = r0 + rr, c = c0 + tx;
-            if (r < n && c < n) {
-                const float2 v = gload(src + (long)r * n + c);
-                __bf16 h, l;
-                const long o = (long)r * n + c;
-                split_bf16(v.x, h, l);
-                dst[o] = h;
-                dst[plane + o] = l;
-                split_bf16(v.y, h, l);
-                dst[2 * plane + o] = h;
-                dst[3 * plane + o] = l;
-            }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = r0 + ty + 4 * i, k = q0 + tx;
+            v[i] = gload(src + ((r < n && k < n) ? (long)r * n + k : 0));
         }
-        return;
-    }
-    // transposed: out[col][k] = B[map[k]][col], tile (k rows r0.., cols c0..)
-    for (int rr = ty; rr < 64; rr += 4) {
-        const int k = r0 + rr, c = c0 + tx;
-        float2 v = make_float2(0.f, 0.f);
-        if (k < n && c < n) v = gload(src + (long)(J.map ? gload_s(J.map + k) : k) * n + c);
-        tile[rr][tx] = v;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = r0 + ty + 4 * i, k = q0 + tx;
+            tile[ty + 4 * i][tx] = (r < n && k < n) ? v[i] : make_float2(0.f, 0.f);
+        }
+    } else {
+        // piece rows are block columns c = r0 + tx; k runs over block rows
+        int mrow[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int k = q0 + ty + 4 * i;
+            mrow[i] = (J.map && k < n) ? gload_s(J.map + k) : k;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int k = q0 + ty + 4 * i, c = r0 + tx;
+            v[i] = gload(src + ((k < n && c < n) ? (long)mrow[i] * n + c : 0));
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int k = q0 + ty + 4 * i, c = r0 + tx;
+            tile[tx][ty + 4 * i] = (k < n && c < n) ? v[i] : make_float2(0.f, 0.f);
+        }
     }
     __syncthreads();
-    for (int cc = ty; cc < 64; cc += 4) {
-        const int c = c0 + cc, k = r0 + tx;
-        if (c < n && k < n) {
-            const float2 v = tile[tx][cc];
+    const long plane = b3_plane(n);
+    const int nrb = b3_nrb(n), nkb = b3_nkb(n);
+    __bf16* dst = (__bf16*)J.dst;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tid = ty + 4 * q;                   // tile of the piece, 0..7
+        const int rbl = tid & 3, kbl = tid >> 2;      // 4 row blocks x 2 k blocks
+        const int rb = r0 / 16 + rbl, kb = q0 / 32 + kbl;
+        if (rb >= nrb || kb >= nkb) continue;         // uniform per wave
+        const int r = rbl * 16 + (lane & 15), kc = kbl * 32 + (lane >> 4) * 8;
+        bf8v rh, rl, ih, il;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float2 x = tile[r][kc + e];
             __bf16 h, l;
-            const long o = (long)c * n + k;
-            split_bf16(v.x, h, l);
-            dst[o] = h;
-            dst[plane + o] = l;
-            split_bf16(v.y, h, l);
-            dst[2 * plane + o] = h;
-            dst[3 * plane + o] = l;
+            split_bf16(x.x, h, l);
+            rh[e] = h;
+            rl[e] = l;
+            split_bf16(x.y, h, l);
+            ih[e] = h;
+            il[e] = l;
         }
+        const long o = ((long)rb * nkb + kb) * 512 + 8 * lane;
+        *(GLOBAL_AS bf8v*)(dst + o) = rh;
+        *(GLOBAL_AS bf8v*)(dst + plane + o) = rl;
+        *(GLOBAL_AS bf8v*)(dst + 2 * plane + o) = ih;
+        *(GLOBAL_AS bf8v*)(dst + 3 * plane + o) = il;
     }
 }
 
@@ -439,70 +463,68 @@ void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n) {
     k_split<<<dim3((n + 63) / 64, (n + 63) / 64, njobs), 256, 0, s>>>(jobs_d, n);
 }
 
-// GEMM on split operands: A = planes [4][n][n] as [row][k], B = planes of
-// B^T as [col][k] (gather already applied), both bf16.  Complex products in
-// the four-multiplication form (the three-multiplication form would need six
-// operand planes):  re += Ar Br - Ai Bi,  im += Ar Bi + Ai Br.
+// GEMM on split operands (tiled planes, b3_plane): A as [row][k], B^T as
+// [col][k] with the gather already applied.  Complex products in the
+// four-multiplication form with Ai Bi kept in its own accumulator:
+//   re = Ar Br - Ai Bi,  im = Ar Bi + Ai Br,
+// each real product as hi hi + hi lo + lo hi on v_mfma_f32_16x16x32_bf16.
 // Tile 128 x 128, 2 x 4 waves of 64 x 32, K staged 32 at a time through
-// double-buffered LDS; a 64-byte LDS row holds the 32 k of one row and
-// plane, its four 16-byte chunks XOR-swizzled by (row / 4) so that the 16
-// rows of an MFMA fragment read fall in distinct banks.
-constexpr int B3_TM = 128, B3_TN = 128, B3_KT = 32;
+// double-buffered LDS.  Every operand access of a stage (global load, LDS
+// store, fragment read) is one wave reading or writing a contiguous 1 KB
+// tile, 16 bytes per lane, which keeps the LDS accesses conflict-free.  The
+// flat grid is ordered so each XCD (workgroups are dealt to the 8 XCDs
+// round-robin) works through a contiguous range of jobs and keeps their
+// operand panels in its own L2.
+constexpr int B3_TM = 128, B3_TN = 128;
 
-__device__ inline int b3_chunk(int row, int c) { return c ^ ((row >> 2) & 3); }
-
-__global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n) {
-    __shared__ __align__(16) bf8v sA[2][4][B3_TM][4];
-    __shared__ __align__(16) bf8v sB[2][4][B3_TN][4];
-    const GemmJob J = jobs[blockIdx.z];
-    const int i0 = blockIdx.y * B3_TM, j0 = blockIdx.x * B3_TN;
+__global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int ntx, int total) {
+    __shared__ __align__(16) bf8v sA[2][4][8][64];  // [buffer][plane][row block][lane]
+    __shared__ __align__(16) bf8v sB[2][4][8][64];
+    const int L = blockIdx.x, per = total / 8;
+    const int lin = (L < 8 * per) ? (L % 8) * per + L / 8 : L;
+    const int job = lin / (ntx * ntx), rem = lin % (ntx * ntx);
+    const GemmJob J = jobs[job];
+    const int i0 = (rem / ntx) * B3_TM, j0 = (rem % ntx) * B3_TN;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int wr = wv >> 2, wc = wv & 3;
     const int lr = lane & 15, lg = lane >> 4;
-    const long plane = (long)n * n;
-    const bool vec = (n % 8) == 0;  // 16-byte chunks are aligned
-    const int spt = (n + B3_KT - 1) / B3_KT;
-    const int ns = J.nt * spt;
-    // loader: 2048 chunks per operand per stage (4 planes x 128 rows x 4
-    // chunks), four per thread
+    const int nkb = b3_nkb(n), nrb = b3_nrb(n);
+    const long plane = b3_plane(n);
+    const int ns = J.nt * nkb;
+    // loader: per stage and operand 4 planes x 8 row blocks of 1 KB, four
+    // 16-byte pieces per thread (a wave loads one whole tile)
     bf8v ra[4], rb[4];
-    auto ld = [&](const __bf16* base, int row0, int kb, int e, bf8v& out) {
-        const int id = t + 512 * e;  // chunk id
-        const int pl = id >> 9, row = (id >> 2) & 127, c = id & 3;
-        const int r = row0 + row, k = kb + 8 * c;
-        const __bf16* p = base + pl * plane + (long)r * n + k;
-        if (r < n && k + 8 <= n && vec) {
-            out = *(const GLOBAL_AS bf8v*)p;
-        } else {
-            for (int q = 0; q < 8; ++q) out[q] = (r < n && k + q < n) ? p[q] : (__bf16)0.f;
-        }
-    };
     auto load = [&](int s) {
-        const int ts = s / spt, kb = (s - ts * spt) * B3_KT;
+        const int ts = s / nkb, kb = s - ts * nkb;
         const __bf16* A = (const __bf16*)(ts ? J.A[1] : J.A[0]);
         const __bf16* B = (const __bf16*)(ts ? J.B[1] : J.B[0]);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            ld(A, i0, kb, e, ra[e]);
-            ld(B, j0, kb, e, rb[e]);
+            const int id = t + 512 * e;
+            const int pl = id >> 9, blk = (id >> 6) & 7, l = id & 63;
+            const int ta = i0 / 16 + blk, tb = j0 / 16 + blk;
+            // past the last row block the loads read tile 0 (zeroed at the store)
+            ra[e] = *(const GLOBAL_AS bf8v*)(A + (ta < nrb ? pl * plane + ((long)ta * nkb + kb) * 512 + 8 * l : 0));
+            rb[e] = *(const GLOBAL_AS bf8v*)(B + (tb < nrb ? pl * plane + ((long)tb * nkb + kb) * 512 + 8 * l : 0));
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int id = t + 512 * e;
-            const int pl = id >> 9, row = (id >> 2) & 127, c = id & 3;
-            sA[buf][pl][row][b3_chunk(row, c)] = ra[e];
-            sB[buf][pl][row][b3_chunk(row, c)] = rb[e];
+            const int pl = id >> 9, blk = (id >> 6) & 7, l = id & 63;
+            sA[buf][pl][blk][l] = (i0 / 16 + blk < nrb) ? ra[e] : bf8v{};
+            sB[buf][pl][blk][l] = (j0 / 16 + blk < nrb) ? rb[e] : bf8v{};
         }
     };
-    f4v cre[4][2], cim[4][2];
+    f4v cre[4][2], cim[4][2], cii[4][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             cre[p][q] = f4v{0, 0, 0, 0};
             cim[p][q] = f4v{0, 0, 0, 0};
+            cii[p][q] = f4v{0, 0, 0, 0};
         }
     load(0);
     store(0);
@@ -512,22 +534,13 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n) {
         if (s + 1 < ns) load(s + 1);
         bf8v a[4][4], b[2][4];  // [block][plane]: re hi, re lo, im hi, im lo
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const int row = wr * 64 + p * 16 + lr;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[p][v] = sA[cur][v][row][b3_chunk(row, lg)];
-        }
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int col = wc * 32 + q * 16 + lr;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b[q][v] = sB[cur][v][col][b3_chunk(col, lg)];
-        }
-        bf8v na[4][2];  // -Ai (hi, lo)
-#pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
-            for (int v = 0; v < 2; ++v) na[p][v] = -a[p][2 + v];
+            for (int v = 0; v < 4; ++v) a[p][v] = sA[cur][v][wr * 4 + p][lane];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[q][v] = sB[cur][v][wc * 2 + q][lane];
         // (x_hi y_hi + x_hi y_lo + x_lo y_hi) for the four real products,
         // blocks innermost so consecutive MFMAs use different accumulators
 #define B3_MM(ACC, X0, X1, Y0, Y1)                                                             \
@@ -537,10 +550,10 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n) {
         ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, Y1, ACC[p][q], 0, 0, 0);         \
     _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
         ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, Y0, ACC[p][q], 0, 0, 0);
-        B3_MM(cre, a[p][0], a[p][1], b[q][0], b[q][1])    // + Ar Br
-        B3_MM(cre, na[p][0], na[p][1], b[q][2], b[q][3])  // - Ai Bi
-        B3_MM(cim, a[p][0], a[p][1], b[q][2], b[q][3])    // + Ar Bi
-        B3_MM(cim, a[p][2], a[p][3], b[q][0], b[q][1])    // + Ai Br
+        B3_MM(cre, a[p][0], a[p][1], b[q][0], b[q][1])  // Ar Br
+        B3_MM(cim, a[p][0], a[p][1], b[q][2], b[q][3])  // Ar Bi
+        B3_MM(cii, a[p][2], a[p][3], b[q][2], b[q][3])  // Ai Bi
+        B3_MM(cim, a[p][2], a[p][3], b[q][0], b[q][1])  // Ai Br
 #undef B3_MM
         if (s + 1 < ns) store(cur ^ 1);
         __syncthreads();
@@ -563,15 +576,16 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n) {
                     ox = old.x;
                     oy = old.y;
                 }
-                const float re = cre[p][q][v], im = cim[p][q][v];
+                const float re = cre[p][q][v] - cii[p][q][v], im = cim[p][q][v];
                 gstore(o, J.sub ? make_float2(ox - re, oy - im) : make_float2(ox + re, oy + im));
             }
         }
 }
 
 void launch_gemm_b3(hipStream_t s, const GemmJob* jobs_d, int njobs, int n) {
-    const dim3 grid((n + B3_TN - 1) / B3_TN, (n + B3_TM - 1) / B3_TM, njobs);
-    k_gemm_b3<<<grid, 512, 0, s>>>(jobs_d, n);
+    const int ntx = (n + B3_TM - 1) / B3_TM;
+    const int total = ntx * ntx * njobs;
+    k_gemm_b3<<<total, 512, 0, s>>>(jobs_d, n, ntx, total);
 }
 
 // ---------------------------------------------------------------------------

@@ -129,12 +129,22 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     static_assert(NB == 32 && WIN == 64, "the staged window load pairs two rows per wave");
     wrow[lane] = row;
     __syncthreads();
-    for (int rr = 0; rr < WIN; rr += 2) {
-        const int ri = rr + (lane >> 5), c = lane & 31;
-        const int rw = wrow[ri];
-        C v = mk<C>(R(0), R(0));
-        if (rw >= 0 && c < nb) v = gload(src + (long)rw * ld + c);
-        Ws[ri][c] = v;
+    {
+        // all WIN / 2 loads in flight before the first LDS store: lanes past
+        // the window or the panel read src[0] and store zero
+        C v[WIN / 2];
+        unsigned okm = 0;
+        const int c = lane & 31;
+#pragma unroll
+        for (int q = 0; q < WIN / 2; ++q) {
+            const int rw = wrow[2 * q + (lane >> 5)];
+            const bool ok = rw >= 0 && c < nb;
+            okm |= (unsigned)ok << q;
+            v[q] = gload(src + (ok ? (long)rw * ld + c : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < WIN / 2; ++q)
+            Ws[2 * q + (lane >> 5)][c] = ((okm >> q) & 1) ? v[q] : mk<C>(R(0), R(0));
     }
     __syncthreads();
     C W[NB];
@@ -368,21 +378,43 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     const C* LUg = (const C*)J.LU;
     const int poff = second ? PMAX : 0;
     const bool zpart = (int)blockIdx.x < nzb;
-    for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
-        if (zpart) Ls[e / PMAX][e % PMAX] = gload(LUg + e);
-        Us[e / PMAX][e % PMAX] = gload(LUg + PMAX * PMAX + e);
+    constexpr int EPT = PMAX * PMAX / 256;  // LU entries per thread (blockDim.x == 256)
+    {
+        // every load of the staging is issued before the first LDS store;
+        // rows k >= nb of L and U are loaded but never read
+        const int t = threadIdx.x;
+        const int p = gload_s(J.piv + poff + (t < nb ? t : 0));
+        C lv[EPT], uv[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = t + 256 * q;
+            if (zpart) lv[q] = gload(LUg + e);
+            uv[q] = gload(LUg + PMAX * PMAX + e);
+        }
+        if (t < nb) pv[t] = p;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = t + 256 * q;
+            if (zpart) Ls[e / PMAX][e % PMAX] = lv[q];
+            Us[e / PMAX][e % PMAX] = uv[q];
+        }
     }
-    if ((int)threadIdx.x < nb) pv[threadIdx.x] = gload_s(J.piv + poff + threadIdx.x);
     __syncthreads();
     if ((int)threadIdx.x < nb) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
     const C* Vb = (const C*)J.Vbuf;
-    if (second && zpart)
-        for (int e = threadIdx.x; e < nb * PMAX; e += blockDim.x) {
-            const int k = e / PMAX, q = e % PMAX;
-            C v = mk<C>(R(0), R(0));
-            if (q < nb1) v = gload(Vb + (long)pv[k] * VMAX + q);
-            V1p[k][q] = v;
+    if (second && zpart) {
+        C v[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, k = e / PMAX, c = e % PMAX;
+            v[q] = gload(Vb + ((k < nb && c < nb1) ? (long)pv[k] * VMAX + c : 0));
         }
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, k = e / PMAX, c = e % PMAX;
+            V1p[k][c] = (k < nb && c < nb1) ? v[q] : mk<C>(R(0), R(0));
+        }
+    }
     __syncthreads();
     const C* A = (const C*)J.Y;
     C* Zg = (C*)J.Z;
@@ -391,21 +423,34 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
         if (col >= n) return;
         const bool inK = col >= k0 && col < k0 + nb;
         const bool inK1 = second && col >= k1 && col < k1 + nb1;
+        const bool ldA = !inK && !inK1;
         C t[PMAX];
+        // the pivot-row loads are issued together (rows k >= nb and columns
+        // in K or K1 read A[0] and are replaced below)
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
-            t[k] = mk<C>(R(0), R(0));
-            if (k < nb) {
-                if (inK) t[k].x = (col - k0 == k) ? R(1) : R(0);
-                else if (!inK1) t[k] = gload(A + (long)pv[k] * n + col);
-            }
+            const bool ok = ldA && k < nb;
+            t[k] = gload(A + (ok ? (long)pv[k] * n + col : 0));
+        }
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            if (!(ldA && k < nb)) t[k] = mk<C>((inK && k < nb && col - k0 == k) ? R(1) : R(0), R(0));
         }
         if (second && !inK) {
-            for (int q = 0; q < nb1; ++q) {
-                const C z1 = gload(Zg + (long)q * n + col);
+            // Z1 rows eight at a time, loads first
+            for (int q0 = 0; q0 < nb1; q0 += 8) {
+                C z1[8];
 #pragma unroll
-                for (int k = 0; k < PMAX; ++k)
-                    if (k < nb) t[k] = csub(t[k], cmul(V1p[k][q], z1));
+                for (int u = 0; u < 8; ++u)
+                    z1[u] = gload(Zg + (q0 + u < nb1 ? (long)(q0 + u) * n + col : 0));
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (q0 + u < nb1) {
+#pragma unroll
+                        for (int k = 0; k < PMAX; ++k)
+                            if (k < nb) t[k] = csub(t[k], cmul(V1p[k][q0 + u], z1[u]));
+                    }
+                }
             }
         }
         // t = L^-1 t (unit lower), then z = U^-1 t
@@ -438,12 +483,22 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     } else {
         const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
         const int rbase = ((int)blockIdx.x - nzb) * 32;
-        for (int rr = half; rr < 32; rr += 8) {
-            const int r = rbase + rr;  // the same for the 32 lanes of a half-wave
+        // the four rows of this half-wave (rows half + 8 i), loads first;
+        // out-of-range lanes read the first panel entry and are zeroed
+        const C* vsrc = second ? (const C*)J.Vbuf + poff : A + k0;
+        C vrow[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = rbase + half + 8 * i;
+            const bool ok = r < n && c < nb;
+            vrow[i] = gload(vsrc + (ok ? (second ? (long)r * VMAX : (long)r * n) + c : 0));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = rbase + half + 8 * i;  // the same for the 32 lanes of a half-wave
             if (r >= n) break;
             C* Vr = (C*)J.Vbuf + (long)r * VMAX + poff;
-            C v = mk<C>(R(0), R(0));
-            if (c < nb) v = second ? gload(Vr + c) : gload(A + (long)r * n + k0 + c);
+            C v = (c < nb) ? vrow[i] : mk<C>(R(0), R(0));
             if (!second && c < nb) gstore(Vr + c, v);
             if (verify && r >= fresh) {
                 bool bad = false;
@@ -488,30 +543,53 @@ __global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int n
     __shared__ C Z1k[PMAX][PMAX + 1];  // Z1[q][K2 column c]
     const C* Zg = (const C*)J.Z;
     const int k2 = k1 + nb1;
-    for (int e = threadIdx.x; e < PMAX * PMAX; e += blockDim.x) {
-        const int q = e / PMAX, c = e % PMAX;
-        C v = mk<C>(R(0), R(0));
-        if (q < nb1 && c < nb2) v = gload(Zg + (long)q * n + k2 + c);
-        Z1k[q][c] = v;
+    constexpr int EPT = PMAX * PMAX / 256;  // blockDim.x == 256
+    {
+        C v[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / PMAX, c = e % PMAX;
+            v[q] = gload(Zg + ((r < nb1 && c < nb2) ? (long)r * n + k2 + c : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / PMAX, c = e % PMAX;
+            Z1k[r][c] = (r < nb1 && c < nb2) ? v[q] : mk<C>(R(0), R(0));
+        }
     }
     __syncthreads();
     const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
     const int rbase = blockIdx.x * 32;
-    for (int rr = half; rr < 32; rr += 8) {
-        const int r = rbase + rr;  // the same for the 32 lanes of a half-wave
+    // the four rows of this half-wave (rows half + 8 i): slots, then the row
+    // loads, all in flight together; out-of-range lanes read entry 0
+    const C* Y = (const C*)J.Y;
+    const C* Vb = (const C*)J.Vbuf;
+    int slot[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rbase + half + 8 * i;
+        slot[i] = gload_s(J.iprow + (r < n ? r : 0)) - k1;
+    }
+    C wv[4], v1v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rbase + half + 8 * i;
+        wv[i] = gload(Y + (r < n && c < nb2 ? (long)r * n + k2 + c : 0));
+        v1v[i] = gload(Vb + (r < n && c < nb1 ? (long)r * VMAX + c : 0));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rbase + half + 8 * i;  // the same for the 32 lanes of a half-wave
         if (r >= n) break;
-        C* Vr = (C*)J.Vbuf + (long)r * VMAX;
-        const int slot = gload_s(J.iprow + r) - k1;
-        C w = mk<C>(R(0), R(0));
-        if (slot >= 0 && slot < nb1) {
-            w = Z1k[slot][c];
+        C w;
+        if (slot[i] >= 0 && slot[i] < nb1) {
+            w = Z1k[slot[i]][c];
         } else {
-            if (c < nb2) w = gload((const C*)J.Y + (long)r * n + k2 + c);
-            C v1 = mk<C>(R(0), R(0));
-            if (c < nb1) v1 = gload(Vr + c);
+            w = (c < nb2) ? wv[i] : mk<C>(R(0), R(0));
+            const C v1 = (c < nb1) ? v1v[i] : mk<C>(R(0), R(0));
             for (int q = 0; q < nb1; ++q) w = csub(w, cmul(shfl_half(v1, q), Z1k[q][c]));
         }
-        if (c < nb2) gstore(Vr + PMAX + c, w);
+        if (c < nb2) gstore((C*)J.Vbuf + (long)r * VMAX + PMAX + c, w);
     }
 }
 

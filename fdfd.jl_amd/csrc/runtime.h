@@ -57,7 +57,8 @@ struct Factor {
     size_t blocks_bytes = 0, blocks_b_bytes = 0;
     int *prow = nullptr, *iprow = nullptr;  // m * n Gauss-Jordan pivot maps per line
     double* pivstat = nullptr;     // 3 per line
-    struct SolveStep { int kind, first, count; };  // kind 0: MvJob range, 1: DiagJob range
+    // kind 0: MvJob range, 1: DiagJob range; mats: n x n blocks the step streams
+    struct SolveStep { int kind, first, count, mats; };
     std::vector<SolveStep> ssteps;
     MvJob* mv_tab = nullptr;
     DiagJob* dg_tab = nullptr;
@@ -181,7 +182,7 @@ int rt_gmres_op(Handle* h, long N, const std::function<int(const double2*, doubl
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);
 int rt_build_factor(Handle* h, Factor* F);
 int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out);
-size_t rt_factor_bytes(int n, int m, int precision);
+size_t rt_factor_bytes(int n, int m, int precision, bool b3);
 
 }  // namespace fdfd
 

@@ -115,6 +115,34 @@ def test_mixed_gemm_modes_match_oracle(monkeypatch, gemm, Nx, Ny, npml):
         eng.close()
 
 
+@pytest.mark.parametrize("Nx,Ny,npml", [(37, 90, 6), (90, 50, 6)])
+def test_bf16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
+    """On line lengths that are not multiples of the 16 x 32 operand tiles
+    (37, 50), the bf16x3 products stay close enough to the FP32 ones that the
+    refinement needs about as many iterations: a wrong or stale operand tile
+    would leave FGMRES converging, but only with a much weaker preconditioner."""
+    d = _device(Nx, Ny, npml, seed=5 + Ny)
+    om = float(d.omega[0])
+    A = O.assemble(d.grid, d.eps_r, om, O.TM)
+    b = O.rhs(om, d.src)
+    its = {}
+    for gemm in ("fp32", "bf16x3"):
+        monkeypatch.setenv("FDFD_MIXED_GEMM", gemm)  # read when the handle is created
+        eng = Engine(0)
+        try:
+            _engine_grid(eng, d)
+            fid, _ = eng.factor(O.TM, om, "mixed")
+            x, st = eng.linsolve(fid, b, tol=1e-13)
+            eng.release(fid)
+        finally:
+            eng.close()
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        its[gemm] = st["iterations"]
+        print(f"{Nx}x{Ny} gemm={gemm}: res={res:.2e} its={its[gemm]}", flush=True)
+        assert res <= 1e-12
+    assert its["bf16x3"] <= its["fp32"] + 4
+
+
 @pytest.mark.parametrize("switch", ["FDFD_FUSE_PANELS=0", "FDFD_GEMM3M=0", "FDFD_GEMM_WAVES=4",
                                     "FDFD_GRAPHS=0"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
