@@ -97,7 +97,7 @@ struct LineJob {
     int* prow;        // n: prow[c] = pivot row of column c
     int* iprow;       // n: inverse of prow (-1 until pivoted)
     void* Vbuf;       // n x VMAX panel copies V (two panels of a fused pair)
-    void* LU;         // 2 x PMAX x PMAX: inverse of the pivot block, then its U factor
+    void* LU;         // 2 x PMAX x PMAX: V[piv, :]^-1 of the pivot block, then U^-1
     int* piv;         // VMAX: pivot rows of the two panels
     void* Z;          // VMAX x n: V[piv, :]^-1 * [A(piv, :) | I] of the two panels
     int* win;         // 64: candidate-row window (first unpivoted rows by index)

@@ -65,25 +65,56 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long k) {
 __device__ inline unsigned long long mag_key(double m) { return (unsigned long long)__double_as_longlong(m); }
 __device__ inline unsigned long long mag_key(float m) { return (unsigned long long)(unsigned)__float_as_int(m); }
 
-// Factors of the pivot block V[piv, :] = L U (unit lower L, upper U, nb x nb,
-// in LDS) to the job's LU buffer: L in the first PMAX x PMAX block, U in the
-// second, rows k < nb (columns >= nb zero).  Z is built from them by two
-// triangular solves (k_zbuild).
+// lane k of this lane's half-wave (lanes 0-31 or 32-63), k wave-uniform
+template <typename C> __device__ inline C bcast_half(C v, int k) {
+    const bool hi = (threadIdx.x & 32) != 0;
+    return mk<C>(hi ? bcast(v.x, 32 + k) : bcast(v.x, k), hi ? bcast(v.y, 32 + k) : bcast(v.y, k));
+}
+
+// Inverse factors of the pivot block V[piv, :] = L U (unit lower L, upper U,
+// nb x nb in LDS) to the job's LU buffer: M = U^-1 L^-1 = V[piv, :]^-1 in the
+// first PMAX x PMAX block, U^-1 in the second, both zero outside nb x nb.
+// A half-wave solves for one column j of L^-1 (forward) and of U^-1
+// (backward) at a time: lane i holds row i and each solved entry is
+// broadcast with readlane.  M is then
+// formed as the product U^-1 L^-1 through LDS (Lis, Uis: scratch).  Called by
+// all nthr threads of the block (a multiple of 64) with Ls, Us complete.
 template <typename C>
-__device__ inline void store_pivot_factors(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1], int nb, int t,
-                                           int nthr, C* LUg) {
+__device__ inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
+                                            C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], int nb,
+                                            int t, int nthr, C* LUg) {
     using R = decltype(C{}.x);
-    C* Lg = LUg;
-    C* Ug = LUg + PMAX * PMAX;
-    for (int e = t; e < nb * PMAX; e += nthr) {
-        const int k = e / PMAX, c = e % PMAX;
-        C l = mk<C>(R(0), R(0)), u = mk<C>(R(0), R(0));
-        if (c < nb) {
-            l = Ls[k][c];
-            u = Us[k][c];
+    const C zero = mk<C>(R(0), R(0)), one = mk<C>(R(1), R(0));
+    const int i = t & 31;
+    const bool rok = i < nb;
+    const C dinv = csel(rok, crcp(Us[i][i]), zero);
+    for (int j0 = 2 * (t >> 6); j0 < PMAX; j0 += nthr / 32) {
+        const int j = j0 + ((t >> 5) & 1);
+        C x = csel(i == j, one, zero);  // L^-1 e_j
+        C y = csel(i == j, one, zero);  // U^-1 e_j
+        // forward over k, backward over kb = nb - 1 - k; rows and columns >= nb
+        // of Ls, Us are never used
+        for (int k = 0; k < nb; ++k) {
+            const int kb = nb - 1 - k;
+            const C xk = bcast_half(x, k);
+            if (i > k && rok) x = csub(x, cmul(Ls[i][k], xk));
+            if (i == kb) y = cmul(y, dinv);
+            const C yk = bcast_half(y, kb);
+            if (i < kb) y = csub(y, cmul(Us[i][kb], yk));
         }
-        gstore(Lg + e, l);
-        gstore(Ug + e, u);
+        const bool ok = rok && j < nb;
+        Lis[i][j] = csel(ok, x, zero);
+        Uis[i][j] = csel(ok, y, zero);
+    }
+    __syncthreads();
+    C* Mg = LUg;
+    C* Ug = LUg + PMAX * PMAX;
+    for (int e = t; e < PMAX * PMAX; e += nthr) {
+        const int r = e / PMAX, c = e % PMAX;
+        C sacc = zero;
+        for (int k = 0; k < PMAX; ++k) sacc = cadd(sacc, cmul(Uis[r][k], Lis[k][c]));
+        gstore(Mg + e, sacc);
+        gstore(Ug + e, Uis[r][c]);
     }
 }
 
@@ -94,23 +125,27 @@ template <typename C> __device__ inline C shfl_half(C v, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Fast panel (one wave): GEPP on the candidate window, the first WIN
-// unpivoted rows by index.  Lane l owns window row win[l].  Every chosen
+// Fast panel: GEPP on the candidate window, the first WIN unpivoted rows by
+// index, by wave 0 (lane l owns window row win[l]); the whole workgroup
+// (WP_THREADS) loads the window and forms the inverse factors.  Every chosen
 // pivot is then checked against all rows outside the window (k_zbuild):
 // threshold partial pivoting, |pivot| >= PIV_U * max |candidate| over every
 // unpivoted row, or verr is raised and the factorization is redone with the
 // full-column search.
-// Outputs: pivots, the L and U factors of the pivot block V[piv, :], updated
-// window list.  The window's rows are read through LDS (lanes across columns,
-// coalesced) before each lane takes its row into registers.
+// Outputs: pivots, V[piv, :]^-1 and U^-1 of the pivot block, updated window
+// list.  The window's rows are read through LDS (lanes across columns,
+// coalesced) before each lane of wave 0 takes its row into registers.
 // ---------------------------------------------------------------------------
+constexpr int WP_THREADS = 512;
+
 template <typename R, int NB>
-__global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int nb, int fresh,
-                                                int second) {
+__global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, int k0, int nb,
+                                                        int fresh, int second) {
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.x];
     const int n = J.n;
-    const int lane = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
+    const bool w0 = t < 64;
     const int nwin = min(WIN, n - k0);
     __shared__ int prs[NB];
     __shared__ R pmags[NB];
@@ -118,7 +153,13 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     __shared__ C Us[PMAX][PMAX + 1];
     __shared__ C Ws[WIN][NB + 1];
     __shared__ int wrow[WIN];
-    const int row = (lane < nwin) ? gload_s(J.win + lane) : -1;
+    static_assert(NB == PMAX && WIN == 2 * PMAX, "window and panel shapes");
+    static_assert(WIN * (NB + 1) >= 2 * PMAX * (PMAX + 1), "the inverse scratch reuses the window");
+    int row = -1;
+    if (w0) {
+        row = (lane < nwin) ? gload_s(J.win + lane) : -1;
+        wrow[lane] = row;
+    }
     bool act = row >= 0;
     int mystep = -1;  // panel step at which this lane's row became the pivot
     // panel values: the block's columns, or (second panel of a pair) the
@@ -126,79 +167,83 @@ __global__ __launch_bounds__(64) void k_wpanel(const LineJob* jobs, int k0, int 
     const C* src = second ? (const C*)J.Vbuf + PMAX : (const C*)J.Y + k0;
     const long ld = second ? VMAX : n;
     const int poff = second ? PMAX : 0;
-    static_assert(NB == 32 && WIN == 64, "the staged window load pairs two rows per wave");
-    wrow[lane] = row;
     __syncthreads();
     {
-        // all WIN / 2 loads in flight before the first LDS store: lanes past
-        // the window or the panel read src[0] and store zero
-        C v[WIN / 2];
-        unsigned okm = 0;
-        const int c = lane & 31;
+        // 32 lanes per row, WIN / 16 rows per thread; all loads in flight
+        // before the first LDS store (lanes past the window or the panel
+        // read src[0] and store zero)
+        constexpr int RPT = WIN * 32 / WP_THREADS;
+        C v[RPT];
+        const int c = t & 31, r0 = t >> 5;
 #pragma unroll
-        for (int q = 0; q < WIN / 2; ++q) {
-            const int rw = wrow[2 * q + (lane >> 5)];
-            const bool ok = rw >= 0 && c < nb;
-            okm |= (unsigned)ok << q;
-            v[q] = gload(src + (ok ? (long)rw * ld + c : 0));
+        for (int q = 0; q < RPT; ++q) {
+            const int rw = wrow[r0 + 16 * q];
+            v[q] = gload(src + ((rw >= 0 && c < nb) ? (long)rw * ld + c : 0));
         }
 #pragma unroll
-        for (int q = 0; q < WIN / 2; ++q)
-            Ws[2 * q + (lane >> 5)][c] = ((okm >> q) & 1) ? v[q] : mk<C>(R(0), R(0));
+        for (int q = 0; q < RPT; ++q) {
+            const int rw = wrow[r0 + 16 * q];
+            Ws[r0 + 16 * q][c] = csel(rw >= 0 && c < nb, v[q], mk<C>(R(0), R(0)));
+        }
     }
     __syncthreads();
-    C W[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        W[c] = mk<C>(R(0), R(0));
-        if (act && c < nb) W[c] = Ws[lane][c];
-    }
-    static_for<0, NB>([&](auto ic) {
-        constexpr int cc = decltype(ic)::value;
-        if (cc < nb) {
-            // argmax of |.|_1 over the active window rows; lanes hold the
-            // window in ascending row order, so the lowest lane among exact
-            // ties is the smallest row index (GEPP's tie rule)
-            const R mag = cabs1(W[cc]);
-            const unsigned long long key = act ? mag_key(mag) : 0ULL;
-            const unsigned long long kmax = wave_max_u64(key);
-            unsigned long long hit = __ballot(act && key == kmax);
-            if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
-            const int pl = (int)__ffsll((long long)hit) - 1;  // wave-uniform
-            const R bm = act ? mag : R(-1);
-            if (lane == pl) {
-                prs[cc] = row;
-                pmags[cc] = bm;
-                act = false;
-                mystep = cc;
-            }
-            // pivot-row entries are broadcast one at a time (uniform control
-            // flow); the pivot lane's registers are not modified in this step
-            const C ucc = mk<C>(bcast(W[cc].x, pl), bcast(W[cc].y, pl));
-            const R pivmag = cabs1(ucc);
-            const C m = csel(act && pivmag > R(0), cmul(W[cc], crcp(ucc)), mk<C>(R(0), R(0)));
-            if (act) W[cc] = m;
-#pragma unroll
-            for (int c = cc + 1; c < NB; ++c) {
-                const C uc = mk<C>(bcast(W[c].x, pl), bcast(W[c].y, pl));
-                if (act) W[c] = csub(W[c], cmul(m, uc));
-            }
-        }
-    });
-    // A pivot lane's registers are frozen from its step on: they hold its row
-    // of L (multipliers, c < step) and of U (c >= step).
-    if (mystep >= 0) {
+    if (w0) {
+        C W[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
-            if (c < nb) {
-                const bool lo = c < mystep;
-                Ls[mystep][c] = csel(lo, W[c], mk<C>(c == mystep ? R(1) : R(0), R(0)));
-                Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[c]);
+            W[c] = mk<C>(R(0), R(0));
+            if (act && c < nb) W[c] = Ws[lane][c];
+        }
+        static_for<0, NB>([&](auto ic) {
+            constexpr int cc = decltype(ic)::value;
+            if (cc < nb) {
+                // argmax of |.|_1 over the active window rows; lanes hold the
+                // window in ascending row order, so the lowest lane among exact
+                // ties is the smallest row index (GEPP's tie rule)
+                const R mag = cabs1(W[cc]);
+                const unsigned long long key = act ? mag_key(mag) : 0ULL;
+                const unsigned long long kmax = wave_max_u64(key);
+                unsigned long long hit = __ballot(act && key == kmax);
+                if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
+                const int pl = (int)__ffsll((long long)hit) - 1;  // wave-uniform
+                const R bm = act ? mag : R(-1);
+                if (lane == pl) {
+                    prs[cc] = row;
+                    pmags[cc] = bm;
+                    act = false;
+                    mystep = cc;
+                }
+                // pivot-row entries are broadcast one at a time (uniform control
+                // flow); the pivot lane's registers are not modified in this step
+                const C ucc = mk<C>(bcast(W[cc].x, pl), bcast(W[cc].y, pl));
+                const R pivmag = cabs1(ucc);
+                const C m = csel(act && pivmag > R(0), cmul(W[cc], crcp(ucc)), mk<C>(R(0), R(0)));
+                if (act) W[cc] = m;
+#pragma unroll
+                for (int c = cc + 1; c < NB; ++c) {
+                    const C uc = mk<C>(bcast(W[c].x, pl), bcast(W[c].y, pl));
+                    if (act) W[c] = csub(W[c], cmul(m, uc));
+                }
+            }
+        });
+        // A pivot lane's registers are frozen from its step on: they hold its row
+        // of L (multipliers, c < step) and of U (c >= step).
+        if (mystep >= 0) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < nb) {
+                    const bool lo = c < mystep;
+                    Ls[mystep][c] = csel(lo, W[c], mk<C>(c == mystep ? R(1) : R(0), R(0)));
+                    Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[c]);
+                }
             }
         }
     }
     __syncthreads();
-    store_pivot_factors<C>(Ls, Us, nb, lane, 64, (C*)J.LU);
+    // the window tile is dead: its LDS holds the inverse scratch
+    C(*Lis)[PMAX + 1] = (C(*)[PMAX + 1]) & Ws[0][0];
+    store_pivot_inverses<C>(Ls, Us, Lis, Lis + PMAX, nb, t, WP_THREADS, (C*)J.LU);
+    if (!w0) return;
     if (lane < nb) {
         const int p = prs[lane];
         J.piv[poff + lane] = p;
@@ -230,7 +275,7 @@ template <typename R>
 void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int second) {
     (void)n;
-    k_wpanel<R, PMAX><<<njobs, 64, 0, s>>>(jobs_d, k0, nb, fresh, second);
+    k_wpanel<R, PMAX><<<njobs, WP_THREADS, 0, s>>>(jobs_d, k0, nb, fresh, second);
 }
 template void launch_wpanel<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
 template void launch_wpanel<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);
@@ -253,6 +298,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
     __shared__ R pmags[NB];
     __shared__ C Ls[PMAX][PMAX + 1];
     __shared__ C Us[PMAX][PMAX + 1];
+    __shared__ C Lis[2 * PMAX][PMAX + 1];  // inverse scratch (L^-1, U^-1)
     const C* A = (const C*)J.Y;
     const int r = t;
     const bool valid = r < n;
@@ -311,7 +357,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    store_pivot_factors<C>(Ls, Us, NB, t, 1024, (C*)J.LU);
+    store_pivot_inverses<C>(Ls, Us, Lis, Lis + PMAX, NB, t, 1024, (C*)J.LU);
     if (t < NB) {
         const int p = prs[t];
         J.piv[t] = p;
@@ -347,177 +393,221 @@ template void launch_panel<double>(hipStream_t, const LineJob*, int, int, int, i
 template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// Z build, panel copy and pivot check.
-//   blocks [0, nzb):  one thread per column:
-//                     Z[:, col] = U^-1 L^-1 T[:, col], T[k][col] = A[piv[k]][col]
-//                     (delta(k, col-k0) for col in K), V[piv, :] = L U;
-//   blocks [nzb, ..): 32 rows per workgroup, 32 lanes per row (coalesced):
-//                     copy V = A[r, K] to the panel buffer and, if verify,
-//                     check rows r >= fresh: the value GEPP would compare at
-//                     step c, a_c = V[r][c] - sum_{c'<c} y_c' U[c'][c] with
-//                     y_c = a_c / U[c][c], must satisfy PIV_U |a_c| <= |U[c][c]|
+// Z build, panel copy and pivot check, on MFMA (16 x 16 x 4 products, complex
+// values as four real products).  M = V[piv, :]^-1 and U^-1 come from the
+// panel kernel.
+//   blocks [0, nzb):  64 columns per workgroup, 16 per wave:
+//                     T[k][col] = A[piv[k]][col] (delta(k, col - k0) for col
+//                     in K), Z[:, col] = M T[:, col];
+//   blocks [nzb, ..): 32 rows per workgroup: copy V = A[r, K] to the panel
+//                     buffer and, if verify, check rows r >= fresh: the value
+//                     GEPP would compare at step c is a_c = y_c U[c][c] with
+//                     y = V[r] U^-1, and PIV_U |a_c| <= |U[c][c]| must hold
 //                     (threshold pivoting).
 // Second panel of a fused pair (columns K2 after K1 = [k1, k1 + nb1)): its
 // pivot rows are taken after the first panel's update,
 //   T[k][col] = [col not in K1] A[p2_k][col] - sum_q V1[p2_k][q] Z1[q][col],
 // Z goes to rows PMAX.. of the Z buffer, and the panel values (already in the
 // panel buffer, columns PMAX..) are only checked.
+// Operand layout of v_mfma_*_16x16x4: A[i = lane%16][k = lane/16],
+// B[k = lane/16][j = lane%16], result D[M::row(lane, v)][lane%16].  In Z = M T
+// the contraction index of step (rb, s) for lane group g is the row T holds
+// in its accumulator register s, rb 16 + M::row(16 g, s), so T feeds the
+// product straight from the registers it was accumulated in.
 // ---------------------------------------------------------------------------
+constexpr int ZB_COLS = 64;  // columns per Z workgroup
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
                                                 int verify, int nzb, int second, int k1,
                                                 int nb1) {
     using C = cplx<R>;
+    using M = mfma_t<R>;
+    using acc_t = typename M::acc;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ C Ls[PMAX][PMAX + 1];
-    __shared__ C Us[PMAX][PMAX + 1];
-    __shared__ C V1p[PMAX][PMAX + 1];  // second panel: V1 rows of its pivots
-    __shared__ C Ud[PMAX];
+    __shared__ R Fr[PMAX][PMAX + 1], Fi[PMAX][PMAX + 1];  // M (Z blocks) or U^-1 (check blocks)
+    __shared__ R Vr[PMAX][PMAX + 1], Vi[PMAX][PMAX + 1];  // V1[piv2, :] (Z) or the V tile (check)
     __shared__ int pv[PMAX];
-    const C* LUg = (const C*)J.LU;
-    const int poff = second ? PMAX : 0;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
     const bool zpart = (int)blockIdx.x < nzb;
-    constexpr int EPT = PMAX * PMAX / 256;  // LU entries per thread (blockDim.x == 256)
+    const int poff = second ? PMAX : 0;
+    constexpr int EPT = PMAX * PMAX / 256;  // factor entries per thread (blockDim.x == 256)
     {
-        // every load of the staging is issued before the first LDS store;
-        // rows k >= nb of L and U are loaded but never read
-        const int t = threadIdx.x;
+        // M or U^-1 and the pivot rows: every load issued before the first use
+        const C* Fg = (const C*)J.LU + (zpart ? 0 : PMAX * PMAX);
         const int p = gload_s(J.piv + poff + (t < nb ? t : 0));
-        C lv[EPT], uv[EPT];
+        C fv[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) fv[q] = gload(Fg + t + 256 * q);
+        if (t < PMAX) pv[t] = (t < nb) ? p : 0;  // rows past nb: a safe row, masked
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = t + 256 * q;
-            if (zpart) lv[q] = gload(LUg + e);
-            uv[q] = gload(LUg + PMAX * PMAX + e);
-        }
-        if (t < nb) pv[t] = p;
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = t + 256 * q;
-            if (zpart) Ls[e / PMAX][e % PMAX] = lv[q];
-            Us[e / PMAX][e % PMAX] = uv[q];
-        }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nb) Ud[threadIdx.x] = crcp(Us[threadIdx.x][threadIdx.x]);
-    const C* Vb = (const C*)J.Vbuf;
-    if (second && zpart) {
-        C v[EPT];
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, k = e / PMAX, c = e % PMAX;
-            v[q] = gload(Vb + ((k < nb && c < nb1) ? (long)pv[k] * VMAX + c : 0));
-        }
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, k = e / PMAX, c = e % PMAX;
-            V1p[k][c] = (k < nb && c < nb1) ? v[q] : mk<C>(R(0), R(0));
+            Fr[e / PMAX][e % PMAX] = fv[q].x;
+            Fi[e / PMAX][e % PMAX] = fv[q].y;
         }
     }
     __syncthreads();
     const C* A = (const C*)J.Y;
     C* Zg = (C*)J.Z;
+    const C* Vb = (const C*)J.Vbuf;
     if (zpart) {
-        const int col = blockIdx.x * blockDim.x + threadIdx.x;
-        if (col >= n) return;
+        if (second) {
+            C v[EPT];
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q, k = e / PMAX, c = e % PMAX;
+                v[q] = gload(Vb + ((k < nb && c < nb1) ? (long)pv[k] * VMAX + c : 0));
+            }
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q, k = e / PMAX, c = e % PMAX;
+                const bool ok = k < nb && c < nb1;
+                Vr[k][c] = ok ? v[q].x : R(0);
+                Vi[k][c] = ok ? v[q].y : R(0);
+            }
+            __syncthreads();
+        }
+        const int col = blockIdx.x * ZB_COLS + wv * 16 + lr;
+        const bool colok = col < n;
         const bool inK = col >= k0 && col < k0 + nb;
         const bool inK1 = second && col >= k1 && col < k1 + nb1;
-        const bool ldA = !inK && !inK1;
-        C t[PMAX];
-        // the pivot-row loads are issued together (rows k >= nb and columns
-        // in K or K1 read A[0] and are replaced below)
+        const bool ldA = colok && !inK && !inK1;
+        // T in accumulator layout: register v of row block rb holds
+        // T[rb 16 + M::row(lane, v)][col]
+        acc_t tre[2], tim[2];
+        {
+            C tv[2][4];
 #pragma unroll
-        for (int k = 0; k < PMAX; ++k) {
-            const bool ok = ldA && k < nb;
-            t[k] = gload(A + (ok ? (long)pv[k] * n + col : 0));
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = rb * 16 + M::row(lane, v);
+                    tv[rb][v] = gload(A + ((ldA && k < nb) ? (long)pv[k] * n + col : 0));
+                }
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = rb * 16 + M::row(lane, v);
+                    const bool id = inK && k < nb && col - k0 == k;
+                    tre[rb][v] = (ldA && k < nb) ? tv[rb][v].x : (id ? R(1) : R(0));
+                    tim[rb][v] = (ldA && k < nb) ? tv[rb][v].y : R(0);
+                }
         }
+        if (second) {
+            // T -= V1[piv2, :] Z1 (columns outside K2): A = V1[k][q], B = Z1[q][col],
+            // q = 4 s + lane group
+            const bool zok = colok && !inK;
+            C zq[PMAX / 4];
 #pragma unroll
-        for (int k = 0; k < PMAX; ++k) {
-            if (!(ldA && k < nb)) t[k] = mk<C>((inK && k < nb && col - k0 == k) ? R(1) : R(0), R(0));
-        }
-        if (second && !inK) {
-            // Z1 rows eight at a time, loads first
-            for (int q0 = 0; q0 < nb1; q0 += 8) {
-                C z1[8];
+            for (int st = 0; st < PMAX / 4; ++st) {
+                const int q = 4 * st + lg;
+                zq[st] = gload(Zg + ((zok && q < nb1) ? (long)q * n + col : 0));
+            }
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    z1[u] = gload(Zg + (q0 + u < nb1 ? (long)(q0 + u) * n + col : 0));
+            for (int st = 0; st < PMAX / 4; ++st) {
+                const int q = 4 * st + lg;
+                const bool ok = zok && q < nb1;
+                const R br = ok ? zq[st].x : R(0), bi = ok ? zq[st].y : R(0);
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (q0 + u < nb1) {
-#pragma unroll
-                        for (int k = 0; k < PMAX; ++k)
-                            if (k < nb) t[k] = csub(t[k], cmul(V1p[k][q0 + u], z1[u]));
-                    }
+                for (int rb = 0; rb < 2; ++rb) {
+                    const R ar = Vr[rb * 16 + lr][q], ai = Vi[rb * 16 + lr][q];
+                    tre[rb] = M::run(-ar, br, tre[rb]);
+                    tre[rb] = M::run(ai, bi, tre[rb]);
+                    tim[rb] = M::run(-ar, bi, tim[rb]);
+                    tim[rb] = M::run(-ai, br, tim[rb]);
                 }
             }
         }
-        // t = L^-1 t (unit lower), then z = U^-1 t
-        static_for<1, PMAX>([&](auto ik) {
-            constexpr int k = decltype(ik)::value;
-            if (k < nb) {
-                C sacc = t[k];
-                static_for<0, k>([&](auto iq) {
-                    constexpr int q = decltype(iq)::value;
-                    sacc = csub(sacc, cmul(Ls[k][q], t[q]));
-                });
-                t[k] = sacc;
+        // Z = M T
+        acc_t zre[2], zim[2];
+#pragma unroll
+        for (int ro = 0; ro < 2; ++ro) {
+            zre[ro] = acc_t{0, 0, 0, 0};
+            zim[ro] = acc_t{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int k = rb * 16 + M::row(lane, st);
+                const R br = tre[rb][st], bi = tim[rb][st];
+#pragma unroll
+                for (int ro = 0; ro < 2; ++ro) {
+                    const R ar = Fr[ro * 16 + lr][k], ai = Fi[ro * 16 + lr][k];
+                    zre[ro] = M::run(ar, br, zre[ro]);
+                    zre[ro] = M::run(-ai, bi, zre[ro]);
+                    zim[ro] = M::run(ar, bi, zim[ro]);
+                    zim[ro] = M::run(ai, br, zim[ro]);
+                }
             }
-        });
-        static_for<0, PMAX>([&](auto ik) {
-            constexpr int k = PMAX - 1 - decltype(ik)::value;
-            if (k < nb) {
-                C sacc = t[k];
-                static_for<k + 1, PMAX>([&](auto iq) {
-                    constexpr int q = decltype(iq)::value;
-                    if (q < nb) sacc = csub(sacc, cmul(Us[k][q], t[q]));
-                });
-                t[k] = cmul(sacc, Ud[k]);
+        if (!colok) return;
+#pragma unroll
+        for (int ro = 0; ro < 2; ++ro)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = ro * 16 + M::row(lane, v);
+                if (row < nb) gstore(Zg + (long)(poff + row) * n + col, mk<C>(zre[ro][v], zim[ro][v]));
             }
-        });
-        static_for<0, PMAX>([&](auto ik) {
-            constexpr int k = decltype(ik)::value;
-            if (k < nb) gstore(Zg + (long)(poff + k) * n + col, t[k]);
-        });
     } else {
-        const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
+        const int c = t & 31, h8 = t >> 5;
         const int rbase = ((int)blockIdx.x - nzb) * 32;
-        // the four rows of this half-wave (rows half + 8 i), loads first;
-        // out-of-range lanes read the first panel entry and are zeroed
-        const C* vsrc = second ? (const C*)J.Vbuf + poff : A + k0;
+        // the V tile (rows rbase.., panel columns), 32 lanes per row, loads
+        // first; out-of-range lanes read the first panel entry and are zeroed
+        const C* vsrc = second ? Vb + poff : A + k0;
         C vrow[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int r = rbase + half + 8 * i;
+            const int r = rbase + h8 + 8 * i;
             const bool ok = r < n && c < nb;
             vrow[i] = gload(vsrc + (ok ? (second ? (long)r * VMAX : (long)r * n) + c : 0));
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int r = rbase + half + 8 * i;  // the same for the 32 lanes of a half-wave
-            if (r >= n) break;
-            C* Vr = (C*)J.Vbuf + (long)r * VMAX + poff;
-            C v = (c < nb) ? vrow[i] : mk<C>(R(0), R(0));
-            if (!second && c < nb) gstore(Vr + c, v);
-            if (verify && r >= fresh) {
-                bool bad = false;
-                for (int k = 0; k < nb; ++k) {
-                    const C ak = shfl_half(v, k);
-                    if (R(PIV_U) * cabs1(ak) > cabs1(Us[k][k])) bad = true;
-                    const C yk = cmul(ak, Ud[k]);
-                    if (c > k && c < nb) v = csub(v, cmul(yk, Us[k][c]));
-                }
-                if (bad && c == 0) *(GLOBAL_AS int*)J.verr = 1;
+            const int r = rbase + h8 + 8 * i;
+            const bool ok = r < n && c < nb;
+            const C v = csel(ok, vrow[i], mk<C>(R(0), R(0)));
+            if (!second && ok) gstore((C*)J.Vbuf + (long)r * VMAX + c, v);
+            Vr[h8 + 8 * i][c] = v.x;
+            Vi[h8 + 8 * i][c] = v.y;
+        }
+        if (!verify) return;  // uniform
+        __syncthreads();
+        // Y = V U^-1, wave (rb, cb) -> rows rb 16.., columns cb 16..
+        const int rb = wv >> 1, cb = wv & 1;
+        acc_t yre = acc_t{0, 0, 0, 0}, yim = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < PMAX / 4; ++st) {
+            const int q = 4 * st + lg;
+            const R ar = Vr[rb * 16 + lr][q], ai = Vi[rb * 16 + lr][q];
+            const R br = Fr[q][cb * 16 + lr], bi = Fi[q][cb * 16 + lr];
+            yre = M::run(ar, br, yre);
+            yre = M::run(-ai, bi, yre);
+            yim = M::run(ar, bi, yim);
+            yim = M::run(ai, br, yim);
+        }
+        const int cc = cb * 16 + lr;
+        bool bad = false;
+        if (cc < nb) {
+            const C ucc = crcp(mk<C>(Fr[cc][cc], Fi[cc][cc]));  // U[c][c] = 1 / U^-1[c][c]
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rbase + rb * 16 + M::row(lane, v);
+                const C ac = cmul(mk<C>(yre[v], yim[v]), ucc);
+                if (r < n && r >= fresh && R(PIV_U) * cabs1(ac) > cabs1(ucc)) bad = true;
             }
         }
+        if (bad) *(GLOBAL_AS int*)J.verr = 1;
     }
 }
 
 template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int verify, int second, int k1, int nb1) {
-    const int nzb = (n + 255) / 256;
+    const int nzb = (n + ZB_COLS - 1) / ZB_COLS;
     const int nvb = (n + 31) / 32;
     k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb,
                                                         second, k1, nb1);
@@ -554,7 +644,7 @@ __global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int n
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = threadIdx.x + 256 * q, r = e / PMAX, c = e % PMAX;
-            Z1k[r][c] = (r < nb1 && c < nb2) ? v[q] : mk<C>(R(0), R(0));
+            Z1k[r][c] = csel(r < nb1 && c < nb2, v[q], mk<C>(R(0), R(0)));
         }
     }
     __syncthreads();
