@@ -67,8 +67,11 @@ __device__ inline unsigned long long mag_key(float m) { return (unsigned long lo
 
 // lane k of this lane's half-wave (lanes 0-31 or 32-63), k wave-uniform
 template <typename C> __device__ inline C bcast_half(C v, int k) {
+    // both halves' values are read (scalar reads, no divergence), then selected
     const bool hi = (threadIdx.x & 32) != 0;
-    return mk<C>(hi ? bcast(v.x, 32 + k) : bcast(v.x, k), hi ? bcast(v.y, 32 + k) : bcast(v.y, k));
+    const auto x0 = bcast(v.x, k), x1 = bcast(v.x, 32 + k);
+    const auto y0 = bcast(v.y, k), y1 = bcast(v.y, 32 + k);
+    return mk<C>(hi ? x1 : x0, hi ? y1 : y0);
 }
 
 // Inverse factors of the pivot block V[piv, :] = L U (unit lower L, upper U,
@@ -79,8 +82,11 @@ template <typename C> __device__ inline C bcast_half(C v, int k) {
 // broadcast with readlane.  M is then
 // formed as the product U^-1 L^-1 through LDS (Lis, Uis: scratch).  Called by
 // all nthr threads of the block (a multiple of 64) with Ls, Us complete.
-template <typename C>
-__device__ inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
+// UNROLL unrolls the substitutions (lane indices and LDS columns become
+// constants, so the row reads are issued ahead of the chain; needs the
+// register room of at most two waves per SIMD).
+template <bool UNROLL, typename C>
+__device__ __attribute__((always_inline)) inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
                                             C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], int nb,
                                             int t, int nthr, C* LUg) {
     using R = decltype(C{}.x);
@@ -92,15 +98,28 @@ __device__ inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1]
         const int j = j0 + ((t >> 5) & 1);
         C x = csel(i == j, one, zero);  // L^-1 e_j
         C y = csel(i == j, one, zero);  // U^-1 e_j
-        // forward over k, backward over kb = nb - 1 - k; rows and columns >= nb
-        // of Ls, Us are never used
-        for (int k = 0; k < nb; ++k) {
-            const int kb = nb - 1 - k;
+        // forward over k, backward over kb; rows and columns >= nb of Ls, Us
+        // are never used
+        auto fwd = [&](int k) {
             const C xk = bcast_half(x, k);
-            if (i > k && rok) x = csub(x, cmul(Ls[i][k], xk));
-            if (i == kb) y = cmul(y, dinv);
+            x = csel(i > k && rok, csub(x, cmul(Ls[i][k], xk)), x);
+        };
+        auto bwd = [&](int kb) {
+            y = csel(i == kb, cmul(y, dinv), y);
             const C yk = bcast_half(y, kb);
-            if (i < kb) y = csub(y, cmul(Us[i][kb], yk));
+            y = csel(i < kb, csub(y, cmul(Us[i][kb], yk)), y);
+        };
+        if constexpr (UNROLL) {
+            static_for<0, PMAX>([&](auto ik) {
+                constexpr int k = decltype(ik)::value;
+                if (k < nb) fwd(k);
+                if (PMAX - 1 - k < nb) bwd(PMAX - 1 - k);
+            });
+        } else {
+            for (int k = 0; k < nb; ++k) {
+                fwd(k);
+                bwd(nb - 1 - k);
+            }
         }
         const bool ok = rok && j < nb;
         Lis[i][j] = csel(ok, x, zero);
@@ -112,6 +131,7 @@ __device__ inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1]
     for (int e = t; e < PMAX * PMAX; e += nthr) {
         const int r = e / PMAX, c = e % PMAX;
         C sacc = zero;
+#pragma unroll
         for (int k = 0; k < PMAX; ++k) sacc = cadd(sacc, cmul(Uis[r][k], Lis[k][c]));
         gstore(Mg + e, sacc);
         gstore(Ug + e, Uis[r][c]);
@@ -242,7 +262,7 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     __syncthreads();
     // the window tile is dead: its LDS holds the inverse scratch
     C(*Lis)[PMAX + 1] = (C(*)[PMAX + 1]) & Ws[0][0];
-    store_pivot_inverses<C>(Ls, Us, Lis, Lis + PMAX, nb, t, WP_THREADS, (C*)J.LU);
+    store_pivot_inverses<true, C>(Ls, Us, Lis, Lis + PMAX, nb, t, WP_THREADS, (C*)J.LU);
     if (!w0) return;
     if (lane < nb) {
         const int p = prs[lane];
@@ -357,7 +377,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    store_pivot_inverses<C>(Ls, Us, Lis, Lis + PMAX, NB, t, 1024, (C*)J.LU);
+    store_pivot_inverses<false, C>(Ls, Us, Lis, Lis + PMAX, NB, t, 1024, (C*)J.LU);
     if (t < NB) {
         const int p = prs[t];
         J.piv[t] = p;
