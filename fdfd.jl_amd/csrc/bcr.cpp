@@ -38,6 +38,8 @@ size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
     const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
     const Plan P = make_plan(m);
     size_t bytes = (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
+    // Givens QR work of the first-level lines, kept for the reduced solve
+    bytes += P.tris.size() * (size_t)tri::work_doubles(n) * sizeof(double);
     // bf16x3 GEMM operands: four tiled bf16 planes per block
     if (b3 && precision != FDFD_FP64) bytes += (size_t)P.nblocks * 4 * b3_plane(n) * 2;
     return bytes;
@@ -112,12 +114,11 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
     const Plan P = make_plan(m);
-    // the QR work of a first-level line lives in its channel's panel buffer
-    if ((size_t)tri::work_doubles(n) * sizeof(double) > (size_t)n * VMAX * elem)
-        return fail(h, FDFD_ERR_INVALID, "internal: QR work does not fit the panel buffer");
     int rc;
     if ((rc = ensure_factor_work(h, n, std::max(1, P.max_gj * B)))) return rc;
     if ((rc = ensure_solve_work(h, N))) return rc;
+    if (trisolve_prepare(n) != (int)hipSuccess)
+        return fail(h, FDFD_ERR_HIP, "line-solve kernel: LDS limit could not be raised");
     for (Factor* F : Fs) {
         if ((rc = rt_assemble(h, F))) return rc;
         F->nblocks = P.nblocks;
@@ -134,6 +135,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 return fail(h, FDFD_ERR_OOM, "bf16 operand storage does not fit in device memory");
             F->blocks_b_bytes = bb;
         }
+        HIPCHK(h, hipMalloc(&F->triw, sizeof(double) * P.tris.size() * (size_t)tri::work_doubles(n)));
         HIPCHK(h, hipMalloc(&F->prow, sizeof(int) * (size_t)m * n));
         HIPCHK(h, hipMalloc(&F->iprow, sizeof(int) * (size_t)m * n));
         HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
@@ -184,7 +186,6 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             for (int i = 0; i < st.count; ++i) {
                 if (st.kind == ST_TRI) {
                     const STri& s = P.tris[st.first + i];
-                    const int c = f * st.count + i;  // work channel
                     TriJob J{};
                     J.out = blk(F, s.line);
                     own_planes(F, N, s.line, &J.sub, &J.diag, &J.sup);
@@ -192,7 +193,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.prow = F->prow + (long)s.line * n;
                     J.iprow = F->iprow + (long)s.line * n;
                     J.pivstat = F->pivstat + 3 * (long)s.line;
-                    J.work = (double*)((char*)h->Vbuf + (size_t)c * n * VMAX * elem);
+                    // kept per factor: the reduced solve applies T^-1 with it
+                    J.work = F->triw + (size_t)(st.first + i) * tri::work_doubles(n);
                     J.si = F->si;
                     tris.push_back(J);
                 } else if (st.kind == ST_FORM) {
@@ -442,13 +444,25 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             J.si = F->si;
             dgs.push_back(J);
         }
-        if ((rc = upload(&F->mv_tab, mvs)) || (rc = upload(&F->dg_tab, dgs))) return rc;
+        std::vector<TrsJob> trs;
+        for (const STrs& s : P.trss) {
+            TrsJob J{};
+            J.in = vec(s.xv, s.line);
+            J.out = vec(s.ov, s.line);
+            J.w = F->triw + (size_t)s.tri * tri::work_doubles(n);
+            J.si = F->si;
+            trs.push_back(J);
+        }
+        if ((rc = upload(&F->mv_tab, mvs)) || (rc = upload(&F->dg_tab, dgs)) ||
+            (rc = upload(&F->trs_tab, trs)))
+            return rc;
         F->ssteps.clear();
         for (const Step& s : P.ssteps) {
             int mats = 0;
             if (s.kind == ST_MV)
                 for (int i = 0; i < s.count; ++i) mats += P.mvs[s.first + i].nt;
-            F->ssteps.push_back({s.kind == ST_MV ? 0 : 1, s.first, s.count, mats});
+            const int kind = s.kind == ST_MV ? 0 : (s.kind == ST_DIAG ? 1 : 2);
+            F->ssteps.push_back({kind, s.first, s.count, mats});
         }
     }
     return FDFD_OK;
@@ -463,12 +477,18 @@ static void launch_solve(Handle* h, Factor* F, bool prof) {
     for (const auto& s : F->ssteps) {
         ProfScope* ps = nullptr;
         if (prof) {
-            const double bytes = s.kind == 0 ? (double)n * n * F->elem * s.mats : 64.0 * n * s.count;
+            // algorithmic bytes: the blocks streamed (matvec), the QR work and
+            // two vectors (line solve), or the diagonal planes and vectors
+            const double bytes = s.kind == 0   ? (double)n * n * F->elem * s.mats
+                                 : s.kind == 2 ? (8.0 * tri::work_doubles(n) + 32.0 * n) * s.count
+                                               : 64.0 * n * s.count;
             ps = new ProfScope(h, P_SWEEP, s.kind == 0 ? 8.0 * n * n * s.mats : 0, bytes);
         }
         if (s.kind == 0) {
             if (fp64) launch_mv<double>(h->stream, F->mv_tab + s.first, s.count, n);
             else launch_mv<float>(h->stream, F->mv_tab + s.first, s.count, n);
+        } else if (s.kind == 2) {
+            launch_trisolve(h->stream, F->trs_tab + s.first, s.count, n);
         } else {
             launch_diag(h->stream, F->dg_tab + s.first, s.count, n);
         }

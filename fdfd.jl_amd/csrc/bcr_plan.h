@@ -10,7 +10,9 @@
 //   ST_GJ    batched Gauss-Jordan inversion of the listed lines
 //   ST_GEMM  batched complex products out = [out] -/+ sum_t A_t B_t[prow_t]
 // Solve steps act on two field-shaped vectors, VX (in/out) and VT (temp):
-//   ST_MV    dense matrix-vector steps, ST_DIAG first-level diagonal steps.
+//   ST_MV    dense matrix-vector steps, ST_DIAG first-level diagonal steps,
+//   ST_TRS   first-level line solves x_out = T_j^-1 x_in by the line's stored
+//            Givens QR (T_j its periodic tridiagonal block, X_j = T_j^-1).
 // With X_j = D_j^-1 stored as X[a, b] = Y_j[prow_j[a], iprow_j[b]]:
 //   Z(u, j) = C(u, j) Y_j[prow_j, :]     (= C(u, j) X_j, columns by iprow_j)
 //   D_s    -= Z(s, j) C(j, s)[prow_j, :] (= C(s, j) X_j C(j, s))
@@ -61,13 +63,17 @@ struct SMv {
     int ov = 0, ol = -1;
     int omap = -1;  // line whose iprow scatters the output
 };
+struct STrs {
+    int tri = -1, line = -1;  // index into tris, line
+    int xv = 0, ov = 0;       // vectors read and written (same line)
+};
 struct SDg {
     int ov = 0, ol = -1, x0v = 0, x0l = -1;
     int nt = 0;
     int xv[2] = {0, 0}, xl[2] = {-1, -1};
     SDiag d[2];
 };
-enum StepKind { ST_FORM = 0, ST_GJ = 1, ST_GEMM = 2, ST_MV = 3, ST_DIAG = 4, ST_TRI = 5 };
+enum StepKind { ST_FORM = 0, ST_GJ = 1, ST_GEMM = 2, ST_MV = 3, ST_DIAG = 4, ST_TRI = 5, ST_TRS = 6 };
 struct Step {
     int kind, first, count;
 };
@@ -81,6 +87,7 @@ struct Plan {
     std::vector<Step> fsteps;
     std::vector<SMv> mvs;
     std::vector<SDg> dgs;
+    std::vector<STrs> trss;
     std::vector<Step> ssteps;
 };
 
@@ -119,7 +126,12 @@ inline Plan make_plan(int m) {
     };
     // solve steps: forward ones are appended directly, backward ones are
     // collected per level and appended in reverse level order at the end
-    struct SolveBlock { std::vector<SMv> mv; std::vector<SDg> dg; std::vector<Step> st; };
+    struct SolveBlock {
+        std::vector<SMv> mv;
+        std::vector<SDg> dg;
+        std::vector<STrs> ts;
+        std::vector<Step> st;
+    };
     auto add_mv = [](SolveBlock& b, const std::vector<SMv>& v) {
         if (v.empty()) return;
         b.st.push_back({ST_MV, (int)b.mv.size(), (int)v.size()});
@@ -130,11 +142,19 @@ inline Plan make_plan(int m) {
         b.st.push_back({ST_DIAG, (int)b.dg.size(), (int)v.size()});
         b.dg.insert(b.dg.end(), v.begin(), v.end());
     };
+    auto add_ts = [](SolveBlock& b, const std::vector<STrs>& v) {
+        if (v.empty()) return;
+        b.st.push_back({ST_TRS, (int)b.ts.size(), (int)v.size()});
+        b.ts.insert(b.ts.end(), v.begin(), v.end());
+    };
     auto emit = [&](const SolveBlock& b) {
         for (const Step& s : b.st) {
             if (s.kind == ST_MV) {
                 P.ssteps.push_back({ST_MV, (int)P.mvs.size(), s.count});
                 P.mvs.insert(P.mvs.end(), b.mv.begin() + s.first, b.mv.begin() + s.first + s.count);
+            } else if (s.kind == ST_TRS) {
+                P.ssteps.push_back({ST_TRS, (int)P.trss.size(), s.count});
+                P.trss.insert(P.trss.end(), b.ts.begin() + s.first, b.ts.begin() + s.first + s.count);
             } else {
                 P.ssteps.push_back({ST_DIAG, (int)P.dgs.size(), s.count});
                 P.dgs.insert(P.dgs.end(), b.dg.begin() + s.first, b.dg.begin() + s.first + s.count);
@@ -165,7 +185,9 @@ inline Plan make_plan(int m) {
         SDg c;
         c.ov = VT; c.ol = 0; c.x0v = VX; c.x0l = 0;
         add_dg(b, {c});
-        add_mv(b, {apply_x(0)});
+        STrs ts;
+        ts.tri = 0; ts.line = 0; ts.xv = VT; ts.ov = VX;
+        add_ts(b, {ts});
         emit(b);
         P.nblocks = next_blk;
         return P;
@@ -216,6 +238,7 @@ inline Plan make_plan(int m) {
                 t.line = j;
                 inv.push_back(t);
             }
+            const int tri0 = (int)P.tris.size();  // tris index of elines[0]
             step_tri(inv);
             std::vector<SForm> post;
             auto add_term = [&](SForm& f, int j, const SDiag& rs, const SDiag& cs) {
@@ -266,21 +289,17 @@ inline Plan make_plan(int m) {
                 }
             }
             step_form(post);
-            // solve: t_j = X_j x_j ; x_s -= C_sj .* t_j
-            std::vector<SMv> mv;
-            for (int j : elines) {
-                SMv v;
-                v.nt = 1;
-                v.M[0] = j;
-                v.imap[0] = j;
-                v.xv[0] = VX;
-                v.xl[0] = j;
+            // solve: t_j = X_j x_j (line solves) ; x_s -= C_sj .* t_j
+            std::vector<STrs> ts;
+            for (size_t q = 0; q < elines.size(); ++q) {
+                STrs v;
+                v.tri = tri0 + (int)q;
+                v.line = elines[q];
+                v.xv = VX;
                 v.ov = VT;
-                v.ol = j;
-                v.omap = j;
-                mv.push_back(v);
+                ts.push_back(v);
             }
-            add_mv(fwd, mv);
+            add_ts(fwd, ts);
             std::vector<SDg> dg;
             for (int p : surv) {
                 const int s = ring[p], right = at(p + 1), left = at(p - 1);
@@ -297,18 +316,24 @@ inline Plan make_plan(int m) {
             add_dg(fwd, dg);
             // backward: t_j = x_j - C_ja .* x_a - C_jb .* x_b ; x_j = X_j t_j
             std::vector<SDg> bd;
-            std::vector<SMv> bm;
-            for (int p : elim) {
+            std::vector<STrs> bt;
+            for (size_t q = 0; q < elim.size(); ++q) {
+                const int p = elim[q];
                 const int j = ring[p], a = at(p - 1), b = at(p + 1);
                 SDg d;
                 d.ov = VT; d.ol = j; d.x0v = VX; d.x0l = j;
                 d.xv[0] = VX; d.xl[0] = a; d.d[0] = C[{j, a}].d; d.nt = 1;
                 if (b != a) { d.xv[1] = VX; d.xl[1] = b; d.d[1] = C[{j, b}].d; d.nt = 2; }
                 bd.push_back(d);
-                bm.push_back(apply_x(j));
+                STrs v;
+                v.tri = tri0 + (int)q;
+                v.line = j;
+                v.xv = VT;
+                v.ov = VX;
+                bt.push_back(v);
             }
             add_dg(bk, bd);
-            add_mv(bk, bm);
+            add_ts(bk, bt);
         } else {
             step_gj(elines);
             // Z(u, j) = C(u, j) X_j for each eliminated j and neighbour u

@@ -145,6 +145,16 @@ struct TriJob {
     long si;
 };
 
+// First-level line solve of the reduced solve: out = T^-1 in along one line
+// (cells at stride si), T the line's periodic tridiagonal block, by its
+// Givens QR work w (tri_qr.h layout, FP64).
+struct TrsJob {
+    const double2* in;
+    double2* out;
+    const double* w;
+    long si;
+};
+
 // bf16 planes (re hi, re lo, im hi, im lo) of an n x n complex64 block, as
 // is (row r, k = column) or transposed with a row gather (row c, k:
 // src[map[k]][c]), in the tiled operand layout of the bf16x3 GEMM: tiles of
@@ -291,6 +301,11 @@ void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);
+// k_trisolve stages a line's QR work and vector in LDS (144 n bytes);
+// trisolve_prepare raises the kernel's dynamic LDS limit (call before any
+// launch or capture; returns a HIP error code)
+int trisolve_prepare(int n);
+void launch_trisolve(hipStream_t s, const TrsJob* jobs_d, int njobs, int n);
 void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2* sxf,
                    const double2* syf, int Nx, int Ny, int inverse);
 void launch_mode(hipStream_t s, const ModeJob* jobs_d, int njobs, int nev, double tol,

@@ -138,6 +138,73 @@ void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n) {
 template void launch_triinv<double>(hipStream_t, const TriJob*, int, int);
 template void launch_triinv<float>(hipStream_t, const TriJob*, int, int);
 
+// First-level line solve of the reduced solve, one line per workgroup: the
+// workgroup stages the line's Givens QR work (16 n doubles, contiguous) and
+// its vector entries in LDS with batched coalesced loads; lane 0 then runs
+// y = Q in and out = R^-1 y (tri_qr.h, FP64) in place in LDS, and the
+// workgroup writes the result back.  The two passes are sequential
+// recurrences over the cells: with the operands in LDS each step costs LDS
+// latency, not memory latency.
+constexpr int TS_THREADS = 256;
+
+__global__ __launch_bounds__(TS_THREADS) void k_trisolve(const TrsJob* jobs, int n) {
+    extern __shared__ double2 tsm[];  // work (8 n double2), then the vector (n)
+    const TrsJob J = jobs[blockIdx.x];
+    const int wd2 = tri::work_doubles(n) / 2;  // work in double2 units
+    double2* w2 = tsm;
+    double2* v = tsm + wd2;
+    const double2* wg = (const double2*)J.w;
+    const int t = threadIdx.x;
+    // batches of 16 loads in flight per thread before the LDS stores
+    for (int e0 = 0; e0 < wd2; e0 += 16 * TS_THREADS) {
+        double2 r[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = e0 + q * TS_THREADS + t;
+            r[q] = gload(wg + (e < wd2 ? e : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int e = e0 + q * TS_THREADS + t;
+            if (e < wd2) w2[e] = r[q];
+        }
+    }
+    for (int k0 = 0; k0 < n; k0 += 4 * TS_THREADS) {
+        double2 r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + q * TS_THREADS + t;
+            r[q] = gload(J.in + (k < n ? (long)k : 0) * J.si);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + q * TS_THREADS + t;
+            if (k < n) v[k] = r[q];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double* w = (const double*)w2;
+        auto get = [&](int k) { return tri::mkz(v[k].x, v[k].y); };
+        auto put = [&](int k, tri::cz z) { v[k] = make_double2(z.re, z.im); };
+        tri::apply_q_vec(n, w, get, put);   // in place: v[k] is read before y[k] is written
+        tri::apply_rinv(n, w, 0, get, put); // in place: y[k] is read before x[k] is written
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += TS_THREADS) gstore(J.out + (long)k * J.si, v[k]);
+}
+
+int trisolve_prepare(int n) {
+    const int bytes = (int)(sizeof(double) * (tri::work_doubles(n) + 2 * n));
+    return (int)hipFuncSetAttribute((const void*)k_trisolve,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+void launch_trisolve(hipStream_t s, const TrsJob* jobs_d, int njobs, int n) {
+    const size_t bytes = sizeof(double) * (tri::work_doubles(n) + 2 * n);
+    k_trisolve<<<njobs, TS_THREADS, bytes, s>>>(jobs_d, n);
+}
+
 // ---------------------------------------------------------------------------
 // Gauss-Jordan start state of a line: no row pivoted, window = rows 0..63.
 // ---------------------------------------------------------------------------

@@ -160,7 +160,7 @@ static void free_factor(Handle* h, Factor* F) {
     arena_give(h, F->blocks, F->blocks_bytes);
     arena_give(h, F->blocks_b, F->blocks_b_bytes);
     dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
-    dfree(F->mv_tab); dfree(F->dg_tab);
+    dfree(F->mv_tab); dfree(F->dg_tab); dfree(F->trs_tab); dfree(F->triw);
     if (F->solve_exec) (void)hipGraphExecDestroy(F->solve_exec);
     delete F;
 }

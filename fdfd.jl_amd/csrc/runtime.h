@@ -57,11 +57,14 @@ struct Factor {
     size_t blocks_bytes = 0, blocks_b_bytes = 0;
     int *prow = nullptr, *iprow = nullptr;  // m * n Gauss-Jordan pivot maps per line
     double* pivstat = nullptr;     // 3 per line
-    // kind 0: MvJob range, 1: DiagJob range; mats: n x n blocks the step streams
+    // kind 0: MvJob range, 1: DiagJob range, 2: TrsJob range; mats: n x n
+    // blocks the step streams
     struct SolveStep { int kind, first, count, mats; };
     std::vector<SolveStep> ssteps;
     MvJob* mv_tab = nullptr;
     DiagJob* dg_tab = nullptr;
+    TrsJob* trs_tab = nullptr;
+    double* triw = nullptr;        // Givens QR work of the first-level lines (tris order)
     hipGraphExec_t solve_exec = nullptr;
     double t_factor_ms = 0;
     double pivot_growth = 0;

@@ -173,6 +173,18 @@ static double run_case(int n, int m, unsigned seed) {
                     const int o = s.omap >= 0 ? iprow[s.omap][r] : r;
                     out[s.ol * n + o] = s.bl >= 0 ? base[s.bl * n + o] - z[r] : z[r];
                 }
+            } else if (st.kind == ST_TRS) {
+                // line solve by the first-level inverse (the device runs the
+                // stored Givens QR; here the dense inverse of the same block)
+                const STrs& s = P.trss[st.first + c];
+                if (P.tris[s.tri].line != s.line) return 1e300;
+                const auto& Mm = blk[s.line];
+                const auto& xv = V(s.xv);
+                std::vector<cd> z(n, cd(0));
+                for (int r = 0; r < n; ++r)
+                    for (int cc = 0; cc < n; ++cc) z[r] += Mm[(size_t)r * n + cc] * xv[s.line * n + cc];
+                auto& out = V(s.ov);
+                for (int r = 0; r < n; ++r) out[s.line * n + r] = z[r];
             } else {
                 const SDg& s = P.dgs[st.first + c];
                 std::vector<cd> res(n);
