@@ -301,9 +301,10 @@ void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n);
 template <typename R>
 void launch_mv(hipStream_t s, const MvJob* jobs_d, int njobs, int n);
 void launch_diag(hipStream_t s, const DiagJob* jobs_d, int njobs, int n);
-// k_trisolve stages a line's QR work and vector in LDS (144 n bytes);
-// trisolve_prepare raises the kernel's dynamic LDS limit (call before any
-// launch or capture; returns a HIP error code)
+// k_trisolve keeps the line's vector and a double-buffered chunk of its QR
+// work in LDS (16 n + 20480 bytes); trisolve_prepare sets the kernel's
+// dynamic LDS limit (call before any launch or capture; returns a HIP error
+// code)
 int trisolve_prepare(int n);
 void launch_trisolve(hipStream_t s, const TrsJob* jobs_d, int njobs, int n);
 void launch_wscale(hipStream_t s, double2* out, const double2* in, const double2* sxf,

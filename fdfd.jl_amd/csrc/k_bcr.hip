@@ -138,71 +138,152 @@ void launch_triinv(hipStream_t s, const TriJob* jobs_d, int njobs, int n) {
 template void launch_triinv<double>(hipStream_t, const TriJob*, int, int);
 template void launch_triinv<float>(hipStream_t, const TriJob*, int, int);
 
-// First-level line solve of the reduced solve, one line per workgroup: the
-// workgroup stages the line's Givens QR work (16 n doubles, contiguous) and
-// its vector entries in LDS with batched coalesced loads; lane 0 then runs
-// y = Q in and out = R^-1 y (tri_qr.h, FP64) in place in LDS, and the
-// workgroup writes the result back.  The two passes are sequential
-// recurrences over the cells: with the operands in LDS each step costs LDS
-// latency, not memory latency.
-constexpr int TS_THREADS = 256;
+// First-level line solve of the reduced solve, one line per workgroup of two
+// waves: out = T^-1 in with T's Givens QR (tri_qr.h, FP64): y = Q in, then
+// out = R^-1 y, both in place on the line's vector in LDS.  The two passes
+// are sequential recurrences over the cells, run by lane 0 of wave 0 with
+// the same arithmetic, in the same order, as tri::apply_q_vec and
+// tri::apply_rinv; wave 1 meanwhile stages the next TS_CH steps of the
+// factor (rotations, then rows of R, which are contiguous in the work array)
+// into the other half of a double buffer, so each step reads its operands
+// from LDS and the memory latency is hidden behind the recurrence.
+constexpr int TS_CH = 128;     // steps per staged chunk
+constexpr int TS_STEP = 10;    // doubles per step: 6 (rotations) or 10 (row of R)
 
-__global__ __launch_bounds__(TS_THREADS) void k_trisolve(const TrsJob* jobs, int n) {
-    extern __shared__ double2 tsm[];  // work (8 n double2), then the vector (n)
+__global__ __launch_bounds__(128) void k_trisolve(const TrsJob* jobs, int n) {
+    extern __shared__ double tsm[];  // vector (2 n doubles), then 2 x TS_CH x TS_STEP
+    double2* v = (double2*)tsm;
+    double* buf = tsm + 2 * n;
     const TrsJob J = jobs[blockIdx.x];
-    const int wd2 = tri::work_doubles(n) / 2;  // work in double2 units
-    double2* w2 = tsm;
-    double2* v = tsm + wd2;
-    const double2* wg = (const double2*)J.w;
     const int t = threadIdx.x;
-    // batches of 16 loads in flight per thread before the LDS stores
-    for (int e0 = 0; e0 < wd2; e0 += 16 * TS_THREADS) {
-        double2 r[16];
+    const bool comp = t == 0, loader = t >= 64;
+    const int ll = t - 64;
+    // stage doubles [src0, src0 + cnt) of the work array into buffer half b
+    // (cnt <= TS_CH * TS_STEP): all loads in flight before the LDS stores
+    auto stage = [&](int b, long src0, int cnt) {
+        constexpr int PER = TS_CH * TS_STEP / 64;
+        double r[PER];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int e = e0 + q * TS_THREADS + t;
-            r[q] = gload(wg + (e < wd2 ? e : 0));
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 64 + ll;
+            r[q] = gload_s(J.w + src0 + (e < cnt ? e : 0));
         }
+        double* d = buf + b * TS_CH * TS_STEP;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int e = e0 + q * TS_THREADS + t;
-            if (e < wd2) w2[e] = r[q];
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 64 + ll;
+            if (e < cnt) d[e] = r[q];
         }
-    }
-    for (int k0 = 0; k0 < n; k0 += 4 * TS_THREADS) {
+    };
+    for (int k0 = 0; k0 < n; k0 += 4 * 128) {
         double2 r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int k = k0 + q * TS_THREADS + t;
+            const int k = k0 + q * 128 + t;
             r[q] = gload(J.in + (k < n ? (long)k : 0) * J.si);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int k = k0 + q * TS_THREADS + t;
+            const int k = k0 + q * 128 + t;
             if (k < n) v[k] = r[q];
         }
     }
+    using tri::cz;
+    auto ld = [&](int k) { return tri::mkz(v[k].x, v[k].y); };
+    auto st = [&](int k, cz z) { v[k] = make_double2(z.re, z.im); };
+    // pass 1, y = Q in: rotation steps k = 0 .. n-2 (6 doubles each at w[6 k])
+    const int n1 = n - 1;
+    const int nc1 = (n1 + TS_CH - 1) / TS_CH;
+    if (loader && n1 > 0) stage(0, 0, tri::ROT * min(TS_CH, n1));
     __syncthreads();
-    if (t == 0) {
-        const double* w = (const double*)w2;
-        auto get = [&](int k) { return tri::mkz(v[k].x, v[k].y); };
-        auto put = [&](int k, tri::cz z) { v[k] = make_double2(z.re, z.im); };
-        tri::apply_q_vec(n, w, get, put);   // in place: v[k] is read before y[k] is written
-        tri::apply_rinv(n, w, 0, get, put); // in place: y[k] is read before x[k] is written
+    cz carry = tri::mkz(0, 0), tail = tri::mkz(0, 0);
+    if (comp && n > 1) {
+        carry = ld(0);
+        tail = ld(n - 1);
+    }
+    for (int c = 0; c < nc1; ++c) {
+        const int k0 = c * TS_CH, len = min(TS_CH, n1 - k0);
+        if (loader && c + 1 < nc1) {
+            const int k1 = k0 + TS_CH;
+            stage((c + 1) & 1, (long)tri::ROT * k1, tri::ROT * min(TS_CH, n1 - k1));
+        }
+        if (comp) {
+            const double* rb = buf + (c & 1) * TS_CH * TS_STEP;
+            for (int i = 0; i < len; ++i) {
+                const int k = k0 + i;
+                const double* r = rb + tri::ROT * i;
+                if (k + 2 < n) {  // rot1 on rows (k, k+1), rot2 on rows (k, n-1)
+                    cz nxt = ld(k + 1);
+                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
+                    tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
+                    st(k, carry);
+                    carry = nxt;
+                } else {          // k = n-2: rot1 on rows (n-2, n-1)
+                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
+                    st(n - 2, carry);
+                    st(n - 1, tail);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // pass 2, out = R^-1 y, rows k = n-1 .. 0 (10 doubles each at w[6 n + 10 k]:
+    // 1/R[k][k], R[k][k+1], R[k][k+2], R[k][n-2], R[k][n-1])
+    const long rbase = (long)tri::ROT * n;
+    const int nc2 = (n + TS_CH - 1) / TS_CH;
+    if (loader) {
+        const int lo = max(0, n - TS_CH);
+        stage(0, rbase + (long)tri::RROW * lo, tri::RROW * (n - lo));
     }
     __syncthreads();
-    for (int k = t; k < n; k += TS_THREADS) gstore(J.out + (long)k * J.si, v[k]);
+    cz x1 = tri::mkz(0, 0), x2 = tri::mkz(0, 0), xA = tri::mkz(0, 0), xB = tri::mkz(0, 0);
+    for (int c = 0; c < nc2; ++c) {
+        const int hi = n - c * TS_CH, lo = max(0, hi - TS_CH);
+        if (loader && c + 1 < nc2) {
+            const int lo2 = max(0, lo - TS_CH);
+            stage((c + 1) & 1, rbase + (long)tri::RROW * lo2, tri::RROW * (lo - lo2));
+        }
+        if (comp) {
+            const double* rb = buf + (c & 1) * TS_CH * TS_STEP;
+            for (int k = hi - 1; k >= lo; --k) {
+                const double* r = rb + tri::RROW * (k - lo);
+                const cz yk = ld(k);
+                const cz dinv = tri::mkz(r[0], r[1]);
+                if (k == n - 1) {
+                    xB = tri::zmul(yk, dinv);
+                    st(k, xB);
+                } else if (k == n - 2) {
+                    xA = tri::zmul(tri::zsub(yk, tri::zmul(tri::mkz(r[8], r[9]), xB)), dinv);
+                    st(k, xA);
+                    x1 = xA;
+                    x2 = xB;
+                } else {
+                    cz y = yk;
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
+                    const cz x = tri::zmul(y, dinv);
+                    st(k, x);
+                    x2 = x1;
+                    x1 = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = t; k < n; k += 128) gstore(J.out + (long)k * J.si, v[k]);
 }
 
+static size_t trisolve_lds(int n) { return sizeof(double) * (2 * (size_t)n + 2 * TS_CH * TS_STEP); }
+
 int trisolve_prepare(int n) {
-    const int bytes = (int)(sizeof(double) * (tri::work_doubles(n) + 2 * n));
     return (int)hipFuncSetAttribute((const void*)k_trisolve,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)trisolve_lds(n));
 }
 
 void launch_trisolve(hipStream_t s, const TrsJob* jobs_d, int njobs, int n) {
-    const size_t bytes = sizeof(double) * (tri::work_doubles(n) + 2 * n);
-    k_trisolve<<<njobs, TS_THREADS, bytes, s>>>(jobs_d, n);
+    k_trisolve<<<njobs, 128, trisolve_lds(n), s>>>(jobs_d, n);
 }
 
 // ---------------------------------------------------------------------------
