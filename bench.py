@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import numpy as np  # noqa: E402
 
-PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
+PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12, "fp16": 2.5e15}
 KERNELS = ["trailing", "panel", "form", "solve", "stencil", "zbuild", "gemm"]
 
 
@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the mode-source and modulated-solve legs (headline only)")
     ap.add_argument("--precisions", default="mixed,fp64,mixed-fp32gemm",
-                    help="mixed: complex64 factors with bf16x3 GEMM products (default mixed "
+                    help="mixed: complex64 factors with fp16x3 GEMM products (default mixed "
                          "path); mixed-fp32gemm: complex64 factors with FP32 MFMA products; "
                          "fp64: complex128 factors")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
@@ -303,8 +303,8 @@ def main():
     if head == "mixed":
         if dom == "gemm":
             # each real product of the default mixed GEMM is formed from
-            # three bf16 MFMA products, so its ceiling is the bf16 peak / 3
-            mpeak, pnote = PEAK["bf16"] / 3, "bf16 dense MFMA peak / 3 (bf16x3 products)"
+            # three fp16 MFMA products, so its ceiling is the fp16 peak / 3
+            mpeak, pnote = PEAK["fp16"] / 3, "fp16 dense MFMA peak / 3 (fp16x3 products)"
         else:
             mpeak, pnote = PEAK["fp32"], "FP32 dense MFMA peak"
     else:
@@ -338,7 +338,7 @@ def main():
             P = json.load(fh)
         rt = "float" if head == "mixed" else "double"
         # kernel-name prefix of the dominant slot (the mixed path's GEMM is
-        # the split-operand bf16x3 kernel)
+        # the split-operand fp16x3 kernel)
         # (FP32 lines with n % 4 == 0 use the 16-byte-access trailing kernel)
         trail = "k_trailing_f32" if head == "mixed" and n % 4 == 0 else "k_trailing<" + rt
         prefix = {"gemm": "k_gemm_b3" if head == "mixed" else "k_gemm<" + rt,
@@ -365,7 +365,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64 factors (GEMM products bf16x3, FP32 accumulation) + c128 refinement"
+        "dtype": ("c64 factors (GEMM products fp16x3 on row-scaled split operands, FP32 accumulation) + c128 refinement"
                   if head == "mixed" else "c128"),
         "data": "synthetic (seeded random-pixel coupler, generated in-process)",
         "config": {

@@ -40,8 +40,8 @@ size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
     size_t bytes = (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
     // Givens QR work of the first-level lines, kept for the reduced solve
     bytes += P.tris.size() * (size_t)tri::work_doubles(n) * sizeof(double);
-    // bf16x3 GEMM operands: four tiled bf16 planes per block
-    if (b3 && precision != FDFD_FP64) bytes += (size_t)P.nblocks * 4 * b3_plane(n) * 2;
+    // three-product GEMM operands: four tiled fp16 planes and exponents per block
+    if (b3 && precision != FDFD_FP64) bytes += (size_t)P.nblocks * b3_block_bytes(n);
     return bytes;
 }
 
@@ -89,9 +89,13 @@ static void own_planes(const Factor* F, long N, int line, const double2** sub,
 static void* blk(const Factor* F, int id) {
     return (char*)F->blocks + (size_t)id * F->n * F->n * F->elem;
 }
-// bf16 planes of block id (bf16x3 mode): 4 tiled planes of bf16
+// split operand of block id (three-product GEMMs): 4 tiled fp16 planes, then
+// the row exponents
 static void* blk_b(const Factor* F, int id) {
-    return (char*)F->blocks_b + (size_t)id * 4 * b3_plane(F->n) * 2;
+    return (char*)F->blocks_b + (size_t)id * b3_block_bytes(F->n);
+}
+static int* blk_e(const Factor* F, int id) {
+    return (int*)((char*)blk_b(F, id) + 4 * (size_t)b3_plane(F->n) * 2);
 }
 
 int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
@@ -109,7 +113,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     }
     const int n = Fs[0]->n, m = Fs[0]->m;
     const size_t elem = Fs[0]->elem;
-    // mixed path with bf16x3 GEMM products: operands are split once per step
+    // mixed path with three-product GEMMs: operands are split once per step
     const bool b3 = (prec != FDFD_FP64) && h->gemm_b3;
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
@@ -129,10 +133,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                                              " MiB) does not fit in device memory");
         F->blocks_bytes = bytes;
         if (b3) {
-            const size_t bb = (size_t)P.nblocks * 4 * b3_plane(n) * 2;
+            const size_t bb = (size_t)P.nblocks * b3_block_bytes(n);
             F->blocks_b = arena_take(h, bb);
             if (!F->blocks_b)
-                return fail(h, FDFD_ERR_OOM, "bf16 operand storage does not fit in device memory");
+                return fail(h, FDFD_ERR_OOM, "split GEMM operand storage does not fit in device memory");
             F->blocks_b_bytes = bb;
         }
         HIPCHK(h, hipMalloc(&F->triw, sizeof(double) * P.tris.size() * (size_t)tri::work_doubles(n)));
@@ -147,7 +151,11 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     std::vector<FormJob> forms;
     std::vector<LineJob> ljs;
     std::vector<GemmJob> gemms;
+    // split-operand GEMM steps run as two launches: every job's first term,
+    // then the second terms (accumulating), since terms carry their own scales
+    std::vector<int> nsecond;
     for (const Step& st : P.fsteps) {
+        std::vector<GemmJob> second;
         if (st.kind == ST_GEMM && b3) {
             // distinct operands of the step: A blocks as is, (B block, map) transposed
             int cnt = 0;
@@ -160,7 +168,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     for (int t = 0; t < g.nt; ++t) {
                         if (std::find(seenA.begin(), seenA.end(), g.A[t]) == seenA.end()) {
                             seenA.push_back(g.A[t]);
-                            splits.push_back({blk(F, g.A[t]), blk_b(F, g.A[t]), nullptr, 0});
+                            splits.push_back({blk(F, g.A[t]), blk_b(F, g.A[t]), nullptr,
+                                              blk_e(F, g.A[t]), 0});
                             ++cnt;
                         }
                         const std::pair<int, int> kb(g.B[t], g.bmap[t]);
@@ -168,7 +177,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                             seenB.push_back(kb);
                             splits.push_back({blk(F, g.B[t]), blk_b(F, g.B[t]),
                                               g.bmap[t] >= 0 ? F->prow + (long)g.bmap[t] * n : nullptr,
-                                              1});
+                                              blk_e(F, g.B[t]), 1});
                             ++cnt;
                         }
                     }
@@ -237,12 +246,23 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.nt = s.nt;
                     J.acc = s.acc;
                     J.sub = s.sub;
-                    for (int t = 0; t < s.nt; ++t) {
-                        if (b3) {  // split planes; the B gather is applied by the split
-                            J.A[t] = blk_b(F, s.A[t]);
-                            J.B[t] = blk_b(F, s.B[t]);
-                            J.bmap[t] = nullptr;
-                        } else {
+                    if (b3) {  // split planes; the B gather is applied by the split
+                        J.A[0] = blk_b(F, s.A[0]);
+                        J.B[0] = blk_b(F, s.B[0]);
+                        J.ea[0] = blk_e(F, s.A[0]);
+                        J.eb[0] = blk_e(F, s.B[0]);
+                        if (s.nt > 1) {
+                            GemmJob J2 = J;
+                            J2.nt = 0;  // counted with the first term
+                            J2.acc = 1;
+                            J2.A[0] = blk_b(F, s.A[1]);
+                            J2.B[0] = blk_b(F, s.B[1]);
+                            J2.ea[0] = blk_e(F, s.A[1]);
+                            J2.eb[0] = blk_e(F, s.B[1]);
+                            second.push_back(J2);
+                        }
+                    } else {
+                        for (int t = 0; t < s.nt; ++t) {
                             J.A[t] = blk(F, s.A[t]);
                             J.B[t] = blk(F, s.B[t]);
                             J.bmap[t] = s.bmap[t] >= 0 ? F->prow + (long)s.bmap[t] * n : nullptr;
@@ -251,6 +271,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     gemms.push_back(J);
                 }
             }
+        }
+        if (st.kind == ST_GEMM && b3) {
+            nsecond.push_back((int)second.size());
+            gemms.insert(gemms.end(), second.begin(), second.end());
         }
     }
     Tables T;
@@ -299,8 +323,13 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
                 ProfScope* ps = scope(P_GEMM, fl, 0);
                 if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
-                else if (b3) launch_gemm_b3(st, T.gemm + og, nj, n);
-                else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
+                else if (b3) {
+                    // first terms, then the second terms accumulating into them
+                    const int n2 = nsecond[ogs - 1];
+                    launch_gemm_b3(st, T.gemm + og, nj, n);
+                    if (n2 > 0) launch_gemm_b3(st, T.gemm + og + nj, n2, n);
+                    og += n2;
+                } else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
                 delete ps;
                 og += nj;
             } else {

@@ -155,19 +155,28 @@ struct TrsJob {
     long si;
 };
 
-// bf16 planes (re hi, re lo, im hi, im lo) of an n x n complex64 block, as
-// is (row r, k = column) or transposed with a row gather (row c, k:
-// src[map[k]][c]), in the tiled operand layout of the bf16x3 GEMM: tiles of
-// 16 rows x 32 k, tile (rb, kb) at (rb * nkb + kb) * 512, entry (r, k) of a
-// tile at (k / 8) * 128 + r * 8 + k % 8, so that lane l of an MFMA fragment
-// reads the 16 bytes at 16 l.  Rows and k past n are zero.
+// Split operand of the three-product GEMM: the four fp16 planes (re hi,
+// re lo, im hi, im lo) of an n x n complex64 block, as is (row r, k =
+// column) or transposed with a row gather (row c, k: src[map[k]][c]), each
+// row r scaled by 2^e_r (e_r in exps, chosen so the row's largest |re|, |im|
+// is at most 2^14), in the tiled operand layout: tiles of 16 rows x 32 k,
+// tile (rb, kb) at (rb * nkb + kb) * 512, entry (r, k) of a tile at
+// (k / 8) * 128 + r * 8 + k % 8, so that lane l of an MFMA fragment reads the
+// 16 bytes at 16 l.  Rows and k past n are zero.
 inline __host__ __device__ int b3_nrb(int n) { return (n + 15) / 16; }
 inline __host__ __device__ int b3_nkb(int n) { return (n + 31) / 32; }
 inline __host__ __device__ long b3_plane(int n) { return (long)b3_nrb(n) * b3_nkb(n) * 512; }
+// bytes of one split block: four planes of fp16, then the row exponents
+// (ints), padded to 256 bytes
+inline __host__ __device__ size_t b3_block_bytes(int n) {
+    const size_t b = 4 * (size_t)b3_plane(n) * 2 + (size_t)b3_nrb(n) * 16 * sizeof(int);
+    return (b + 255) / 256 * 256;
+}
 struct SplitJob {
     const void* src;
     void* dst;
     const int* map;
+    int* exps;  // b3_nrb(n) * 16 row exponents
     int trans;
 };
 
@@ -178,6 +187,8 @@ struct GemmJob {
     const void* A[2];
     const void* B[2];
     const int* bmap[2];
+    const int* ea[2];  // split operands: row exponents of A_t and of B_t^T
+    const int* eb[2];
     int nt;
     int acc;
     int sub;

@@ -516,118 +516,183 @@ template void launch_gemm<double>(hipStream_t, const GemmJob*, int, int, int, in
 template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
-// bf16x3 products for the mixed path (its default).  Each FP32 value x is split
-// once into bf16 hi = bf16(x), lo = bf16(x - hi); a real product is
-// a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in FP32 by
-// v_mfma_f32_16x16x32_bf16 (relative error ~2^-16 per product, removed by the
-// FP64 refinement).
-//
-// k_split writes the four bf16 planes (re hi, re lo, im hi, im lo) of an
-// n x n complex block in the tiled operand layout (b3_plane): as is (an A
-// operand, row = block row, k = block column) or transposed with the row
-// gather of the B operand applied (row = block column c, k: B[map[k]][c]).
-// A workgroup converts a 64 x 64 piece: 4 x 2 tiles per plane, each written
-// by one wave as 1 KB contiguous.
+// Three-product GEMMs for the mixed path (its default).  Every FP32 operand
+// row is scaled by a power of two 2^e (so its largest |re|, |im| is at most
+// 2^14) and split once into fp16 hi = f16(2^e x), lo = f16(2^e x - hi); a real
+// product is hi hi + hi lo + lo hi accumulated in FP32 by
+// v_mfma_f32_16x16x32_f16 (about 22 bits of each operand: relative error
+// ~2^-21 per product, close to an FP32 GEMM), and each accumulated term is
+// unscaled exactly by 2^-(e_row + e_col).  The row scaling keeps fp16's range
+// (2^-24 .. 65504) around each row's largest entry.
 // ---------------------------------------------------------------------------
-typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
-__device__ inline void split_bf16(float x, __bf16& h, __bf16& l) {
-    h = (__bf16)x;
-    l = (__bf16)(x - (float)h);
+constexpr int SP_EMAX = 60;  // row exponents are clamped to [-SP_EMAX, SP_EMAX]
+
+// 2^e as a float, |e| <= 126
+__device__ inline float exp2i(int e) { return __int_as_float((127 + e) << 23); }
+
+__device__ inline void split_f16(float x, _Float16& h, _Float16& l) {
+    h = (_Float16)x;
+    l = (_Float16)(x - (float)h);
 }
 
+// One workgroup per 16-row strip (tile row block rb) of an operand: as is,
+// strip rows are block rows 16 rb + r and k runs over the columns;
+// transposed, strip rows are block columns c = 16 rb + r and k runs over the
+// block rows through the gather, (r, k) = src[map[k]][c].  Pass 1 takes each
+// row's largest |re|, |im| and its exponent; pass 2 (re-reading the strip,
+// now in L2) writes the four scaled fp16 planes, one wave per 1 KB tile.
 __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
-    const SplitJob J = jobs[blockIdx.z];
-    __shared__ float2 tile[64][65];  // [row][k] of the piece
-    const int r0 = blockIdx.y * 64, q0 = blockIdx.x * 64;  // rows r0.., k q0..
+    const SplitJob J = jobs[blockIdx.y];
+    const int rb = blockIdx.x, r0 = 16 * rb;
+    const int nkb = b3_nkb(n);
     const float2* src = (const float2*)J.src;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    // all 16 loads of a thread in flight before the LDS stores; entries past n
-    // read src[0] and are stored as zero
-    float2 v[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    __shared__ float wmax[4][16];
+    __shared__ int rexp[16];
+    auto mag = [](float2 v) { return fmaxf(fabsf(v.x), fabsf(v.y)); };
     if (!J.trans) {
+        // wave wv reads rows wv + 4 q (q < 4), its lanes along k (coalesced)
+        float m[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < n; k0 += 256) {
+            float2 v[4][4];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = r0 + ty + 4 * i, k = q0 + tx;
-            v[i] = gload(src + ((r < n && k < n) ? (long)r * n + k : 0));
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + wv + 4 * q, k = k0 + 64 * u + lane;
+                    v[q][u] = gload(src + ((r < n && k < n) ? (long)r * n + k : 0));
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + wv + 4 * q, k = k0 + 64 * u + lane;
+                    if (r < n && k < n) m[q] = fmaxf(m[q], mag(v[q][u]));
+                }
         }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = r0 + ty + 4 * i, k = q0 + tx;
-            tile[ty + 4 * i][tx] = (r < n && k < n) ? v[i] : make_float2(0.f, 0.f);
+        for (int q = 0; q < 4; ++q) {
+            for (int o = 32; o > 0; o >>= 1) m[q] = fmaxf(m[q], __shfl_xor(m[q], o, 64));
+            if (lane == 0) wmax[0][wv + 4 * q] = m[q];
         }
     } else {
-        // piece rows are block columns c = r0 + tx; k runs over block rows
-        int mrow[16];
+        // thread t reads column r0 + t % 16 of block rows map[g + 16 i],
+        // g = t / 16: 16 lanes cover 128 contiguous bytes of one block row
+        const int r = t & 15, g = t >> 4, c = r0 + r;
+        float m = 0.f;
+        for (int k0 = 0; k0 < n; k0 += 16 * 8) {
+            int mr[8];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int k = q0 + ty + 4 * i;
-            mrow[i] = (J.map && k < n) ? gload_s(J.map + k) : k;
-        }
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + g + 16 * u;
+                mr[u] = (J.map && k < n) ? gload_s(J.map + k) : k;
+            }
+            float2 v[8];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int k = q0 + ty + 4 * i, c = r0 + tx;
-            v[i] = gload(src + ((k < n && c < n) ? (long)mrow[i] * n + c : 0));
-        }
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + g + 16 * u;
+                v[u] = gload(src + ((k < n && c < n) ? (long)mr[u] * n + c : 0));
+            }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int k = q0 + ty + 4 * i, c = r0 + tx;
-            tile[tx][ty + 4 * i] = (k < n && c < n) ? v[i] : make_float2(0.f, 0.f);
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + g + 16 * u;
+                if (k < n && c < n) m = fmaxf(m, mag(v[u]));
+            }
         }
+        // the 16 threads of column r: lanes r, r + 16, r + 32, r + 48 of each wave
+        m = fmaxf(m, __shfl_xor(m, 16, 64));
+        m = fmaxf(m, __shfl_xor(m, 32, 64));
+        if (lane < 16) wmax[wv][lane] = m;
     }
     __syncthreads();
+    if (t < 16) {
+        float mx = wmax[0][t];
+        if (J.trans) mx = fmaxf(fmaxf(mx, wmax[1][t]), fmaxf(wmax[2][t], wmax[3][t]));
+        int e = 0;
+        if (mx > 0.f) {
+            int ex;
+            const float f = frexpf(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
+            const int cl = (f == 0.5f) ? ex - 1 : ex;  // ceil(log2 mx)
+            e = min(SP_EMAX, max(-SP_EMAX, 14 - cl));
+        }
+        rexp[t] = e;
+        J.exps[r0 + t] = e;
+    }
+    __syncthreads();
+    // pass 2: wave wv writes tiles kb = wv, wv + 4, ..; lane l holds row
+    // l % 16, k = 32 kb + 8 (l / 16) + [0, 8)
     const long plane = b3_plane(n);
-    const int nrb = b3_nrb(n), nkb = b3_nkb(n);
-    __bf16* dst = (__bf16*)J.dst;
-    const int lane = threadIdx.x & 63;
+    _Float16* dst = (_Float16*)J.dst;
+    const int r = lane & 15, row = r0 + r, kq = 8 * (lane >> 4);
+    const float sc = exp2i(rexp[r]);
+    for (int kb = wv; kb < nkb; kb += 4) {
+        const int kbase = 32 * kb + kq;
+        float2 v[8];
+        if (!J.trans) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int tid = ty + 4 * q;                   // tile of the piece, 0..7
-        const int rbl = tid & 3, kbl = tid >> 2;      // 4 row blocks x 2 k blocks
-        const int rb = r0 / 16 + rbl, kb = q0 / 32 + kbl;
-        if (rb >= nrb || kb >= nkb) continue;         // uniform per wave
-        const int r = rbl * 16 + (lane & 15), kc = kbl * 32 + (lane >> 4) * 8;
-        bf8v rh, rl, ih, il;
+            for (int e = 0; e < 8; ++e) {
+                const int k = kbase + e;
+                v[e] = gload(src + ((row < n && k < n) ? (long)row * n + k : 0));
+            }
+        } else {
+            int mr[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kbase + e;
+                mr[e] = (J.map && k < n) ? gload_s(J.map + k) : k;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kbase + e;
+                v[e] = gload(src + ((row < n && k < n) ? (long)mr[e] * n + row : 0));
+            }
+        }
+        h8v rh, rl, ih, il;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float2 x = tile[r][kc + e];
-            __bf16 h, l;
-            split_bf16(x.x, h, l);
+            const int k = kbase + e;
+            const bool ok = row < n && k < n;
+            _Float16 h, l;
+            split_f16(ok ? v[e].x * sc : 0.f, h, l);
             rh[e] = h;
             rl[e] = l;
-            split_bf16(x.y, h, l);
+            split_f16(ok ? v[e].y * sc : 0.f, h, l);
             ih[e] = h;
             il[e] = l;
         }
         const long o = ((long)rb * nkb + kb) * 512 + 8 * lane;
-        *(GLOBAL_AS bf8v*)(dst + o) = rh;
-        *(GLOBAL_AS bf8v*)(dst + plane + o) = rl;
-        *(GLOBAL_AS bf8v*)(dst + 2 * plane + o) = ih;
-        *(GLOBAL_AS bf8v*)(dst + 3 * plane + o) = il;
+        *(GLOBAL_AS h8v*)(dst + o) = rh;
+        *(GLOBAL_AS h8v*)(dst + plane + o) = rl;
+        *(GLOBAL_AS h8v*)(dst + 2 * plane + o) = ih;
+        *(GLOBAL_AS h8v*)(dst + 3 * plane + o) = il;
     }
 }
 
 void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n) {
-    k_split<<<dim3((n + 63) / 64, (n + 63) / 64, njobs), 256, 0, s>>>(jobs_d, n);
+    k_split<<<dim3(b3_nrb(n), njobs), 256, 0, s>>>(jobs_d, n);
 }
 
 // GEMM on split operands (tiled planes, b3_plane): A as [row][k], B^T as
 // [col][k] with the gather already applied.  Complex products in the
 // four-multiplication form with Ai Bi kept in its own accumulator:
 //   re = Ar Br - Ai Bi,  im = Ar Bi + Ai Br,
-// each real product as hi hi + hi lo + lo hi on v_mfma_f32_16x16x32_bf16.
+// each real product as hi hi + hi lo + lo hi on v_mfma_f32_16x16x32_f16.
 // Tile 128 x 128, 2 x 4 waves of 64 x 32, K staged 32 at a time through
 // double-buffered LDS.  Every operand access of a stage (global load, LDS
 // store, fragment read) is one wave reading or writing a contiguous 1 KB
 // tile, 16 bytes per lane, which keeps the LDS accesses conflict-free.  The
 // flat grid is ordered so each XCD (workgroups are dealt to the 8 XCDs
 // round-robin) works through a contiguous range of jobs and keeps their
-// operand panels in its own L2.
+// operand panels in its own L2.  A job is one term out = [acc ? out : 0] -/+
+// A B: the terms of a sum carry their own row scales, so the host issues a
+// sum of two as two launches, the second accumulating into the first.
 constexpr int B3_TM = 128, B3_TN = 128;
 
 __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int ntx, int total) {
-    __shared__ __align__(16) bf8v sA[2][4][8][64];  // [buffer][plane][row block][lane]
-    __shared__ __align__(16) bf8v sB[2][4][8][64];
+    __shared__ __align__(16) h8v sA[2][4][8][64];  // [buffer][plane][row block][lane]
+    __shared__ __align__(16) h8v sB[2][4][8][64];
     const int L = blockIdx.x, per = total / 8;
     const int lin = (L < 8 * per) ? (L % 8) * per + L / 8 : L;
     const int job = lin / (ntx * ntx), rem = lin % (ntx * ntx);
@@ -638,22 +703,20 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
     const int lr = lane & 15, lg = lane >> 4;
     const int nkb = b3_nkb(n), nrb = b3_nrb(n);
     const long plane = b3_plane(n);
-    const int ns = J.nt * nkb;
     // loader: per stage and operand 4 planes x 8 row blocks of 1 KB, four
     // 16-byte pieces per thread (a wave loads one whole tile)
-    bf8v ra[4], rb[4];
-    auto load = [&](int s) {
-        const int ts = s / nkb, kb = s - ts * nkb;
-        const __bf16* A = (const __bf16*)(ts ? J.A[1] : J.A[0]);
-        const __bf16* B = (const __bf16*)(ts ? J.B[1] : J.B[0]);
+    h8v ra[4], rb[4];
+    auto load = [&](int kb) {
+        const _Float16* A = (const _Float16*)J.A[0];
+        const _Float16* B = (const _Float16*)J.B[0];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int id = t + 512 * e;
             const int pl = id >> 9, blk = (id >> 6) & 7, l = id & 63;
             const int ta = i0 / 16 + blk, tb = j0 / 16 + blk;
             // past the last row block the loads read tile 0 (zeroed at the store)
-            ra[e] = *(const GLOBAL_AS bf8v*)(A + (ta < nrb ? pl * plane + ((long)ta * nkb + kb) * 512 + 8 * l : 0));
-            rb[e] = *(const GLOBAL_AS bf8v*)(B + (tb < nrb ? pl * plane + ((long)tb * nkb + kb) * 512 + 8 * l : 0));
+            ra[e] = *(const GLOBAL_AS h8v*)(A + (ta < nrb ? pl * plane + ((long)ta * nkb + kb) * 512 + 8 * l : 0));
+            rb[e] = *(const GLOBAL_AS h8v*)(B + (tb < nrb ? pl * plane + ((long)tb * nkb + kb) * 512 + 8 * l : 0));
         }
     };
     auto store = [&](int buf) {
@@ -661,26 +724,61 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
         for (int e = 0; e < 4; ++e) {
             const int id = t + 512 * e;
             const int pl = id >> 9, blk = (id >> 6) & 7, l = id & 63;
-            sA[buf][pl][blk][l] = (i0 / 16 + blk < nrb) ? ra[e] : bf8v{};
-            sB[buf][pl][blk][l] = (j0 / 16 + blk < nrb) ? rb[e] : bf8v{};
+            sA[buf][pl][blk][l] = (i0 / 16 + blk < nrb) ? ra[e] : h8v{};
+            sB[buf][pl][blk][l] = (j0 / 16 + blk < nrb) ? rb[e] : h8v{};
         }
     };
     f4v cre[4][2], cim[4][2], cii[4][2];
+    auto zero = [&]() {
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+        for (int p = 0; p < 4; ++p)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            cre[p][q] = f4v{0, 0, 0, 0};
-            cim[p][q] = f4v{0, 0, 0, 0};
-            cii[p][q] = f4v{0, 0, 0, 0};
-        }
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int s = 0; s < ns; ++s) {
-        const int cur = s & 1;
-        if (s + 1 < ns) load(s + 1);
-        bf8v a[4][4], b[2][4];  // [block][plane]: re hi, re lo, im hi, im lo
+            for (int q = 0; q < 2; ++q) {
+                cre[p][q] = f4v{0, 0, 0, 0};
+                cim[p][q] = f4v{0, 0, 0, 0};
+                cii[p][q] = f4v{0, 0, 0, 0};
+            }
+    };
+    float2* out = (float2*)J.out;
+    // add term ts (unscaled) to the output tile, one output row at a time (a
+    // scheduling barrier after each keeps the old values in flight, beside
+    // the live accumulators, to two per lane)
+    auto epilogue = [&]() {
+        const int* ea = J.ea[0];
+        const int* eb = J.eb[0];
+        const bool rd = J.acc;
+        const int jb = j0 + wc * 32 + lr;
+        int ej[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) ej[q] = gload_s(eb + (jb + 16 * q < n ? jb + 16 * q : 0));
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wr * 64 + p * 16 + 4 * lg + v;
+                if (i < n) {
+                    const int ei = gload_s(ea + i);
+                    float2* orow = out + (long)i * n + jb;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        if (jb + 16 * q < n) {
+                            const float sc = exp2i(-(ei + ej[q]));
+                            float ox = 0.f, oy = 0.f;
+                            if (rd) {
+                                const float2 old = gload(orow + 16 * q);
+                                ox = old.x;
+                                oy = old.y;
+                            }
+                            const float re = (cre[p][q][v] - cii[p][q][v]) * sc, im = cim[p][q][v] * sc;
+                            gstore(orow + 16 * q, J.sub ? make_float2(ox - re, oy - im) : make_float2(ox + re, oy + im));
+                        }
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+    };
+    auto compute = [&](int cur) {
+        h8v a[4][4], b[2][4];  // [block][plane]: re hi, re lo, im hi, im lo
 #pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -693,41 +791,29 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
         // blocks innermost so consecutive MFMAs use different accumulators
 #define B3_MM(ACC, X0, X1, Y0, Y1)                                                             \
     _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, Y0, ACC[p][q], 0, 0, 0);         \
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y0, ACC[p][q], 0, 0, 0);          \
     _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X0, Y1, ACC[p][q], 0, 0, 0);         \
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y1, ACC[p][q], 0, 0, 0);          \
     _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X1, Y0, ACC[p][q], 0, 0, 0);
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X1, Y0, ACC[p][q], 0, 0, 0);
         B3_MM(cre, a[p][0], a[p][1], b[q][0], b[q][1])  // Ar Br
         B3_MM(cim, a[p][0], a[p][1], b[q][2], b[q][3])  // Ar Bi
         B3_MM(cii, a[p][2], a[p][3], b[q][2], b[q][3])  // Ai Bi
         B3_MM(cim, a[p][2], a[p][3], b[q][0], b[q][1])  // Ai Br
 #undef B3_MM
-        if (s + 1 < ns) store(cur ^ 1);
+    };
+    zero();
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int s = 0; s < nkb; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < nkb) load(s + 1);
+        compute(cur);
+        if (s + 1 < nkb) store(cur ^ 1);
         __syncthreads();
     }
-    float2* out = (float2*)J.out;
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = i0 + wr * 64 + p * 16 + 4 * lg + v;
-            if (i >= n) continue;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int j = j0 + wc * 32 + q * 16 + lr;
-                if (j >= n) continue;
-                float2* o = out + (long)i * n + j;
-                float ox = 0.f, oy = 0.f;
-                if (J.acc) {
-                    const float2 old = gload(o);
-                    ox = old.x;
-                    oy = old.y;
-                }
-                const float re = cre[p][q][v] - cii[p][q][v], im = cim[p][q][v];
-                gstore(o, J.sub ? make_float2(ox - re, oy - im) : make_float2(ox + re, oy + im));
-            }
-        }
+    epilogue();
 }
 
 void launch_gemm_b3(hipStream_t s, const GemmJob* jobs_d, int njobs, int n) {

@@ -53,7 +53,7 @@ struct Factor {
     size_t elem = 16;              // bytes per complex factor entry
     void* blocks = nullptr;        // nblocks * n * n complex (double2 or float2)
     int nblocks = 0;
-    void* blocks_b = nullptr;      // bf16x3 mode: bf16 planes of GEMM operands, 8 B per entry
+    void* blocks_b = nullptr;      // split GEMM operands (fp16 planes + row exponents), ~8 B per entry
     size_t blocks_bytes = 0, blocks_b_bytes = 0;
     int *prow = nullptr, *iprow = nullptr;  // m * n Gauss-Jordan pivot maps per line
     double* pivstat = nullptr;     // 3 per line
@@ -90,7 +90,7 @@ struct Handle {
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
     int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)
-    int gemm_b3 = 1;                   // mixed path: bf16x3 GEMM products (FDFD_MIXED_GEMM=fp32: FP32)
+    int gemm_b3 = 1;                   // mixed path: three-product fp16 GEMMs (FDFD_MIXED_GEMM=fp32: FP32)
     int fuse_panels = 1;               // two Gauss-Jordan panels per trailing pass (FDFD_FUSE_PANELS=0)
     // sweep / Krylov work
     long work_N = 0;

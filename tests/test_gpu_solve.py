@@ -87,11 +87,11 @@ def test_linsolve_matches_oracle(engine, Nx, Ny, npml, pol, precision):
         engine.release(fid)
 
 
-@pytest.mark.parametrize("gemm", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("gemm", ["fp32", "fp16x3"])
 @pytest.mark.parametrize("Nx,Ny,npml", [(96, 80, 8), (72, 100, 6)])
 def test_mixed_gemm_modes_match_oracle(monkeypatch, gemm, Nx, Ny, npml):
     """Both GEMM kernels of the mixed path (complex64 factors with FP32 MFMA
-    products, or bf16x3 products) give the oracle's solution through the FP64
+    products, or fp16x3 products) give the oracle's solution through the FP64
     refinement, on grids deep enough for several dense reduction levels."""
     monkeypatch.setenv("FDFD_MIXED_GEMM", gemm)  # read when the handle is created
     eng = Engine(0)
@@ -116,9 +116,9 @@ def test_mixed_gemm_modes_match_oracle(monkeypatch, gemm, Nx, Ny, npml):
 
 
 @pytest.mark.parametrize("Nx,Ny,npml", [(37, 90, 6), (90, 50, 6)])
-def test_bf16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
+def test_fp16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
     """On line lengths that are not multiples of the 16 x 32 operand tiles
-    (37, 50), the bf16x3 products stay close enough to the FP32 ones that the
+    (37, 50), the fp16x3 products stay close enough to the FP32 ones that the
     refinement needs about as many iterations: a wrong or stale operand tile
     would leave FGMRES converging, but only with a much weaker preconditioner."""
     d = _device(Nx, Ny, npml, seed=5 + Ny)
@@ -126,7 +126,7 @@ def test_bf16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
     A = O.assemble(d.grid, d.eps_r, om, O.TM)
     b = O.rhs(om, d.src)
     its = {}
-    for gemm in ("fp32", "bf16x3"):
+    for gemm in ("fp32", "fp16x3"):
         monkeypatch.setenv("FDFD_MIXED_GEMM", gemm)  # read when the handle is created
         eng = Engine(0)
         try:
@@ -140,7 +140,7 @@ def test_bf16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
         its[gemm] = st["iterations"]
         print(f"{Nx}x{Ny} gemm={gemm}: res={res:.2e} its={its[gemm]}", flush=True)
         assert res <= 1e-12
-    assert its["bf16x3"] <= its["fp32"] + 4
+    assert its["fp16x3"] <= its["fp32"] + 4
 
 
 @pytest.mark.parametrize("switch", ["FDFD_FUSE_PANELS=0", "FDFD_GEMM3M=0", "FDFD_GEMM_WAVES=4",
