@@ -385,6 +385,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                         ProfScope* ps = scope(P_TRAIL, 8.0 * n * (double)n * (nb1 + nb2) * nj,
                                               2.0 * n * (double)n * elem * nj);
                         if (fp64) launch_trailing<double>(st, J, nj, n, k0, nb1, nb2);
+                        else if (h->trail_h3 && n % 4 == 0) launch_trailing_h3(st, J, nj, n, k0, nb1, nb2);
                         else launch_trailing<float>(st, J, nj, n, k0, nb1, nb2);
                         delete ps;
                     }

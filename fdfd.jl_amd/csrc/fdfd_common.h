@@ -299,6 +299,9 @@ void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1,
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
                      int nb2);
+// mixed path, n % 4 == 0: the trailing update on fp16x3 products
+void launch_trailing_h3(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
+                        int nb2);
 template <typename R>
 void launch_form(hipStream_t s, const FormJob* jobs_d, int njobs, int n);
 void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n);

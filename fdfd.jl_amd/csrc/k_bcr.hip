@@ -525,18 +525,6 @@ template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int
 // unscaled exactly by 2^-(e_row + e_col).  The row scaling keeps fp16's range
 // (2^-24 .. 65504) around each row's largest entry.
 // ---------------------------------------------------------------------------
-typedef _Float16 h8v __attribute__((ext_vector_type(8)));
-
-constexpr int SP_EMAX = 60;  // row exponents are clamped to [-SP_EMAX, SP_EMAX]
-
-// 2^e as a float, |e| <= 126
-__device__ inline float exp2i(int e) { return __int_as_float((127 + e) << 23); }
-
-__device__ inline void split_f16(float x, _Float16& h, _Float16& l) {
-    h = (_Float16)x;
-    l = (_Float16)(x - (float)h);
-}
-
 // One workgroup per 16-row strip (tile row block rb) of an operand: as is,
 // strip rows are block rows 16 rb + r and k runs over the columns;
 // transposed, strip rows are block columns c = 16 rb + r and k runs over the
@@ -610,13 +598,7 @@ __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     if (t < 16) {
         float mx = wmax[0][t];
         if (J.trans) mx = fmaxf(fmaxf(mx, wmax[1][t]), fmaxf(wmax[2][t], wmax[3][t]));
-        int e = 0;
-        if (mx > 0.f) {
-            int ex;
-            const float f = frexpf(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
-            const int cl = (f == 0.5f) ? ex - 1 : ex;  // ceil(log2 mx)
-            e = min(SP_EMAX, max(-SP_EMAX, 14 - cl));
-        }
+        const int e = split_exp(mx);
         rexp[t] = e;
         J.exps[r0 + t] = e;
     }

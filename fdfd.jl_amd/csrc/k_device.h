@@ -40,6 +40,30 @@ template <typename T> __device__ inline T gload_s(const T* p) { return *(const G
 __device__ inline f4v gload2c(const float2* p) { return *(const GLOBAL_AS f4v*)p; }
 __device__ inline void gstore2c(float2* p, f4v v) { *(GLOBAL_AS f4v*)p = v; }
 
+// Split operands of the three-product (fp16x3) matrix products: a row of
+// FP32 values with largest |re|, |im| = mx is scaled by 2^split_exp(mx), which
+// brings mx to at most 2^14, and each scaled value x is split into
+// hi = f16(x), lo = f16(x - hi) (x - hi is exact in FP32).
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+constexpr int SP_EMAX = 60;  // row exponents are clamped to [-SP_EMAX, SP_EMAX]
+
+// 2^e as a float, |e| <= 126
+__device__ inline float exp2i(int e) { return __int_as_float((127 + e) << 23); }
+
+__device__ inline int split_exp(float mx) {
+    if (!(mx > 0.f)) return 0;
+    int ex;
+    const float f = frexpf(mx, &ex);            // mx = f 2^ex, f in [0.5, 1)
+    const int cl = (f == 0.5f) ? ex - 1 : ex;  // ceil(log2 mx)
+    return min(SP_EMAX, max(-SP_EMAX, 14 - cl));
+}
+
+__device__ inline void split_f16(float x, _Float16& h, _Float16& l) {
+    h = (_Float16)x;
+    l = (_Float16)(x - (float)h);
+}
+
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (B < E) {

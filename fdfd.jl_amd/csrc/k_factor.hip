@@ -1037,6 +1037,226 @@ __global__ __launch_bounds__(256) void k_trailing_f32(const LineJob* jobs, int k
     }
 }
 
+// Mixed-path trailing update on fp16x3 products (n % 4 == 0): the product,
+// the exclusions and the output layout of k_trailing_f32 (each lane holds
+// row i and four consecutive columns j of the result), with every chunk of
+// at most 32 pivot columns -- one K step of v_mfma_f32_16x16x32_f16 -- split
+// inside the workgroup.  The V tile's rows and the Z tile's columns are
+// scaled by 2^split_exp(largest |re|, |im| over the chunk) and written to LDS
+// as fp16 hi / lo planes in the fragment layout (lane l of a 16-row block
+// reads the 16 bytes at 16 l: row l % 16, k = 8 (l / 16) + [0, 8)); each
+// real product is hi hi + hi lo + lo hi with FP32 accumulation, and a
+// chunk's sums are unscaled exactly by 2^-(e_i + e_j) into FP32 totals.  All
+// global loads (both chunks of V and Z, the old tile) are issued before the
+// first split, and one barrier separates the splits from the products.
+__global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int k0, int nb1,
+                                                        int nb2, int ntx, int total) {
+    using C = float2;
+    const int L = blockIdx.x, per = total / 8;
+    const int lin = (L < 8 * per) ? (L % 8) * per + L / 8 : L;
+    const int job = lin / (ntx * ntx), rem = lin % (ntx * ntx);
+    const LineJob J = jobs[job];
+    const int n = J.n;
+    const int i0 = (rem / ntx) * 64, j0 = (rem % ntx) * 64;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    const bool fused = nb2 > 0;
+    const int nchunk = fused ? 2 : 1;
+    const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
+    const int k2 = k0 + PMAX;                               // start of K2 (fused)
+    // [chunk][plane: re hi, re lo, im hi, im lo][16-row block][lane]
+    __shared__ __align__(16) h8v sV[2][4][4][64];
+    __shared__ __align__(16) h8v sZ[2][4][4][64];
+    __shared__ int eV[2][64], eZ[2][64];
+    C* A = (C*)J.Y;
+    const C* Vb = (const C*)J.Vbuf;
+    const C* Zg = (const C*)J.Z;
+    // loader roles: V row iv (k = 8 lk + [0, 8) of each chunk) and Z column
+    // jz (the same k), one 16-row block per wave
+    const int iv = i0 + 16 * wv + lr, jz = j0 + 16 * wv + lr;
+    const int slv = (iv < n) ? gload_s(J.iprow + iv) - k0 : -1;
+    f4v pv[2][4];
+    C pz[2][8];
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (ch < nchunk) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                pv[ch][u] = gload2c(Vb + (iv < n ? (long)iv * VMAX + ch * PMAX + 8 * lk + 2 * u : 0));
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                pz[ch][e] = gload(Zg + (jz < n ? (long)(ch * PMAX + 8 * lk + e) * n + jz : 0));
+        }
+    }
+    // old values of this lane's rows i (two blocks) and columns jb .. jb + 3
+    f4v raw[2][2][2];
+    int slot[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int i = i0 + wr * 32 + bi * 16 + lr;
+        const int sl = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
+        slot[bi] = sl;
+        const bool p1 = fused && sl >= 0 && sl < PMAX;
+        const C* src = p1 ? Zg : A;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
+            const long off = (i < n && jb < n) ? (p1 ? (long)sl * n + jb : (long)i * n + jb) : 0;
+            raw[bi][bj][0] = gload2c(src + off);
+            raw[bi][bj][1] = gload2c(src + off + 2);
+        }
+    }
+    // split: masked values, the maximum over the 4 lanes (lk) sharing the
+    // row / column, exponent, scaled fp16 planes
+    auto put = [&](h8v (*dst)[4][64], int* ex, const float* xr, const float* xi) {
+        float m = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) m = fmaxf(m, fmaxf(fabsf(xr[q]), fabsf(xi[q])));
+        m = fmaxf(m, __shfl_xor(m, 16, 64));
+        m = fmaxf(m, __shfl_xor(m, 32, 64));
+        const int e = split_exp(m);
+        const float sc = exp2i(e);
+        h8v rh, rl, ih, il;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            _Float16 h, l;
+            split_f16(xr[q] * sc, h, l);
+            rh[q] = h;
+            rl[q] = l;
+            split_f16(xi[q] * sc, h, l);
+            ih[q] = h;
+            il[q] = l;
+        }
+        dst[0][wv][lane] = rh;
+        dst[1][wv][lane] = rl;
+        dst[2][wv][lane] = ih;
+        dst[3][wv][lane] = il;
+        if (lk == 0) ex[16 * wv + lr] = e;
+    };
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (ch < nchunk) {
+            const int nbc = (ch == 0) ? nb1 : nb2;
+            // the first panel's pivot rows take no V1 part in a fused pair
+            const bool okv = iv < n && !(fused && ch == 0 && slv >= 0 && slv < PMAX);
+            // Z1' : Z1 with the K2 columns zeroed
+            const bool okz = jz < n && !(fused && ch == 0 && jz >= k2 && jz < kend);
+            float xr[8], xi[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const bool ok = okv && 8 * lk + q < nbc;
+                xr[q] = ok ? pv[ch][q >> 1][2 * (q & 1)] : 0.f;
+                xi[q] = ok ? pv[ch][q >> 1][2 * (q & 1) + 1] : 0.f;
+            }
+            put(sV[ch], eV[ch], xr, xi);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const bool ok = okz && 8 * lk + q < nbc;
+                xr[q] = ok ? pz[ch][q].x : 0.f;
+                xi[q] = ok ? pz[ch][q].y : 0.f;
+            }
+            put(sZ[ch], eZ[ch], xr, xi);
+        }
+    }
+    __syncthreads();
+    // D[j][i] = sum_k Z[k][j] V[i][k]: the A operand is the Z^T block (rows
+    // j), the B operand the V^T block (columns i)
+    f4v tre[2][2], tim[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            tre[bi][bj] = f4v{0, 0, 0, 0};
+            tim[bi][bj] = f4v{0, 0, 0, 0};
+        }
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch) {
+        if (ch < nchunk) {
+            h8v a[2][4], b[2][4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) a[bj][p] = sZ[ch][p][wc * 2 + bj][lane];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi) b[bi][p] = sV[ch][p][wr * 2 + bi][lane];
+            }
+            f4v cre[2][2], cim[2][2], cii[2][2];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    cre[bi][bj] = f4v{0, 0, 0, 0};
+                    cim[bi][bj] = f4v{0, 0, 0, 0};
+                    cii[bi][bj] = f4v{0, 0, 0, 0};
+                }
+#define H3_MM(ACC, X0, X1, Y0, Y1)                                                                 \
+    _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
+        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y0, ACC[bi][bj], 0, 0, 0);            \
+    _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
+        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y1, ACC[bi][bj], 0, 0, 0);            \
+    _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
+        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X1, Y0, ACC[bi][bj], 0, 0, 0);
+            H3_MM(cre, a[bj][0], a[bj][1], b[bi][0], b[bi][1])  // Zr Vr
+            H3_MM(cii, a[bj][2], a[bj][3], b[bi][2], b[bi][3])  // Zi Vi
+            H3_MM(cim, a[bj][0], a[bj][1], b[bi][2], b[bi][3])  // Zr Vi
+            H3_MM(cim, a[bj][2], a[bj][3], b[bi][0], b[bi][1])  // Zi Vr
+#undef H3_MM
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi) {
+                const int ei = eV[ch][wr * 32 + bi * 16 + lr];
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const float sc = exp2i(-(ei + eZ[ch][wc * 32 + bj * 16 + 4 * lk + v]));
+                        tre[bi][bj][v] += (cre[bi][bj][v] - cii[bi][bj][v]) * sc;
+                        tim[bi][bj][v] += cim[bi][bj][v] * sc;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int i = i0 + wr * 32 + bi * 16 + lr;
+        if (i >= n) continue;
+        const int sl = slot[bi];
+        const bool p1 = fused && sl >= 0 && sl < PMAX;
+        // rows replaced by Z: the panel's pivots (single) or the second
+        // panel's pivots (fused)
+        const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
+            if (jb >= n) continue;
+            float o[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = jb + v;
+                const bool ex = p1 ? (j >= k2 && j < kend) : (j >= k0 && j < kend);
+                const f4v& rw = raw[bi][bj][v >> 1];
+                const float orr = ex ? 0.f : rw[(v & 1) * 2], oii = ex ? 0.f : rw[(v & 1) * 2 + 1];
+                o[2 * v] = orr - tre[bi][bj][v];
+                o[2 * v + 1] = oii - tim[bi][bj][v];
+            }
+            f4v o0 = {o[0], o[1], o[2], o[3]}, o1 = {o[4], o[5], o[6], o[7]};
+            if (zrow) {
+                o0 = gload2c(Zg + (long)sl * n + jb);
+                o1 = gload2c(Zg + (long)sl * n + jb + 2);
+            }
+            gstore2c(A + (long)i * n + jb, o0);
+            gstore2c(A + (long)i * n + jb + 2, o1);
+        }
+    }
+}
+
+void launch_trailing_h3(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
+                        int nb2) {
+    const int ntx = (n + 63) / 64;
+    const int total = ntx * ntx * njobs;
+    k_trailing_h3<<<total, 256, 0, s>>>(jobs_d, k0, nb1, nb2, ntx, total);
+}
+
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
                      int nb2) {

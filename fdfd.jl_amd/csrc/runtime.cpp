@@ -406,6 +406,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.gemm_waves = (ew && ew[0] == '4') ? 4 : 8;
     const char* eb = std::getenv("FDFD_MIXED_GEMM");
     hh->impl.gemm_b3 = (eb && std::strcmp(eb, "fp32") == 0) ? 0 : 1;
+    const char* et = std::getenv("FDFD_MIXED_TRAIL");
+    hh->impl.trail_h3 = (et && std::strcmp(et, "fp32") == 0) ? 0 : 1;
     const char* ef = std::getenv("FDFD_FUSE_PANELS");
     hh->impl.fuse_panels = (ef && ef[0] == '0') ? 0 : 1;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {

@@ -91,6 +91,7 @@ struct Handle {
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
     int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)
     int gemm_b3 = 1;                   // mixed path: three-product fp16 GEMMs (FDFD_MIXED_GEMM=fp32: FP32)
+    int trail_h3 = 1;                  // mixed path: three-product fp16 trailing updates (FDFD_MIXED_TRAIL=fp32: FP32)
     int fuse_panels = 1;               // two Gauss-Jordan panels per trailing pass (FDFD_FUSE_PANELS=0)
     // sweep / Krylov work
     long work_N = 0;

@@ -143,8 +143,39 @@ def test_fp16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
     assert its["fp16x3"] <= its["fp32"] + 4
 
 
+@pytest.mark.parametrize("Nx,Ny,npml", [(100, 84, 6), (76, 120, 6)])
+def test_fp16x3_trailing_tracks_fp32_trailing(monkeypatch, Nx, Ny, npml):
+    """The Gauss-Jordan trailing update on fp16x3 products (line lengths
+    n % 4 == 0; here 84 and 76, not multiples of the 64 x 64 tiles, with a
+    last panel pair shorter than 64 columns) needs about as many refinement
+    iterations as the FP32 one and reaches the same residual."""
+    d = _device(Nx, Ny, npml, seed=7 + Nx)
+    om = float(d.omega[0])
+    A = O.assemble(d.grid, d.eps_r, om, O.TM)
+    b = O.rhs(om, d.src)
+    xo, _ = O.linear_solve(A, b)
+    its = {}
+    for trail in ("fp32", "fp16x3"):
+        monkeypatch.setenv("FDFD_MIXED_TRAIL", trail)  # read when the handle is created
+        eng = Engine(0)
+        try:
+            _engine_grid(eng, d)
+            fid, _ = eng.factor(O.TM, om, "mixed")
+            x, st = eng.linsolve(fid, b, tol=1e-13)
+            eng.release(fid)
+        finally:
+            eng.close()
+        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        its[trail] = st["iterations"]
+        print(f"{Nx}x{Ny} trail={trail}: res={res:.2e} err={err:.2e} its={its[trail]}", flush=True)
+        assert res <= 1e-12
+        assert err <= FIELD_TOL
+    assert its["fp16x3"] <= its["fp32"] + 3
+
+
 @pytest.mark.parametrize("switch", ["FDFD_FUSE_PANELS=0", "FDFD_GEMM3M=0", "FDFD_GEMM_WAVES=4",
-                                    "FDFD_GRAPHS=0"])
+                                    "FDFD_GRAPHS=0", "FDFD_MIXED_TRAIL=fp32"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_handle_switches_match_oracle(monkeypatch, switch, precision):
     """Every kernel variant selectable at handle creation (one panel per
