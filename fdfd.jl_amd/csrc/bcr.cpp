@@ -319,19 +319,29 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     delete pss;
                     os += ns;
                 }
-                double fl = 0;
-                for (int i = 0; i < nj; ++i) fl += 8.0 * n * (double)n * n * gemms[og + i].nt;
-                ProfScope* ps = scope(P_GEMM, fl, 0);
-                if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
-                else if (b3) {
+                const double fl1 = 8.0 * n * (double)n * n;  // one complex n^3 product
+                if (b3) {
                     // first terms, then the second terms accumulating into them
+                    // (one profile scope per launch)
                     const int n2 = nsecond[ogs - 1];
+                    ProfScope* ps = scope(P_GEMM, fl1 * nj, 0);
                     launch_gemm_b3(st, T.gemm + og, nj, n);
-                    if (n2 > 0) launch_gemm_b3(st, T.gemm + og + nj, n2, n);
-                    og += n2;
-                } else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
-                delete ps;
-                og += nj;
+                    delete ps;
+                    if (n2 > 0) {
+                        ps = scope(P_GEMM, fl1 * n2, 0);
+                        launch_gemm_b3(st, T.gemm + og + nj, n2, n);
+                        delete ps;
+                    }
+                    og += nj + n2;
+                } else {
+                    double fl = 0;
+                    for (int i = 0; i < nj; ++i) fl += fl1 * gemms[og + i].nt;
+                    ProfScope* ps = scope(P_GEMM, fl, 0);
+                    if (fp64) launch_gemm<double>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
+                    else launch_gemm<float>(st, T.gemm + og, nj, n, h->gemm3m, h->gemm_waves);
+                    delete ps;
+                    og += nj;
+                }
             } else {
                 const LineJob* J = T.lj + ol;
                 launch_gjreset(st, J, nj, n);

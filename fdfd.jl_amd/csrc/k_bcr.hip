@@ -528,127 +528,95 @@ template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int
 // One workgroup per 16-row strip (tile row block rb) of an operand: as is,
 // strip rows are block rows 16 rb + r and k runs over the columns;
 // transposed, strip rows are block columns c = 16 rb + r and k runs over the
-// block rows through the gather, (r, k) = src[map[k]][c].  Pass 1 takes each
-// row's largest |re|, |im| and its exponent; pass 2 (re-reading the strip,
-// now in L2) writes the four scaled fp16 planes, one wave per 1 KB tile.
+// block rows through the gather, (r, k) = src[map[k]][c].  The strip is read
+// once into registers in the tile layout -- wave wv holds tiles kb = wv,
+// wv + 4, .. (at most SP_TPW for n <= MAX_LINE), lane l row l % 16 and
+// k = 32 kb + 8 (l / 16) + [0, 8) -- then each row's largest |re|, |im|
+// (lanes l, l ^ 16, l ^ 32, l ^ 48 of all four waves) gives its exponent and
+// the scaled fp16 planes are written, one wave per 1 KB tile.
+constexpr int SP_TPW = (MAX_LINE / 32 + 3) / 4;
+
 __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     const SplitJob J = jobs[blockIdx.y];
     const int rb = blockIdx.x, r0 = 16 * rb;
     const int nkb = b3_nkb(n);
     const float2* src = (const float2*)J.src;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int r = lane & 15, row = r0 + r, kq = 8 * (lane >> 4);
     __shared__ float wmax[4][16];
-    __shared__ int rexp[16];
-    auto mag = [](float2 v) { return fmaxf(fabsf(v.x), fabsf(v.y)); };
-    if (!J.trans) {
-        // wave wv reads rows wv + 4 q (q < 4), its lanes along k (coalesced)
-        float m[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < n; k0 += 256) {
-            float2 v[4][4];
+    float2 v[SP_TPW][8];
+    // loads: clamped addresses, no branches (entries past n are masked below)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < SP_TPW; ++q) {
+        const int kb = wv + 4 * q;
+        if (kb < nkb) {
+            if (!J.trans) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int r = r0 + wv + 4 * q, k = k0 + 64 * u + lane;
-                    v[q][u] = gload(src + ((r < n && k < n) ? (long)r * n + k : 0));
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 32 * kb + kq + e;
+                    v[q][e] = gload(src + ((row < n && k < n) ? (long)row * n + k : 0));
+                }
+            } else {
+                int mr[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 32 * kb + kq + e;
+                    mr[e] = (J.map && k < n) ? gload_s(J.map + k) : k;
                 }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int r = r0 + wv + 4 * q, k = k0 + 64 * u + lane;
-                    if (r < n && k < n) m[q] = fmaxf(m[q], mag(v[q][u]));
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 32 * kb + kq + e;
+                    v[q][e] = gload(src + ((row < n && k < n) ? (long)mr[e] * n + row : 0));
                 }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            for (int o = 32; o > 0; o >>= 1) m[q] = fmaxf(m[q], __shfl_xor(m[q], o, 64));
-            if (lane == 0) wmax[0][wv + 4 * q] = m[q];
-        }
-    } else {
-        // thread t reads column r0 + t % 16 of block rows map[g + 16 i],
-        // g = t / 16: 16 lanes cover 128 contiguous bytes of one block row
-        const int r = t & 15, g = t >> 4, c = r0 + r;
-        float m = 0.f;
-        for (int k0 = 0; k0 < n; k0 += 16 * 8) {
-            int mr[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + g + 16 * u;
-                mr[u] = (J.map && k < n) ? gload_s(J.map + k) : k;
-            }
-            float2 v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + g + 16 * u;
-                v[u] = gload(src + ((k < n && c < n) ? (long)mr[u] * n + c : 0));
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + g + 16 * u;
-                if (k < n && c < n) m = fmaxf(m, mag(v[u]));
             }
         }
-        // the 16 threads of column r: lanes r, r + 16, r + 32, r + 48 of each wave
-        m = fmaxf(m, __shfl_xor(m, 16, 64));
-        m = fmaxf(m, __shfl_xor(m, 32, 64));
-        if (lane < 16) wmax[wv][lane] = m;
     }
-    __syncthreads();
-    if (t < 16) {
-        float mx = wmax[0][t];
-        if (J.trans) mx = fmaxf(fmaxf(mx, wmax[1][t]), fmaxf(wmax[2][t], wmax[3][t]));
-        const int e = split_exp(mx);
-        rexp[t] = e;
-        J.exps[r0 + t] = e;
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < SP_TPW; ++q) {
+        const int kb = wv + 4 * q;
+        if (kb < nkb) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = 32 * kb + kq + e;
+                const bool ok = row < n && k < n;
+                v[q][e].x = ok ? v[q][e].x : 0.f;
+                v[q][e].y = ok ? v[q][e].y : 0.f;
+                m = fmaxf(m, fmaxf(fabsf(v[q][e].x), fabsf(v[q][e].y)));
+            }
+        }
     }
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    if (lane < 16) wmax[wv][lane] = m;
     __syncthreads();
-    // pass 2: wave wv writes tiles kb = wv, wv + 4, ..; lane l holds row
-    // l % 16, k = 32 kb + 8 (l / 16) + [0, 8)
+    const float mx = fmaxf(fmaxf(wmax[0][r], wmax[1][r]), fmaxf(wmax[2][r], wmax[3][r]));
+    const int ex = split_exp(mx);
+    if (t < 16) J.exps[r0 + t] = ex;
+    const float sc = exp2i(ex);
     const long plane = b3_plane(n);
     _Float16* dst = (_Float16*)J.dst;
-    const int r = lane & 15, row = r0 + r, kq = 8 * (lane >> 4);
-    const float sc = exp2i(rexp[r]);
-    for (int kb = wv; kb < nkb; kb += 4) {
-        const int kbase = 32 * kb + kq;
-        float2 v[8];
-        if (!J.trans) {
+#pragma unroll
+    for (int q = 0; q < SP_TPW; ++q) {
+        const int kb = wv + 4 * q;
+        if (kb < nkb) {
+            h8v rh, rl, ih, il;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int k = kbase + e;
-                v[e] = gload(src + ((row < n && k < n) ? (long)row * n + k : 0));
+                _Float16 h, l;
+                split_f16(v[q][e].x * sc, h, l);
+                rh[e] = h;
+                rl[e] = l;
+                split_f16(v[q][e].y * sc, h, l);
+                ih[e] = h;
+                il[e] = l;
             }
-        } else {
-            int mr[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = kbase + e;
-                mr[e] = (J.map && k < n) ? gload_s(J.map + k) : k;
-            }
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = kbase + e;
-                v[e] = gload(src + ((row < n && k < n) ? (long)mr[e] * n + row : 0));
-            }
+            const long o = ((long)rb * nkb + kb) * 512 + 8 * lane;
+            *(GLOBAL_AS h8v*)(dst + o) = rh;
+            *(GLOBAL_AS h8v*)(dst + plane + o) = rl;
+            *(GLOBAL_AS h8v*)(dst + 2 * plane + o) = ih;
+            *(GLOBAL_AS h8v*)(dst + 3 * plane + o) = il;
         }
-        h8v rh, rl, ih, il;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = kbase + e;
-            const bool ok = row < n && k < n;
-            _Float16 h, l;
-            split_f16(ok ? v[e].x * sc : 0.f, h, l);
-            rh[e] = h;
-            rl[e] = l;
-            split_f16(ok ? v[e].y * sc : 0.f, h, l);
-            ih[e] = h;
-            il[e] = l;
-        }
-        const long o = ((long)rb * nkb + kb) * 512 + 8 * lane;
-        *(GLOBAL_AS h8v*)(dst + o) = rh;
-        *(GLOBAL_AS h8v*)(dst + plane + o) = rl;
-        *(GLOBAL_AS h8v*)(dst + 2 * plane + o) = ih;
-        *(GLOBAL_AS h8v*)(dst + 3 * plane + o) = il;
     }
 }
 
