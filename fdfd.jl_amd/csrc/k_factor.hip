@@ -1037,18 +1037,23 @@ __global__ __launch_bounds__(256) void k_trailing_f32(const LineJob* jobs, int k
     }
 }
 
-// Mixed-path trailing update on fp16x3 products (n % 4 == 0): the product,
-// the exclusions and the output layout of k_trailing_f32 (each lane holds
-// row i and four consecutive columns j of the result), with every chunk of
-// at most 32 pivot columns -- one K step of v_mfma_f32_16x16x32_f16 -- split
-// inside the workgroup.  The V tile's rows and the Z tile's columns are
-// scaled by 2^split_exp(largest |re|, |im| over the chunk) and written to LDS
-// as fp16 hi / lo planes in the fragment layout (lane l of a 16-row block
-// reads the 16 bytes at 16 l: row l % 16, k = 8 (l / 16) + [0, 8)); each
-// real product is hi hi + hi lo + lo hi with FP32 accumulation, and a
-// chunk's sums are unscaled exactly by 2^-(e_i + e_j) into FP32 totals.  All
-// global loads (both chunks of V and Z, the old tile) are issued before the
-// first split, and one barrier separates the splits from the products.
+// Mixed-path trailing update on fp16x3 products (n % 4 == 0): the update,
+// exclusions and pivot-row rules of k_trailing_f32 with the products formed
+// in fp16x3.  Every chunk of at most 32 pivot columns is one K step of
+// v_mfma_f32_16x16x32_f16, so the workgroup splits it itself: the V tile's
+// rows and the Z tile's columns are scaled by 2^split_exp(largest |re|,
+// |im| over the chunk) and written to LDS as fp16 hi / lo planes in the
+// fragment layout (lane l of a 16-row block reads the 16 bytes at 16 l: row
+// l % 16, k = 8 (l / 16) + [0, 8)); each real product is
+// hi hi + hi lo + lo hi with FP32 accumulation, and each chunk's sums are
+// unscaled exactly by 2^-(e_i + e_j) into FP32 totals.  The totals then go
+// through LDS (over the operand planes, after a barrier) so that the old
+// tile is read and the result written as whole 512-byte rows: row-contiguous
+// tiles stream at about 1.5x the rate of the MFMA output layout's
+// 32-byte pieces.  A row's source is chosen when it is loaded (A, or its Z
+// row for pivot rows), so the epilogue issues no dependent load.
+constexpr int H3_PROW = 66;  // product tile row stride (complex): 528 bytes
+
 __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int k0, int nb1,
                                                         int nb2, int ntx, int total) {
     using C = float2;
@@ -1065,9 +1070,9 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
     const int nchunk = fused ? 2 : 1;
     const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
     const int k2 = k0 + PMAX;                               // start of K2 (fused)
-    // [chunk][plane: re hi, re lo, im hi, im lo][16-row block][lane]
-    __shared__ __align__(16) h8v sV[2][4][4][64];
-    __shared__ __align__(16) h8v sZ[2][4][4][64];
+    // [operand: V, Z][chunk][plane: re hi, re lo, im hi, im lo][16-row block][lane];
+    // after the products, the 64 x H3_PROW complex product tile
+    __shared__ __align__(16) h8v sOp[2][2][4][4][64];
     __shared__ int eV[2][64], eZ[2][64];
     C* A = (C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
@@ -1089,23 +1094,24 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
                 pz[ch][e] = gload(Zg + (jz < n ? (long)(ch * PMAX + 8 * lk + e) * n + jz : 0));
         }
     }
-    // old values of this lane's rows i (two blocks) and columns jb .. jb + 3
-    f4v raw[2][2][2];
-    int slot[2];
+    // row roles: row r = t / 32 + 8 q of the tile, columns c, c + 1 with
+    // c = 2 (t % 32); pivot rows read their Z row (first-panel pivots of a
+    // fused pair: Z1 - V2 Z2; rows replaced by Z: Z as is)
+    const int rc = 2 * (t & 31);
+    f4v old[8];
+    int rsl[8];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        const int i = i0 + wr * 32 + bi * 16 + lr;
-        const int sl = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
-        slot[bi] = sl;
-        const bool p1 = fused && sl >= 0 && sl < PMAX;
-        const C* src = p1 ? Zg : A;
+    for (int q = 0; q < 8; ++q) {
+        const int i = i0 + (t >> 5) + 8 * q;
+        rsl[q] = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
+    }
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
-            const long off = (i < n && jb < n) ? (p1 ? (long)sl * n + jb : (long)i * n + jb) : 0;
-            raw[bi][bj][0] = gload2c(src + off);
-            raw[bi][bj][1] = gload2c(src + off + 2);
-        }
+    for (int q = 0; q < 8; ++q) {
+        const int i = i0 + (t >> 5) + 8 * q, j = j0 + rc;
+        const int sl = rsl[q];
+        const bool zsrc = fused ? (sl >= 0 && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+        const long off = (i < n && j < n) ? (zsrc ? (long)sl * n + j : (long)i * n + j) : 0;
+        old[q] = gload2c((zsrc ? Zg : A) + off);
     }
     // split: masked values, the maximum over the 4 lanes (lk) sharing the
     // row / column, exponent, scaled fp16 planes
@@ -1149,19 +1155,20 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
                 xr[q] = ok ? pv[ch][q >> 1][2 * (q & 1)] : 0.f;
                 xi[q] = ok ? pv[ch][q >> 1][2 * (q & 1) + 1] : 0.f;
             }
-            put(sV[ch], eV[ch], xr, xi);
+            put(sOp[0][ch], eV[ch], xr, xi);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const bool ok = okz && 8 * lk + q < nbc;
                 xr[q] = ok ? pz[ch][q].x : 0.f;
                 xi[q] = ok ? pz[ch][q].y : 0.f;
             }
-            put(sZ[ch], eZ[ch], xr, xi);
+            put(sOp[1][ch], eZ[ch], xr, xi);
         }
     }
     __syncthreads();
     // D[j][i] = sum_k Z[k][j] V[i][k]: the A operand is the Z^T block (rows
-    // j), the B operand the V^T block (columns i)
+    // j), the B operand the V^T block (columns i); lane (lr, lk) of block
+    // (bi, bj) holds row i = 32 wr + 16 bi + lr, columns 32 wc + 16 bj + 4 lk + v
     f4v tre[2][2], tim[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
@@ -1177,9 +1184,9 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) a[bj][p] = sZ[ch][p][wc * 2 + bj][lane];
+                for (int bj = 0; bj < 2; ++bj) a[bj][p] = sOp[1][ch][p][wc * 2 + bj][lane];
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi) b[bi][p] = sV[ch][p][wr * 2 + bi][lane];
+                for (int bi = 0; bi < 2; ++bi) b[bi][p] = sOp[0][ch][p][wr * 2 + bi][lane];
             }
             f4v cre[2][2], cim[2][2], cii[2][2];
 #pragma unroll
@@ -1216,36 +1223,37 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
             }
         }
     }
+    __syncthreads();  // every wave is done with the operand planes
+    C* sP = (C*)&sOp[0][0][0][0][0];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        const int i = i0 + wr * 32 + bi * 16 + lr;
-        if (i >= n) continue;
-        const int sl = slot[bi];
-        const bool p1 = fused && sl >= 0 && sl < PMAX;
-        // rows replaced by Z: the panel's pivots (single) or the second
-        // panel's pivots (fused)
-        const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) {
-            const int jb = j0 + wc * 32 + bj * 16 + 4 * lk;
-            if (jb >= n) continue;
-            float o[8];
+            const int o = (wr * 32 + bi * 16 + lr) * H3_PROW + wc * 32 + bj * 16 + 4 * lk;
+            *(f4v*)(sP + o) = f4v{tre[bi][bj][0], tim[bi][bj][0], tre[bi][bj][1], tim[bi][bj][1]};
+            *(f4v*)(sP + o + 2) = f4v{tre[bi][bj][2], tim[bi][bj][2], tre[bi][bj][3], tim[bi][bj][3]};
+        }
+    __syncthreads();
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = jb + v;
-                const bool ex = p1 ? (j >= k2 && j < kend) : (j >= k0 && j < kend);
-                const f4v& rw = raw[bi][bj][v >> 1];
-                const float orr = ex ? 0.f : rw[(v & 1) * 2], oii = ex ? 0.f : rw[(v & 1) * 2 + 1];
-                o[2 * v] = orr - tre[bi][bj][v];
-                o[2 * v + 1] = oii - tim[bi][bj][v];
+    for (int q = 0; q < 8; ++q) {
+        const int r = (t >> 5) + 8 * q, i = i0 + r, j = j0 + rc;
+        if (i < n && j < n) {
+            const int sl = rsl[q];
+            const bool p1 = fused && sl >= 0 && sl < PMAX;
+            // rows replaced by Z: the panel's pivots (single) or the second
+            // panel's pivots (fused)
+            const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+            const f4v pr = *(const f4v*)(sP + r * H3_PROW + rc);
+            f4v o = old[q];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int jj = j + h;
+                const bool ex = p1 ? (jj >= k2 && jj < kend) : (jj >= k0 && jj < kend);
+                const float orr = ex ? 0.f : old[q][2 * h], oii = ex ? 0.f : old[q][2 * h + 1];
+                o[2 * h] = zrow ? old[q][2 * h] : orr - pr[2 * h];
+                o[2 * h + 1] = zrow ? old[q][2 * h + 1] : oii - pr[2 * h + 1];
             }
-            f4v o0 = {o[0], o[1], o[2], o[3]}, o1 = {o[4], o[5], o[6], o[7]};
-            if (zrow) {
-                o0 = gload2c(Zg + (long)sl * n + jb);
-                o1 = gload2c(Zg + (long)sl * n + jb + 2);
-            }
-            gstore2c(A + (long)i * n + jb, o0);
-            gstore2c(A + (long)i * n + jb + 2, o1);
+            gstore2c(A + (long)i * n + j, o);
         }
     }
 }
