@@ -738,18 +738,19 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
 #pragma unroll
             for (int v = 0; v < 4; ++v) b[q][v] = sB[cur][v][wc * 2 + q][lane];
         // (x_hi y_hi + x_hi y_lo + x_lo y_hi) for the four real products,
-        // blocks innermost so consecutive MFMAs use different accumulators
-#define B3_MM(ACC, X0, X1, Y0, Y1)                                                             \
+        // terms outermost: each term round runs the four products over all
+        // 8 blocks, so an accumulator's next MFMA is 16-32 MFMAs away
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int xa = (r == 2) ? 1 : 0, yb = (r == 1) ? 1 : 0;  // hi hi, hi lo, lo hi
+#define B3_MM(ACC, XP, YP)                                                                      \
     _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y0, ACC[p][q], 0, 0, 0);          \
-    _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y1, ACC[p][q], 0, 0, 0);          \
-    _Pragma("unroll") for (int p = 0; p < 4; ++p) _Pragma("unroll") for (int q = 0; q < 2; ++q) \
-        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X1, Y0, ACC[p][q], 0, 0, 0);
-        B3_MM(cre, a[p][0], a[p][1], b[q][0], b[q][1])  // Ar Br
-        B3_MM(cim, a[p][0], a[p][1], b[q][2], b[q][3])  // Ar Bi
-        B3_MM(cii, a[p][2], a[p][3], b[q][2], b[q][3])  // Ai Bi
-        B3_MM(cim, a[p][2], a[p][3], b[q][0], b[q][1])  // Ai Br
+        ACC[p][q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[p][XP + xa], b[q][YP + yb], ACC[p][q], 0, 0, 0);
+            B3_MM(cre, 0, 0)  // Ar Br
+            B3_MM(cim, 0, 2)  // Ar Bi
+            B3_MM(cii, 2, 2)  // Ai Bi
+            B3_MM(cim, 2, 0)  // Ai Br
+        }
 #undef B3_MM
     };
     zero();

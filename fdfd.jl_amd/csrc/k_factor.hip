@@ -1197,17 +1197,19 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
                     cim[bi][bj] = f4v{0, 0, 0, 0};
                     cii[bi][bj] = f4v{0, 0, 0, 0};
                 }
-#define H3_MM(ACC, X0, X1, Y0, Y1)                                                                 \
+            // terms outermost (hi hi, hi lo, lo hi), the four products over
+            // the four blocks inside, so dependent MFMAs are 8-16 apart
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const int xa = (r == 2) ? 1 : 0, yb = (r == 1) ? 1 : 0;
+#define H3_MM(ACC, XP, YP)                                                                          \
     _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
-        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y0, ACC[bi][bj], 0, 0, 0);            \
-    _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
-        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X0, Y1, ACC[bi][bj], 0, 0, 0);            \
-    _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
-        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(X1, Y0, ACC[bi][bj], 0, 0, 0);
-            H3_MM(cre, a[bj][0], a[bj][1], b[bi][0], b[bi][1])  // Zr Vr
-            H3_MM(cii, a[bj][2], a[bj][3], b[bi][2], b[bi][3])  // Zi Vi
-            H3_MM(cim, a[bj][0], a[bj][1], b[bi][2], b[bi][3])  // Zr Vi
-            H3_MM(cim, a[bj][2], a[bj][3], b[bi][0], b[bi][1])  // Zi Vr
+        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[bj][XP + xa], b[bi][YP + yb], ACC[bi][bj], 0, 0, 0);
+                H3_MM(cre, 0, 0)  // Zr Vr
+                H3_MM(cim, 0, 2)  // Zr Vi
+                H3_MM(cii, 2, 2)  // Zi Vi
+                H3_MM(cim, 2, 0)  // Zi Vr
+            }
 #undef H3_MM
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi) {
