@@ -339,8 +339,8 @@ def main():
         rt = "float" if head == "mixed" else "double"
         # kernel-name prefix of the dominant slot (the mixed path's GEMM is
         # the split-operand fp16x3 kernel)
-        # (FP32 lines with n % 4 == 0 use the 16-byte-access trailing kernel)
-        trail = "k_trailing_f32" if head == "mixed" and n % 4 == 0 else "k_trailing<" + rt
+        # (mixed lines with n % 4 == 0 use the fp16x3 row-strip trailing kernel)
+        trail = "k_trailing_h3" if head == "mixed" and n % 4 == 0 else "k_trailing<" + rt
         prefix = {"gemm": "k_gemm_b3" if head == "mixed" else "k_gemm<" + rt,
                   "trailing": trail, "solve": "k_mv<" + rt}.get(dom)
         for name, v in P["kernels"].items():

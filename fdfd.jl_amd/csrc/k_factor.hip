@@ -1040,29 +1040,38 @@ __global__ __launch_bounds__(256) void k_trailing_f32(const LineJob* jobs, int k
 // Mixed-path trailing update on fp16x3 products (n % 4 == 0): the update,
 // exclusions and pivot-row rules of k_trailing_f32 with the products formed
 // in fp16x3.  Every chunk of at most 32 pivot columns is one K step of
-// v_mfma_f32_16x16x32_f16, so the workgroup splits it itself: the V tile's
-// rows and the Z tile's columns are scaled by 2^split_exp(largest |re|,
-// |im| over the chunk) and written to LDS as fp16 hi / lo planes in the
-// fragment layout (lane l of a 16-row block reads the 16 bytes at 16 l: row
-// l % 16, k = 8 (l / 16) + [0, 8)); each real product is
-// hi hi + hi lo + lo hi with FP32 accumulation, and each chunk's sums are
-// unscaled exactly by 2^-(e_i + e_j) into FP32 totals.  The totals then go
-// through LDS (over the operand planes, after a barrier) so that the old
-// tile is read and the result written as whole 512-byte rows: row-contiguous
-// tiles stream at about 1.5x the rate of the MFMA output layout's
-// 32-byte pieces.  A row's source is chosen when it is loaded (A, or its Z
-// row for pivot rows), so the epilogue issues no dependent load.
+// v_mfma_f32_16x16x32_f16, so the workgroup splits the operands itself: the
+// V rows and Z columns are scaled by 2^split_exp(largest |re|, |im| over both
+// chunks) and written to LDS as fp16 hi / lo planes in the fragment layout
+// (lane l of a 16-row block reads the 16 bytes at 16 l: row l % 16,
+// k = 8 (l / 16) + [0, 8)); each real product is hi hi + hi lo + lo hi with
+// FP32 accumulation, unscaled exactly by 2^-(e_i + e_j).  The products go
+// through LDS so that the old tile is read and the result written as whole
+// 512-byte rows (row-contiguous tiles stream at about 1.5x the rate of the
+// MFMA output layout's 32-byte pieces); a row's source (A, or its Z row for
+// pivot rows) is chosen when it is loaded, so the epilogue issues no
+// dependent load.
+//
+// One workgroup per 64-row strip and TPW consecutive 64 x 64 tiles of it:
+// the strip's V rows are split once, and while a tile's products and
+// epilogue run the next tile's Z columns are already loading; its old rows
+// are issued right after the epilogue.  LDS: the V planes (32 KB, kept for
+// the strip) and one 36 KB region used in turn by a tile's Z planes and its
+// product tile (two workgroups per CU).
 constexpr int H3_PROW = 66;  // product tile row stride (complex): 528 bytes
+constexpr int H3_TPW = 4;    // tiles per workgroup (2, 8 and 16 measured slower)
 
-__global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int k0, int nb1,
-                                                        int nb2, int ntx, int total) {
+template <int TPW>
+__global__ __launch_bounds__(256, 2) void k_trailing_h3s(const LineJob* jobs, int k0, int nb1,
+                                                         int nb2, int nts, int ngrp, int total) {
     using C = float2;
     const int L = blockIdx.x, per = total / 8;
     const int lin = (L < 8 * per) ? (L % 8) * per + L / 8 : L;
-    const int job = lin / (ntx * ntx), rem = lin % (ntx * ntx);
+    const int job = lin / (nts * ngrp), rem = lin % (nts * ngrp);
     const LineJob J = jobs[job];
     const int n = J.n;
-    const int i0 = (rem / ntx) * 64, j0 = (rem % ntx) * 64;
+    const int i0 = (rem / ngrp) * 64, tj0 = (rem % ngrp) * TPW;
+    const int ntl = min(TPW, nts - tj0);  // tiles of this workgroup
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     const int lr = lane & 15, lk = lane >> 4;
@@ -1070,202 +1079,240 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3(const LineJob* jobs, int
     const int nchunk = fused ? 2 : 1;
     const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
     const int k2 = k0 + PMAX;                               // start of K2 (fused)
-    // [operand: V, Z][chunk][plane: re hi, re lo, im hi, im lo][16-row block][lane];
-    // after the products, the 64 x H3_PROW complex product tile
-    __shared__ __align__(16) h8v sOp[2][2][4][4][64];
-    __shared__ int eV[2][64], eZ[2][64];
+    __shared__ __align__(16) h8v sV[2][4][4][64];  // [chunk][plane][16-row block][lane]
+    __shared__ __align__(16) h8v sZP[2][4][4][64 + 8];  // Z planes, then the product tile
+    __shared__ int eV[2][64], eZ[2][64];  // per chunk
+    h8v (*sZ)[4][4][72] = sZP;
+    C* sP = (C*)&sZP[0][0][0][0];
     C* A = (C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
     const C* Zg = (const C*)J.Z;
-    // loader roles: V row iv (k = 8 lk + [0, 8) of each chunk) and Z column
-    // jz (the same k), one 16-row block per wave
-    const int iv = i0 + 16 * wv + lr, jz = j0 + 16 * wv + lr;
-    const int slv = (iv < n) ? gload_s(J.iprow + iv) - k0 : -1;
-    f4v pv[2][4];
-    C pz[2][8];
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        if (ch < nchunk) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                pv[ch][u] = gload2c(Vb + (iv < n ? (long)iv * VMAX + ch * PMAX + 8 * lk + 2 * u : 0));
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                pz[ch][e] = gload(Zg + (jz < n ? (long)(ch * PMAX + 8 * lk + e) * n + jz : 0));
-        }
-    }
-    // row roles: row r = t / 32 + 8 q of the tile, columns c, c + 1 with
-    // c = 2 (t % 32); pivot rows read their Z row (first-panel pivots of a
-    // fused pair: Z1 - V2 Z2; rows replaced by Z: Z as is)
+    const int iv = i0 + 16 * wv + lr;
     const int rc = 2 * (t & 31);
-    f4v old[8];
+    // row roles (t / 32 + 8 q) and their slots, fixed for the strip
     int rsl[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int i = i0 + (t >> 5) + 8 * q;
         rsl[q] = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
     }
+    // ---- V rows of the strip (both chunks), split once
+    {
+        const int slv = (iv < n) ? gload_s(J.iprow + iv) - k0 : -1;
+        f4v pv[2][4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int i = i0 + (t >> 5) + 8 * q, j = j0 + rc;
-        const int sl = rsl[q];
-        const bool zsrc = fused ? (sl >= 0 && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
-        const long off = (i < n && j < n) ? (zsrc ? (long)sl * n + j : (long)i * n + j) : 0;
-        old[q] = gload2c((zsrc ? Zg : A) + off);
-    }
-    // split: masked values, the maximum over the 4 lanes (lk) sharing the
-    // row / column, exponent, scaled fp16 planes
-    auto put = [&](h8v (*dst)[4][64], int* ex, const float* xr, const float* xi) {
-        float m = 0.f;
+        for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) m = fmaxf(m, fmaxf(fabsf(xr[q]), fabsf(xi[q])));
-        m = fmaxf(m, __shfl_xor(m, 16, 64));
-        m = fmaxf(m, __shfl_xor(m, 32, 64));
-        const int e = split_exp(m);
-        const float sc = exp2i(e);
-        h8v rh, rl, ih, il;
+            for (int u = 0; u < 4; ++u)
+                pv[ch][u] = gload2c(Vb + (iv < n && ch < nchunk ? iv * VMAX + ch * PMAX + 8 * lk + 2 * u : 0));
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            _Float16 h, l;
-            split_f16(xr[q] * sc, h, l);
-            rh[q] = h;
-            rl[q] = l;
-            split_f16(xi[q] * sc, h, l);
-            ih[q] = h;
-            il[q] = l;
-        }
-        dst[0][wv][lane] = rh;
-        dst[1][wv][lane] = rl;
-        dst[2][wv][lane] = ih;
-        dst[3][wv][lane] = il;
-        if (lk == 0) ex[16 * wv + lr] = e;
-    };
-#pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        if (ch < nchunk) {
+        for (int ch = 0; ch < 2; ++ch) {
             const int nbc = (ch == 0) ? nb1 : nb2;
             // the first panel's pivot rows take no V1 part in a fused pair
-            const bool okv = iv < n && !(fused && ch == 0 && slv >= 0 && slv < PMAX);
-            // Z1' : Z1 with the K2 columns zeroed
-            const bool okz = jz < n && !(fused && ch == 0 && jz >= k2 && jz < kend);
-            float xr[8], xi[8];
+            const bool okv = ch < nchunk && iv < n && !(fused && ch == 0 && slv >= 0 && slv < PMAX);
+            float m = 0.f;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const bool ok = okv && 8 * lk + q < nbc;
-                xr[q] = ok ? pv[ch][q >> 1][2 * (q & 1)] : 0.f;
-                xi[q] = ok ? pv[ch][q >> 1][2 * (q & 1) + 1] : 0.f;
+                pv[ch][q >> 1][2 * (q & 1)] = ok ? pv[ch][q >> 1][2 * (q & 1)] : 0.f;
+                pv[ch][q >> 1][2 * (q & 1) + 1] = ok ? pv[ch][q >> 1][2 * (q & 1) + 1] : 0.f;
+                m = fmaxf(m, fmaxf(fabsf(pv[ch][q >> 1][2 * (q & 1)]), fabsf(pv[ch][q >> 1][2 * (q & 1) + 1])));
             }
-            put(sOp[0][ch], eV[ch], xr, xi);
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            const int e = split_exp(m);
+            const float sc = exp2i(e);
+            if (lk == 0) eV[ch][16 * wv + lr] = e;
+            h8v rh, rl, ih, il;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const bool ok = okz && 8 * lk + q < nbc;
-                xr[q] = ok ? pz[ch][q].x : 0.f;
-                xi[q] = ok ? pz[ch][q].y : 0.f;
+                _Float16 h, l;
+                split_f16(pv[ch][q >> 1][2 * (q & 1)] * sc, h, l);
+                rh[q] = h;
+                rl[q] = l;
+                split_f16(pv[ch][q >> 1][2 * (q & 1) + 1] * sc, h, l);
+                ih[q] = h;
+                il[q] = l;
             }
-            put(sOp[1][ch], eZ[ch], xr, xi);
+            sV[ch][0][wv][lane] = rh;
+            sV[ch][1][wv][lane] = rl;
+            sV[ch][2][wv][lane] = ih;
+            sV[ch][3][wv][lane] = il;
         }
     }
-    __syncthreads();
-    // D[j][i] = sum_k Z[k][j] V[i][k]: the A operand is the Z^T block (rows
-    // j), the B operand the V^T block (columns i); lane (lr, lk) of block
-    // (bi, bj) holds row i = 32 wr + 16 bi + lr, columns 32 wc + 16 bj + 4 lk + v
-    f4v tre[2][2], tim[2][2];
+    // ---- tiles
+    C pz[2][8];
+    auto load_z = [&](int j0) {
+        const int jz = j0 + 16 * wv + lr;
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+        for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            tre[bi][bj] = f4v{0, 0, 0, 0};
-            tim[bi][bj] = f4v{0, 0, 0, 0};
+            for (int e = 0; e < 8; ++e)
+                pz[ch][e] = gload(Zg + (jz < n && ch < nchunk ? (ch * PMAX + 8 * lk + e) * n + jz : 0));
+    };
+    f4v old[8];
+    auto load_old = [&](int j0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = i0 + (t >> 5) + 8 * q, j = j0 + rc;
+            const int sl = rsl[q];
+            const bool zsrc = fused ? (sl >= 0 && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+            // element offsets fit in 32 bits (n <= MAX_LINE)
+            const int off = (i < n && j < n) ? (zsrc ? sl * n + j : i * n + j) : 0;
+            old[q] = gload2c((zsrc ? Zg : A) + off);
         }
+    };
+    load_z(tj0 * 64);
+    load_old(tj0 * 64);
+    for (int tl = 0; tl < ntl; ++tl) {
+        const int j0 = (tj0 + tl) * 64;
+        const int jz = j0 + 16 * wv + lr;
+        __syncthreads();  // the previous tile's product reads are done (and the V split, first time)
+        {
 #pragma unroll
-    for (int ch = 0; ch < 2; ++ch) {
-        if (ch < nchunk) {
-            h8v a[2][4], b[2][4];
+            for (int ch = 0; ch < 2; ++ch) {
+                const int nbc = (ch == 0) ? nb1 : nb2;
+                // Z1' : Z1 with the K2 columns zeroed
+                const bool okz = ch < nchunk && jz < n && !(fused && ch == 0 && jz >= k2 && jz < kend);
+                float m = 0.f;
 #pragma unroll
-            for (int p = 0; p < 4; ++p) {
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) a[bj][p] = sOp[1][ch][p][wc * 2 + bj][lane];
-#pragma unroll
-                for (int bi = 0; bi < 2; ++bi) b[bi][p] = sOp[0][ch][p][wr * 2 + bi][lane];
-            }
-            f4v cre[2][2], cim[2][2], cii[2][2];
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    cre[bi][bj] = f4v{0, 0, 0, 0};
-                    cim[bi][bj] = f4v{0, 0, 0, 0};
-                    cii[bi][bj] = f4v{0, 0, 0, 0};
+                for (int q = 0; q < 8; ++q) {
+                    const bool ok = okz && 8 * lk + q < nbc;
+                    pz[ch][q].x = ok ? pz[ch][q].x : 0.f;
+                    pz[ch][q].y = ok ? pz[ch][q].y : 0.f;
+                    m = fmaxf(m, fmaxf(fabsf(pz[ch][q].x), fabsf(pz[ch][q].y)));
                 }
-            // terms outermost (hi hi, hi lo, lo hi), the four products over
-            // the four blocks inside, so dependent MFMAs are 8-16 apart
+                m = fmaxf(m, __shfl_xor(m, 16, 64));
+                m = fmaxf(m, __shfl_xor(m, 32, 64));
+                const int e = split_exp(m);
+                const float sc = exp2i(e);
+                if (lk == 0) eZ[ch][16 * wv + lr] = e;
+                h8v rh, rl, ih, il;
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const int xa = (r == 2) ? 1 : 0, yb = (r == 1) ? 1 : 0;
+                for (int q = 0; q < 8; ++q) {
+                    _Float16 h, l;
+                    split_f16(pz[ch][q].x * sc, h, l);
+                    rh[q] = h;
+                    rl[q] = l;
+                    split_f16(pz[ch][q].y * sc, h, l);
+                    ih[q] = h;
+                    il[q] = l;
+                }
+                sZ[ch][0][wv][lane] = rh;
+                sZ[ch][1][wv][lane] = rl;
+                sZ[ch][2][wv][lane] = ih;
+                sZ[ch][3][wv][lane] = il;
+            }
+        }
+        // the next tile's Z columns load while this tile is computed
+        if (tl + 1 < ntl) load_z(j0 + 64);
+        __syncthreads();
+        // per chunk: re = Zr Vr + Zi (-Vi) and im = Zr Vi + Zi Vr accumulated
+        // on MFMA (-Vi: the sign bits of the Vi fragments flipped), then
+        // unscaled by the chunk's exponents into FP32 totals
+        f4v tre[2][2], tim[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                tre[bi][bj] = f4v{0, 0, 0, 0};
+                tim[bi][bj] = f4v{0, 0, 0, 0};
+            }
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch) {
+            if (ch < nchunk) {
+                f4v cre[2][2], cim[2][2];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        cre[bi][bj] = f4v{0, 0, 0, 0};
+                        cim[bi][bj] = f4v{0, 0, 0, 0};
+                    }
+                // terms outermost (hi hi, hi lo, lo hi): each term reads its
+                // own fragments (re and im planes of the hi or lo part)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const int xa = (r == 2) ? 1 : 0, yb = (r == 1) ? 1 : 0;
+                    __builtin_amdgcn_sched_barrier(0);  // keep each term's reads beside its products
+                    h8v a[2][2], b[2][3];  // [block][re, im]; b[.][2] = -Vi
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        a[bj][0] = sZ[ch][xa][wc * 2 + bj][lane];
+                        a[bj][1] = sZ[ch][2 + xa][wc * 2 + bj][lane];
+                    }
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi) {
+                        b[bi][0] = sV[ch][yb][wr * 2 + bi][lane];
+                        b[bi][1] = sV[ch][2 + yb][wr * 2 + bi][lane];
+                        b[bi][2] = -b[bi][1];
+                    }
 #define H3_MM(ACC, XP, YP)                                                                          \
     _Pragma("unroll") for (int bi = 0; bi < 2; ++bi) _Pragma("unroll") for (int bj = 0; bj < 2; ++bj) \
-        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[bj][XP + xa], b[bi][YP + yb], ACC[bi][bj], 0, 0, 0);
-                H3_MM(cre, 0, 0)  // Zr Vr
-                H3_MM(cim, 0, 2)  // Zr Vi
-                H3_MM(cii, 2, 2)  // Zi Vi
-                H3_MM(cim, 2, 0)  // Zi Vr
-            }
+        ACC[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[bj][XP], b[bi][YP], ACC[bi][bj], 0, 0, 0);
+                    H3_MM(cre, 0, 0)  // Zr Vr
+                    H3_MM(cim, 0, 1)  // Zr Vi
+                    H3_MM(cre, 1, 2)  // Zi (-Vi)
+                    H3_MM(cim, 1, 0)  // Zi Vr
 #undef H3_MM
+                }
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi) {
-                const int ei = eV[ch][wr * 32 + bi * 16 + lr];
+                for (int bi = 0; bi < 2; ++bi) {
+                    const int ei = eV[ch][wr * 32 + bi * 16 + lr];
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
+                    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const float sc = exp2i(-(ei + eZ[ch][wc * 32 + bj * 16 + 4 * lk + v]));
-                        tre[bi][bj][v] += (cre[bi][bj][v] - cii[bi][bj][v]) * sc;
-                        tim[bi][bj][v] += cim[bi][bj][v] * sc;
-                    }
+                        for (int v = 0; v < 4; ++v) {
+                            const float sc = exp2i(-(ei + eZ[ch][wc * 32 + bj * 16 + 4 * lk + v]));
+                            tre[bi][bj][v] += cre[bi][bj][v] * sc;
+                            tim[bi][bj][v] += cim[bi][bj][v] * sc;
+                        }
+                }
             }
         }
-    }
-    __syncthreads();  // every wave is done with the operand planes
-    C* sP = (C*)&sOp[0][0][0][0][0];
+        __syncthreads();  // every wave is done with the Z planes
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const int o = (wr * 32 + bi * 16 + lr) * H3_PROW + wc * 32 + bj * 16 + 4 * lk;
-            *(f4v*)(sP + o) = f4v{tre[bi][bj][0], tim[bi][bj][0], tre[bi][bj][1], tim[bi][bj][1]};
-            *(f4v*)(sP + o + 2) = f4v{tre[bi][bj][2], tim[bi][bj][2], tre[bi][bj][3], tim[bi][bj][3]};
-        }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int r = (t >> 5) + 8 * q, i = i0 + r, j = j0 + rc;
-        if (i < n && j < n) {
-            const int sl = rsl[q];
-            const bool p1 = fused && sl >= 0 && sl < PMAX;
-            // rows replaced by Z: the panel's pivots (single) or the second
-            // panel's pivots (fused)
-            const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
-            const f4v pr = *(const f4v*)(sP + r * H3_PROW + rc);
-            f4v o = old[q];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int jj = j + h;
-                const bool ex = p1 ? (jj >= k2 && jj < kend) : (jj >= k0 && jj < kend);
-                const float orr = ex ? 0.f : old[q][2 * h], oii = ex ? 0.f : old[q][2 * h + 1];
-                o[2 * h] = zrow ? old[q][2 * h] : orr - pr[2 * h];
-                o[2 * h + 1] = zrow ? old[q][2 * h + 1] : oii - pr[2 * h + 1];
+            for (int bj = 0; bj < 2; ++bj) {
+                const int o = (wr * 32 + bi * 16 + lr) * H3_PROW + wc * 32 + bj * 16 + 4 * lk;
+                *(f4v*)(sP + o) = f4v{tre[bi][bj][0], tim[bi][bj][0], tre[bi][bj][1], tim[bi][bj][1]};
+                *(f4v*)(sP + o + 2) = f4v{tre[bi][bj][2], tim[bi][bj][2], tre[bi][bj][3], tim[bi][bj][3]};
             }
-            gstore2c(A + (long)i * n + j, o);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int r = (t >> 5) + 8 * q, i = i0 + r, j = j0 + rc;
+            if (i < n && j < n) {
+                const int sl = rsl[q];
+                const bool p1 = fused && sl >= 0 && sl < PMAX;
+                // rows replaced by Z: the panel's pivots (single) or the
+                // second panel's pivots (fused)
+                const bool zrow = fused ? (sl >= PMAX && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+                const f4v pp = *(const f4v*)(sP + r * H3_PROW + rc);
+                f4v o = old[q];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int jj = j + h;
+                    const bool ex = p1 ? (jj >= k2 && jj < kend) : (jj >= k0 && jj < kend);
+                    const float orr = ex ? 0.f : old[q][2 * h], oii = ex ? 0.f : old[q][2 * h + 1];
+                    o[2 * h] = zrow ? old[q][2 * h] : orr - pp[2 * h];
+                    o[2 * h + 1] = zrow ? old[q][2 * h + 1] : oii - pp[2 * h + 1];
+                }
+                gstore2c(A + (i * n + j), o);
+            }
         }
+        if (tl + 1 < ntl) load_old(j0 + 64);
     }
 }
 
 void launch_trailing_h3(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
                         int nb2) {
-    const int ntx = (n + 63) / 64;
-    const int total = ntx * ntx * njobs;
-    k_trailing_h3<<<total, 256, 0, s>>>(jobs_d, k0, nb1, nb2, ntx, total);
+    const int nts = (n + 63) / 64, ngrp = (nts + H3_TPW - 1) / H3_TPW;
+    const int total = nts * ngrp * njobs;
+    k_trailing_h3s<H3_TPW><<<total, 256, 0, s>>>(jobs_d, k0, nb1, nb2, nts, ngrp, total);
 }
+
 
 template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
