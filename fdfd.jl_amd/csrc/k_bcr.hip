@@ -110,6 +110,8 @@ void launch_trifactor(hipStream_t s, const TriJob* jobs_d, int njobs, int n) {
     k_trifactor<<<(njobs + 63) / 64, 64, 0, s>>>(jobs_d, njobs, n);
 }
 
+constexpr int TRI_U = 8;  // back-substitution steps whose operands are loaded together
+
 template <typename R>
 __global__ __launch_bounds__(64) void k_triinv(const TriJob* jobs, int n) {
     using C = cplx<R>;
@@ -126,7 +128,52 @@ __global__ __launch_bounds__(64) void k_triinv(const TriJob* jobs, int n) {
         return tri::mkz((double)y.x, (double)y.y);
     };
     tri::apply_q(n, w, b, kstart, put);
-    tri::apply_rinv(n, w, b, get, put);
+    // x = R^-1 y: the arithmetic of tri::apply_rinv, with the y[k] of TRI_U
+    // steps loaded before the steps run (y[k] is overwritten only at step k)
+    // and the rows of R of those steps, the same for every column of the
+    // line, staged in LDS by the wave (one wave per workgroup), so the back
+    // substitution does not wait on each load
+    using tri::cz;
+    __shared__ double rsh[TRI_U * tri::RROW];
+    const int lane = threadIdx.x;  // b0 + lane = b < n
+    const int k0 = tri::first_nonzero(n, b);
+    auto y = [&](int k, cz v) { return k >= k0 ? v : tri::mkz(0, 0); };
+    const cz xB = tri::zmul(y(n - 1, get(n - 1)), tri::get_r(w, n, n - 1, 0));
+    put(n - 1, xB);
+    if (n > 1) {
+        const cz xA = tri::zmul(tri::zsub(y(n - 2, get(n - 2)), tri::zmul(tri::get_r(w, n, n - 2, 4), xB)),
+                                tri::get_r(w, n, n - 2, 0));
+        put(n - 2, xA);
+        cz x1 = xA, x2 = xB;  // x[k+1], x[k+2]
+        const double* wr = w + tri::ROT * n;
+        // lanes past n have returned: the nact active lanes stage the rows
+        const int nact = min(64, n - b0);
+        for (int kt = n - 3; kt >= 0; kt -= TRI_U) {
+            cz yy[TRI_U];
+#pragma unroll
+            for (int u = 0; u < TRI_U; ++u) yy[u] = get(max(kt - u, 0));
+            for (int e = lane; e < TRI_U * tri::RROW; e += nact)
+                rsh[e] = gload_s(wr + (long)tri::RROW * max(kt - e / tri::RROW, 0) + e % tri::RROW);
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < TRI_U; ++u) {
+                const int k = kt - u;
+                if (k >= 0) {
+                    const double* r = rsh + tri::RROW * u;
+                    cz v = y(k, yy[u]);
+                    v = tri::zsub(v, tri::zmul(tri::mkz(r[2], r[3]), x1));
+                    v = tri::zsub(v, tri::zmul(tri::mkz(r[4], r[5]), x2));
+                    v = tri::zsub(v, tri::zmul(tri::mkz(r[6], r[7]), xA));
+                    v = tri::zsub(v, tri::zmul(tri::mkz(r[8], r[9]), xB));
+                    const cz x = tri::zmul(v, tri::mkz(r[0], r[1]));
+                    put(k, x);
+                    x2 = x1;
+                    x1 = x;
+                }
+            }
+            __syncthreads();  // the rows are read before the next batch is staged
+        }
+    }
     J.prow[b] = b;
     J.iprow[b] = b;
 }
