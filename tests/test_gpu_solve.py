@@ -143,11 +143,12 @@ def test_fp16x3_gemm_tracks_fp32_gemm(monkeypatch, Nx, Ny, npml):
     assert its["fp16x3"] <= its["fp32"] + 4
 
 
-@pytest.mark.parametrize("Nx,Ny,npml", [(100, 84, 6), (76, 120, 6)])
+@pytest.mark.parametrize("Nx,Ny,npml", [(100, 84, 6), (76, 120, 6), (300, 264, 8)])
 def test_fp16x3_trailing_tracks_fp32_trailing(monkeypatch, Nx, Ny, npml):
     """The Gauss-Jordan trailing update on fp16x3 products (line lengths
-    n % 4 == 0; here 84 and 76, not multiples of the 64 x 64 tiles, with a
-    last panel pair shorter than 64 columns) needs about as many refinement
+    n % 4 == 0; here 84, 76 and 264, not multiples of the 64 x 64 tiles, with
+    a last panel pair shorter than 64 columns; at 264 a 64-row strip spans
+    two workgroups of 4 and 1 tiles) needs about as many refinement
     iterations as the FP32 one and reaches the same residual."""
     d = _device(Nx, Ny, npml, seed=7 + Nx)
     om = float(d.omega[0])
