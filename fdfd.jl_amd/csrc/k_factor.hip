@@ -650,28 +650,16 @@ __global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int n
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ C Z1k[PMAX][PMAX + 1];  // Z1[q][K2 column c]
+    __shared__ C Z1k[PMAX][PMAX + 1];        // Z1[q][K2 column c]
+    __shared__ __align__(16) C V1s[32][PMAX + 2];  // V1 of the workgroup's 32 rows
     const C* Zg = (const C*)J.Z;
     const int k2 = k1 + nb1;
     constexpr int EPT = PMAX * PMAX / 256;  // blockDim.x == 256
-    {
-        C v[EPT];
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, r = e / PMAX, c = e % PMAX;
-            v[q] = gload(Zg + ((r < nb1 && c < nb2) ? (long)r * n + k2 + c : 0));
-        }
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, r = e / PMAX, c = e % PMAX;
-            Z1k[r][c] = csel(r < nb1 && c < nb2, v[q], mk<C>(R(0), R(0)));
-        }
-    }
-    __syncthreads();
     const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
     const int rbase = blockIdx.x * 32;
-    // the four rows of this half-wave (rows half + 8 i): slots, then the row
-    // loads, all in flight together; out-of-range lanes read entry 0
+    // the four rows of this half-wave (rows half + 8 i): slots and row loads
+    // are issued first, so their latency overlaps the Z1 staging below;
+    // out-of-range lanes read entry 0
     const C* Y = (const C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
     int slot[4];
@@ -687,17 +675,45 @@ __global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int n
         wv[i] = gload(Y + (r < n && c < nb2 ? (long)r * n + k2 + c : 0));
         v1v[i] = gload(Vb + (r < n && c < nb1 ? (long)r * VMAX + c : 0));
     }
+    {
+        C v[EPT];
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / PMAX, cc = e % PMAX;
+            v[q] = gload(Zg + ((r < nb1 && cc < nb2) ? (long)r * n + k2 + cc : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = threadIdx.x + 256 * q, r = e / PMAX, cc = e % PMAX;
+            Z1k[r][cc] = csel(r < nb1 && cc < nb2, v[q], mk<C>(R(0), R(0)));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        V1s[half + 8 * i][c] = csel(c < nb1, v1v[i], mk<C>(R(0), R(0)));
+    __syncthreads();
+    // this lane's column of Z1 in registers, reused for its four rows; the
+    // rows of V1 are read from LDS two entries at a time (one address per
+    // half-wave: a broadcast), so a step costs half an LDS read
+    C z[PMAX];
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) z[q] = Z1k[q][c];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int r = rbase + half + 8 * i;  // the same for the 32 lanes of a half-wave
         if (r >= n) break;
         C w;
         if (slot[i] >= 0 && slot[i] < nb1) {
-            w = Z1k[slot[i]][c];
+            w = Z1k[slot[i]][c];  // (a dynamic index into z would put z on the stack)
         } else {
             w = (c < nb2) ? wv[i] : mk<C>(R(0), R(0));
-            const C v1 = (c < nb1) ? v1v[i] : mk<C>(R(0), R(0));
-            for (int q = 0; q < nb1; ++q) w = csub(w, cmul(shfl_half(v1, q), Z1k[q][c]));
+            const C* vr = &V1s[half + 8 * i][0];
+#pragma unroll
+            for (int q = 0; q < PMAX; q += 2) {
+                const C va = vr[q], vb = vr[q + 1];  // zero past nb1
+                w = csub(w, cmul(va, z[q]));
+                w = csub(w, cmul(vb, z[q + 1]));
+            }
         }
         if (c < nb2) gstore((C*)J.Vbuf + (long)r * VMAX + PMAX + c, w);
     }
