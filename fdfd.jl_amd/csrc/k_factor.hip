@@ -454,6 +454,21 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     const bool zpart = (int)blockIdx.x < nzb;
     const int poff = second ? PMAX : 0;
     constexpr int EPT = PMAX * PMAX / 256;  // factor entries per thread (blockDim.x == 256)
+    // check blocks: the V tile (rows rbase.., panel columns), 32 lanes per
+    // row, issued before the factor staging so the two latencies overlap;
+    // out-of-range lanes read the first panel entry and are zeroed
+    const int vc = t & 31, h8 = t >> 5;
+    const int rbase = ((int)blockIdx.x - nzb) * 32;
+    C vrow[4];
+    if (!zpart) {
+        const C* vsrc = second ? (const C*)J.Vbuf + poff : (const C*)J.Y + k0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = rbase + h8 + 8 * i;
+            const bool ok = r < n && vc < nb;
+            vrow[i] = gload(vsrc + (ok ? (second ? (long)r * VMAX : (long)r * n) + vc : 0));
+        }
+    }
     {
         // M or U^-1 and the pivot rows: every load issued before the first use
         const C* Fg = (const C*)J.LU + (zpart ? 0 : PMAX * PMAX);
@@ -573,18 +588,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
                 if (row < nb) gstore(Zg + (long)(poff + row) * n + col, mk<C>(zre[ro][v], zim[ro][v]));
             }
     } else {
-        const int c = t & 31, h8 = t >> 5;
-        const int rbase = ((int)blockIdx.x - nzb) * 32;
-        // the V tile (rows rbase.., panel columns), 32 lanes per row, loads
-        // first; out-of-range lanes read the first panel entry and are zeroed
-        const C* vsrc = second ? Vb + poff : A + k0;
-        C vrow[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = rbase + h8 + 8 * i;
-            const bool ok = r < n && c < nb;
-            vrow[i] = gload(vsrc + (ok ? (second ? (long)r * VMAX : (long)r * n) + c : 0));
-        }
+        const int c = vc;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = rbase + h8 + 8 * i;
