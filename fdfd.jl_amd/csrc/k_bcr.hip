@@ -723,6 +723,7 @@ void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n) {
 // A B: the terms of a sum carry their own row scales, so the host issues a
 // sum of two as two launches, the second accumulating into the first.
 constexpr int B3_TM = 128, B3_TN = 128;
+constexpr int EP_BLK = 4;  // row blocks whose old values the epilogue loads together (1, 2: same or slower)
 
 __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int ntx, int total) {
     __shared__ __align__(16) h8v sA[2][4][8][64];  // [buffer][plane][row block][lane]
@@ -774,15 +775,17 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
             }
     };
     float2* out = (float2*)J.out;
-    // add term ts (unscaled) to the output tile, one output row at a time (a
-    // scheduling barrier after each keeps the old values in flight, beside
-    // the live accumulators, to two per lane)
+    // add the term (unscaled) to the output tile: the 16 row exponents are
+    // loaded first, then one row block (4 rows) at a time, its old values
+    // all in flight before its stores (a scheduling barrier after each block
+    // keeps the old values of one block, beside the live accumulators, per
+    // lane)
     auto epilogue = [&]() {
         const int* ea = J.ea[0];
         const int* eb = J.eb[0];
         const bool rd = J.acc;
         const int jb = j0 + wc * 32 + lr;
-        int ej[2];
+        int ej[2], ei[4][4];
 #pragma unroll
         for (int q = 0; q < 2; ++q) ej[q] = gload_s(eb + (jb + 16 * q < n ? jb + 16 * q : 0));
 #pragma unroll
@@ -790,26 +793,53 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wr * 64 + p * 16 + 4 * lg + v;
-                if (i < n) {
-                    const int ei = gload_s(ea + i);
-                    float2* orow = out + (long)i * n + jb;
+                ei[p][v] = gload_s(ea + (i < n ? i : 0));
+            }
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        if (jb + 16 * q < n) {
-                            const float sc = exp2i(-(ei + ej[q]));
-                            float ox = 0.f, oy = 0.f;
-                            if (rd) {
-                                const float2 old = gload(orow + 16 * q);
-                                ox = old.x;
-                                oy = old.y;
+        for (int pp = 0; pp < 4; pp += EP_BLK) {
+            float2 old[EP_BLK][4][2];
+            if (rd) {  // uniform; clamped addresses, no select after a load
+#pragma unroll
+                for (int pb = 0; pb < EP_BLK; ++pb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = i0 + wr * 64 + (pp + pb) * 16 + 4 * lg + v;
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            const bool ok = i < n && jb + 16 * q < n;
+                            old[pb][v][q] = gload(out + (ok ? (long)i * n + jb + 16 * q : 0));
+                        }
+                    }
+            } else {
+#pragma unroll
+                for (int pb = 0; pb < EP_BLK; ++pb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) old[pb][v][q] = make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int pb = 0; pb < EP_BLK; ++pb) {
+                const int p = pp + pb;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wr * 64 + p * 16 + 4 * lg + v;
+                    if (i < n) {
+                        float2* orow = out + (long)i * n + jb;
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            if (jb + 16 * q < n) {
+                                const float sc = exp2i(-(ei[p][v] + ej[q]));
+                                const float re = (cre[p][q][v] - cii[p][q][v]) * sc, im = cim[p][q][v] * sc;
+                                const float2 o = old[pb][v][q];
+                                gstore(orow + 16 * q, J.sub ? make_float2(o.x - re, o.y - im) : make_float2(o.x + re, o.y + im));
                             }
-                            const float re = (cre[p][q][v] - cii[p][q][v]) * sc, im = cim[p][q][v] * sc;
-                            gstore(orow + 16 * q, J.sub ? make_float2(ox - re, oy - im) : make_float2(ox + re, oy + im));
                         }
                     }
                 }
-                __builtin_amdgcn_sched_barrier(0);
             }
+            __builtin_amdgcn_sched_barrier(0);
+        }
     };
     auto compute = [&](int cur) {
         h8v a[4][4], b[2][4];  // [block][plane]: re hi, re lo, im hi, im lo
