@@ -132,6 +132,10 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 Fs.push_back(F);
             }
             rc = rt_build_factors(h, Fs);
+            for (Factor* F : Fs) {
+                pivot_fallback |= F->pivot_fallback;
+                growth = std::max(growth, F->pivot_growth);
+            }
             if (rc == FDFD_OK) {
                 t_factor += Fs[0]->t_factor_ms;
                 hipEvent_t e0, e1;
@@ -141,8 +145,6 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 const double eps0n = EPS0 * g.L0, mu0n = MU0 * g.L0;
                 for (int q = 0; q < nb && rc == FDFD_OK; ++q) {
                     Factor* F = Fs[q];
-                    growth = std::max(growth, F->pivot_growth);
-                    pivot_fallback |= F->pivot_fallback;
                     for (int sidx = 0; sidx < job.nsrc && rc == FDFD_OK; ++sidx) {
                         // b' = scale * (i omega) * src   (driven.jl:36,46)
                         launch_scale_copy(h->stream, h->bvec, dsrc + (long)sidx * N,
@@ -172,7 +174,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         (void)hipFree(dsrc);
         if (rc) break;
     }
-    if (stats) {
+    if (stats) {  // also on failure: what was reached
         memset(stats, 0, sizeof(*stats));
         stats->iterations = iters;
         stats->residual = worst;

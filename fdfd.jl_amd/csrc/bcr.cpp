@@ -17,6 +17,8 @@
 // stays in block j); coupling and product blocks are drawn from a pool.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -119,6 +121,12 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
     const Plan P = make_plan(m);
     int rc;
+    const int inject = h->inject;  // one-shot: consumed by this factorization
+    if (inject == FDFD_INJECT_PIVOT_FALLBACK || inject == FDFD_INJECT_SINGULAR ||
+        inject == FDFD_INJECT_OOM)
+        h->inject = 0;
+    if (inject == FDFD_INJECT_OOM)
+        return fail(h, FDFD_ERR_OOM, "factor storage allocation failed (injected)");
     if ((rc = ensure_factor_work(h, n, std::max(1, P.max_gj * B)))) return rc;
     if ((rc = ensure_solve_work(h, N))) return rc;
     if (trisolve_prepare(n) != (int)hipSuccess)
@@ -423,14 +431,16 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         HIPCHK(h, hipEventSynchronize(e1));
         int verr = 0;
         COPY(h, &verr, h->verr, sizeof(int), hipMemcpyDeviceToHost);
+        if (pass == 0 && inject == FDFD_INJECT_PIVOT_FALLBACK) verr = 1;
         if (full || !verr) break;
     }
     float ms = 0;
     HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    int first_singular = -1;
-    for (Factor* F : Fs) {
+    int first_singular = -1, singular_factor = -1, nsingular = 0;
+    for (int fi = 0; fi < B; ++fi) {
+        Factor* F = Fs[fi];
         F->t_factor_ms = ms;
         F->pivot_fallback = used_full;
         std::vector<double> ps(3 * (size_t)m);
@@ -443,10 +453,29 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             if (ps[3 * j + 2] != 0 && F->singular_line < 0) F->singular_line = j;
         }
         F->pivot_growth = (mn > 0) ? mx / mn : std::numeric_limits<double>::infinity();
-        if (F->singular_line >= 0 && first_singular < 0) first_singular = F->singular_line;
+        const char* dbg = std::getenv("FDFD_DEBUG_PIVSTAT");
+        if (dbg && (F->singular_line >= 0 || dbg[0] == '2')) {
+            for (int j = 0; j < m; ++j)
+                fprintf(stderr, "pivstat factor %d line %d: min %.3e max %.3e flag %g\n", fi, j,
+                        ps[3 * j], ps[3 * j + 1], ps[3 * j + 2]);
+        }
+        if (inject == FDFD_INJECT_SINGULAR && F == Fs[0])
+            F->singular_line = std::min(std::max(h->inject_arg, 0), m - 1);
+        if (F->singular_line >= 0) ++nsingular;
+        if (F->singular_line >= 0 && first_singular < 0) {
+            first_singular = F->singular_line;
+            singular_factor = fi;
+        }
     }
-    if (first_singular >= 0)
-        return fail(h, FDFD_ERR_SINGULAR, "zero pivot in line block " + std::to_string(first_singular));
+    if (first_singular >= 0) {
+        std::string msg = "zero pivot in line block " + std::to_string(first_singular);
+        if (B > 1)
+            msg += " of factor " + std::to_string(singular_factor) + " of " + std::to_string(B) +
+                   " built together (omega " + std::to_string(Fs[singular_factor]->omega) +
+                   "; " + std::to_string(nsingular) + " factors with a zero pivot)";
+        if (used_full) msg += " after the full-column pivot search";
+        return fail(h, FDFD_ERR_SINGULAR, msg);
+    }
 
     // --- reduced-solve program of each factor
     for (Factor* F : Fs) {

@@ -48,6 +48,10 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
         return fail(h, FDFD_ERR_STATE, "grid and eps must be set before fdfd_eigenfrequency");
     if (pol != FDFD_TM && pol != FDFD_TE) return fail(h, FDFD_ERR_INVALID, "pol must be TM or TE");
     if (!(omega0 > 0)) return fail(h, FDFD_ERR_INVALID, "omega0 must be > 0");
+    if (precision != FDFD_FP64 && precision != FDFD_FP32_IR)
+        return fail(h, FDFD_ERR_INVALID, "precision must be FDFD_FP64 or FDFD_FP32_IR");
+    if (which != FDFD_WHICH_LM && which != FDFD_WHICH_LR)
+        return fail(h, FDFD_ERR_INVALID, "which must be FDFD_WHICH_LM or FDFD_WHICH_LR");
     const GridDesc& g = h->g;
     const long N = (long)g.Nx * g.Ny;
     if (ncv <= 0) ncv = std::max(20, 2 * nev + 1);  // Arpack.jl default
@@ -71,13 +75,20 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
     const double inner_tol = std::max(1e-13, std::min(1e-12, tol * 1e-1));
 
     int iters = 0;
-    // OP y = x
+    double op_err = 0;  // largest relative residual of the OP applications
+    // OP y = x.  With an eigenvalue very close to the shift, A - sigma is nearly
+    // singular and FP64 rounding floors the solve's residual above inner_tol
+    // (about |A| eps / |lambda - sigma|); a solve that stagnates below 1e-8 is
+    // accepted and Arnoldi's convergence test follows its error.
     auto op = [&](const double2* x, double2* y) -> int {
         if (pol == FDFD_TM) launch_mul_eps(h->stream, h->tmp, x, h->eps, make_double2(opscale, 0), N);
         else launch_scale_copy(h->stream, h->tmp, x, make_double2(opscale, 0), N);
         launch_zero(h->stream, y, N);
         double res = 0;
-        return rt_gmres(h, F, 0, h->tmp, y, inner_tol, &iters, &res);
+        int r = rt_gmres(h, F, 0, h->tmp, y, inner_tol, &iters, &res);
+        if (r == FDFD_ERR_NOCONV && res <= 1e-8) r = FDFD_OK;
+        op_err = std::max(op_err, res);
+        return r;
     };
     double2* ritz = nullptr;
     if (fields && hipMalloc(&ritz, sizeof(double2) * N * nev) != hipSuccess) {
@@ -90,7 +101,7 @@ extern "C" int fdfd_eigenfrequency(fdfd_handle* hh, int pol, double omega0, int 
     };
     std::vector<cd> theta;
     int nconv = 0, restarts = 0;
-    rc = rt_arnoldi(h, N, nev, ncv, which, tol, maxiter, op, ritz, theta, &nconv, &restarts, nullptr);
+    rc = rt_arnoldi(h, N, nev, ncv, which, tol, maxiter, op, ritz, theta, &nconv, &restarts, &op_err);
     if (rc) { cleanup(); return rc; }
     std::vector<cd> lam(nev), om(nev);
     for (int q = 0; q < nev; ++q) {

@@ -52,9 +52,14 @@ template <typename C> __host__ __device__ inline C crcp(C z) {
     }
 }
 template <typename C> __host__ __device__ inline C cdiv(C a, C b) { return cmul(a, crcp(b)); }
-// LAPACK-style pivot magnitude |re| + |im| (izamax / cabs1)
+// LAPACK-style pivot magnitude |re| + |im| (izamax / cabs1).  The sign bits
+// are cleared, so a signed zero entry has magnitude +0 (a comparison-based
+// |x| returns -0 for -0, whose bit pattern would order above every positive
+// magnitude in an integer argmax).
+__host__ __device__ inline float rabs(float x) { return __builtin_fabsf(x); }
+__host__ __device__ inline double rabs(double x) { return __builtin_fabs(x); }
 template <typename C> __host__ __device__ inline decltype(C{}.x) cabs1(C z) {
-    return (z.x < 0 ? -z.x : z.x) + (z.y < 0 ? -z.y : z.y);
+    return rabs(z.x) + rabs(z.y);
 }
 template <typename C> __host__ __device__ inline decltype(C{}.x) cnorm2(C z) { return z.x * z.x + z.y * z.y; }
 
@@ -285,6 +290,8 @@ void launch_mul_eps(hipStream_t s, double2* y, const double2* x, const double2* 
                     double2 a, long n);
 void launch_grad(hipStream_t s, double* grad, const double2* e, const double2* lam,
                  double2 coef, long n);
+void launch_field_adjoint(hipStream_t s, double2* out, const double2* dfield, const double2* sxb,
+                          const double2* syb, const GridDesc& g, double2 c, double scale);
 
 template <typename R>
 void launch_panel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb);

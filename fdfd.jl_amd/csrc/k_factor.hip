@@ -62,8 +62,14 @@ __device__ inline unsigned long long wave_max_u64(unsigned long long k) {
     return c > a ? c : a;
 }
 // monotone key of a non-negative magnitude
-__device__ inline unsigned long long mag_key(double m) { return (unsigned long long)__double_as_longlong(m); }
-__device__ inline unsigned long long mag_key(float m) { return (unsigned long long)(unsigned)__float_as_int(m); }
+// (sign bit cleared: -0 orders as +0; a NaN orders above +inf, so a NaN
+// candidate is picked and flagged rather than hidden)
+__device__ inline unsigned long long mag_key(double m) {
+    return (unsigned long long)__double_as_longlong(m) & 0x7fffffffffffffffULL;
+}
+__device__ inline unsigned long long mag_key(float m) {
+    return (unsigned long long)((unsigned)__float_as_int(m) & 0x7fffffffu);
+}
 
 // lane k of this lane's half-wave (lanes 0-31 or 32-63), k wave-uniform
 template <typename C> __device__ inline C bcast_half(C v, int k) {
@@ -273,14 +279,21 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     }
     if (lane == 0) {
         double mn = J.pivstat[0], mx = J.pivstat[1];
+        bool zero = false;
         for (int c = 0; c < nb; ++c) {
             double v = (double)pmags[c];
             mn = v < mn ? v : mn;
             mx = v > mx ? v : mx;
-            if (!(v > 0)) J.pivstat[2] = 1.0;
+            if (!(v > 0)) zero = true;
         }
         J.pivstat[0] = mn;
         J.pivstat[1] = mx;
+        // a zero (or NaN) pivot in the window is not final: the full-column
+        // search redoes the factorization and decides
+        if (zero) {
+            J.pivstat[2] = 1.0;
+            *(GLOBAL_AS int*)J.verr = 1;
+        }
     }
     // new window: surviving rows, then the next nb fresh rows
     const bool keep = (row >= 0) && mystep < 0;
@@ -621,7 +634,8 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
             for (int v = 0; v < 4; ++v) {
                 const int r = rbase + rb * 16 + M::row(lane, v);
                 const C ac = cmul(mk<C>(yre[v], yim[v]), ucc);
-                if (r < n && r >= fresh && R(PIV_U) * cabs1(ac) > cabs1(ucc)) bad = true;
+                // (written so that a NaN, from a singular pivot block, also fails)
+                if (r < n && r >= fresh && !(R(PIV_U) * cabs1(ac) <= cabs1(ucc))) bad = true;
             }
         }
         if (bad) *(GLOBAL_AS int*)J.verr = 1;

@@ -607,4 +607,41 @@ void launch_grad(hipStream_t s, double* grad, const double2* e, const double2* l
     k_grad<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(grad, e, lam, coef, n);
 }
 
+// Right-hand side of the adjoint solve of a TM driven solve from the
+// cotangent (bar Ez, bar Hx, bar Hy) of its fields (3 x N, x fastest).  The H
+// recovery Hx = -c Syb dyb Ez, Hy = c Sxb dxb Ez (driven.jl:40-41,
+// c = 1/(i w mu0')) is linear in Ez, so the cotangent reaching Ez is
+//   bar e = bar Ez + Mx^H bar Hx + My^H bar Hy,
+//   Mx^H = -c dyf conj(Syb),  My^H = c dxf conj(Sxb)
+// (conj(c) = -c and dyb^T = -dyf for the periodic differences).  The adjoint
+// solve needs A^T lam = conj(bar e); out = scale * conj(bar e), the scaled
+// operator's right-hand side.
+__global__ void k_field_adjoint(double2* out, const double2* dfield, const double2* sxb,
+                                const double2* syb, int Nx, int Ny, double rdx, double rdy,
+                                double2 c, double scale) {
+    const long N = (long)Nx * Ny;
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = (int)(k % Nx), j = (int)(k / Nx);
+    const long ke = (i + 1 == Nx) ? k - (Nx - 1) : k + 1;
+    const long kn = (j + 1 == Ny) ? k - (N - Nx) : k + Nx;
+    const int ie = (i + 1 == Nx) ? 0 : i + 1;
+    const int jn = (j + 1 == Ny) ? 0 : j + 1;
+    const double2* ez = dfield;
+    const double2* hx = dfield + N;
+    const double2* hy = dfield + 2 * N;
+    // dyf(conj(syb) hx) and dxf(conj(sxb) hy) at k
+    const double2 ty = csub(cconjmul(syb[jn], hx[kn]), cconjmul(syb[j], hx[k]));
+    const double2 tx = csub(cconjmul(sxb[ie], hy[ke]), cconjmul(sxb[i], hy[k]));
+    const double2 d = mk<double2>(tx.x * rdx - ty.x * rdy, tx.y * rdx - ty.y * rdy);
+    const double2 e = cadd(ez[k], cmul(c, d));
+    out[k] = mk<double2>(scale * e.x, -scale * e.y);
+}
+void launch_field_adjoint(hipStream_t s, double2* out, const double2* dfield, const double2* sxb,
+                          const double2* syb, const GridDesc& g, double2 c, double scale) {
+    const long N = (long)g.Nx * g.Ny;
+    k_field_adjoint<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(out, dfield, sxb, syb, g.Nx, g.Ny,
+                                                                1.0 / g.dx, 1.0 / g.dy, c, scale);
+}
+
 }  // namespace fdfd

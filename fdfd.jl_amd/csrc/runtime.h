@@ -87,6 +87,8 @@ struct Handle {
     int *used = nullptr, *piv = nullptr, *win = nullptr, *verr = nullptr;
     void *Vbuf = nullptr, *LU = nullptr, *Zb = nullptr;
     bool full_pivot_search = false;
+    // one-shot failure injection (fdfd_debug_inject): FDFD_INJECT_* and its argument
+    int inject = 0, inject_arg = 0;
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
     int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)

@@ -14,6 +14,8 @@ LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "lib", "libfdfd.so"))
 FDFD_TM, FDFD_TE = 1, 2
 FDFD_FP64, FDFD_FP32_IR = 0, 1
 FDFD_WHICH_LM, FDFD_WHICH_LR = 0, 1
+(FDFD_INJECT_NONE, FDFD_INJECT_PIVOT_FALLBACK, FDFD_INJECT_SINGULAR, FDFD_INJECT_NOCONV,
+ FDFD_INJECT_OOM) = range(5)
 
 STATUS = {
     0: "FDFD_OK", -1: "FDFD_ERR_INVALID", -2: "FDFD_ERR_OOM", -3: "FDFD_ERR_SINGULAR",
@@ -94,6 +96,9 @@ PROTOTYPES = {
     "fdfd_profile_enable": (_I, [_P, _I]),
     "fdfd_profile_read": (_I, [_P, _I, _PD, C.POINTER(C.c_longlong), _PD, _PD]),
     "fdfd_set_tuning": (_I, [_P, _I, _I]),
+    "fdfd_debug_inject": (_I, [_P, _I, _I]),
+    "fdfd_solve_pullback": (_I, [_P, _I, _P, _P, _D, _P, _PS]),
+    "fdfd_solve_pullback_d": (_I, [_P, _I, _P, _P, _D, _P, _PS]),
 }
 
 _lib = None

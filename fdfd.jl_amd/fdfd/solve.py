@@ -140,10 +140,13 @@ class Engine:
             fl = fl.reshape(nev, 3, g.N[1], g.N[0]).transpose(0, 3, 2, 1)
         return om, fl, st.as_dict()
 
-    def batch_solve(self, jobs, precision="mixed", tol: float = 1e-10, keep: bool = True):
+    def batch_solve(self, jobs, precision="mixed", tol: float = 1e-10, keep: bool = True,
+                    fetch=None):
         """Batched driven solves.  jobs: list of (grid, pol, eps_r, omegas,
         srcs[Nx, Ny, nsrc]).  Returns (results, stats) where results[j] is an
-        array [nomega, nsrc, Nx, Ny, 3] (or None when keep=False).
+        array [nomega, nsrc, Nx, Ny, 3] (or None when keep=False).  With
+        fetch = [(job, iomega, isrc), ...] only those fields are copied back,
+        and results is a dict keyed by those triples.
         Resets the engine's grid (and releases its factors)."""
         b = C.c_void_p()
         self._chk(self.lib.fdfd_batch_create(self.h, C.byref(b)))
@@ -164,6 +167,15 @@ class Engine:
                 ids.append((jid.value, g, om.size, ss.shape[0]))
             st = fdfd_stats()
             self._chk(self.lib.fdfd_batch_run(b, PRECISIONS[precision], tol, C.byref(st)))
+            if fetch is not None:
+                sel = {}
+                for (j, o, r) in fetch:
+                    jid, g, _, _ = ids[j]
+                    buf = np.empty(3 * len(g), dtype=np.complex128)
+                    self._chk(self.lib.fdfd_batch_result(b, jid, o, r, _ptr(buf)))
+                    sel[(j, o, r)] = buf.reshape(3, g.N[1], g.N[0]).transpose(2, 1, 0)
+                self.grid = None
+                return sel, st.as_dict()
             out = []
             for jid, g, nom, ns in ids:
                 if not keep:
@@ -192,6 +204,10 @@ class Engine:
 
     def set_tuning(self, panel_width: int = 0, force_full_search: bool = False):
         self._chk(self.lib.fdfd_set_tuning(self.h, panel_width, int(force_full_search)))
+
+    def debug_inject(self, what: int, arg: int = 0):
+        """One-shot failure injection (fdfd_debug_inject) for error-path tests."""
+        self._chk(self.lib.fdfd_debug_inject(self.h, int(what), int(arg)))
 
 
 _default_engine = None

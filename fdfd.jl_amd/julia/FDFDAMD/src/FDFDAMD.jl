@@ -18,6 +18,7 @@ module FDFDAMD
 
 using FDFD
 using Libdl
+using ChainRulesCore
 
 const libfdfd = get(ENV, "FDFD_LIB",
                     normpath(joinpath(@__DIR__, "..", "..", "..", "lib", "libfdfd.so")))
@@ -254,6 +255,127 @@ function solve(d::FDFD.Device, pol::FDFD.Polarization=FDFD.TM; precision::Symbol
         push!(fields, pol == FDFD.TM ? FDFD.FieldTM(d.grid, ω, out) : FDFD.FieldTE(d.grid, ω, out))
     end
     return length(fields) == 1 ? fields[1] : fields
+end
+
+# ---- reverse-mode derivative of solve with respect to ϵᵣ (ChainRulesCore) --
+# The reference has no derivative.  The forward pass is `solve`, keeping every
+# frequency's factor alive until the pullback has run; the pullback is one
+# `fdfd_solve_pullback` per frequency: a transposed FGMRES solve on the same
+# factor (A^T λ = conj(ē), ē the cotangent reaching Ez through the H recovery
+# of driven.jl:40-41) and the gradient kernel dL/dϵᵣ[k] = -Re(λ_k ω² ϵ₀' Ez_k)
+# (∂A/∂ϵᵣ[k] = ω² ϵ₀' e_k e_kᵀ, driven.jl:21,35).  TM only; the source is
+# held fixed (mode sources whose slice crosses the design are not followed).
+
+mutable struct FactorSet
+    handle::Handle
+    ids::Vector{Cint}
+end
+
+function release!(fs::FactorSet)
+    for id in fs.ids
+        ccall((:fdfd_release_factor, libfdfd), Cint, (Ptr{Cvoid}, Cint), fs.handle.ptr, id)
+    end
+    empty!(fs.ids)
+    return nothing
+end
+
+function solve_keep(d::FDFD.Device, precision::Symbol, tol::Real, h::Handle)
+    isempty(d.modes) || throw(ArgumentError("the solve rrule needs an explicit source (d.modes is not empty)"))
+    (Nx, Ny) = set_problem!(h, d)
+    src = Array{ComplexF64}(d.src)
+    fs = FactorSet(h, Cint[])
+    finalizer(release!, fs)
+    fields = FDFD.FieldTM[]
+    for ω in d.ω
+        fid = Ref{Cint}(0)
+        st = Ref(Stats())
+        check(h, ccall((:fdfd_factor, libfdfd), Cint,
+                       (Ptr{Cvoid}, Cint, Cdouble, Cint, Ref{Cint}, Ref{Stats}),
+                       h.ptr, Int(FDFD.TM), ω, precision_code(precision), fid, st))
+        push!(fs.ids, fid[])
+        out = Array{ComplexF64}(undef, Nx, Ny, 3)
+        GC.@preserve src out check(h, ccall((:fdfd_solve, libfdfd), Cint,
+            (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Ptr{ComplexF64}, Cdouble, Ref{Stats}),
+            h.ptr, fid[], 1, src, out, tol, st))
+        push!(fields, FDFD.FieldTM(d.grid, ω, out))
+    end
+    return fields, fs
+end
+
+# cotangent of a FieldTM: a Tangent with a `data` field, the data array
+# itself, or nothing / ZeroTangent
+_field_bar(Δ, Nx, Ny) = Δ isa AbstractZero || Δ === nothing ? nothing :
+                        Δ isa AbstractArray ? Array{ComplexF64}(Δ) :
+                        _field_bar(unthunk(Δ).data, Nx, Ny)
+
+function pullback_eps(fields, fs::FactorSet, Δs, tol::Real)
+    h = fs.handle
+    (Nx, Ny) = size(fields[1].grid)
+    total = zeros(Float64, Nx, Ny)
+    g = Array{Float64}(undef, Nx, Ny)
+    for (f, id, Δ) in zip(fields, fs.ids, Δs)
+        bar = _field_bar(Δ, Nx, Ny)
+        bar === nothing && continue
+        if size(bar) == (Nx, Ny)      # cotangent of Ez alone
+            full = zeros(ComplexF64, Nx, Ny, 3)
+            full[:, :, 1] .= bar
+            bar = full
+        end
+        ez = Array{ComplexF64}(f.data[:, :, 1])
+        st = Ref(Stats())
+        GC.@preserve ez bar g check(h, ccall((:fdfd_solve_pullback, libfdfd), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{ComplexF64}, Ptr{ComplexF64}, Cdouble, Ptr{Float64},
+             Ref{Stats}),
+            h.ptr, id, ez, bar, tol, g, st))
+        total .+= g
+    end
+    return total
+end
+
+"""
+    solve_eps(ϵᵣ, d::Device; precision=:fp64, tol=1e-12)
+
+The TM driven solve of `d` with permittivity `ϵᵣ` (d.ϵᵣ is not modified): the
+differentiable form of `solve(d, TM)` for `Zygote` / `ChainRulesCore`, whose
+`rrule` returns dL/dϵᵣ as a real array.
+"""
+function solve_eps(ϵᵣ::AbstractMatrix, d::FDFD.Device; precision::Symbol=:fp64,
+                   tol::Real=1e-12, handle::Handle=default_handle())
+    d2 = deepcopy(d)
+    d2.ϵᵣ = Array{Complex}(ϵᵣ)
+    return solve(d2, FDFD.TM; precision=precision, tol=tol, handle=handle)
+end
+
+function ChainRulesCore.rrule(::typeof(solve_eps), ϵᵣ::AbstractMatrix, d::FDFD.Device;
+                              precision::Symbol=:fp64, tol::Real=1e-12,
+                              handle::Handle=default_handle())
+    d2 = deepcopy(d)
+    d2.ϵᵣ = Array{Complex}(ϵᵣ)
+    fields, fs = solve_keep(d2, precision, tol, handle)
+    y = length(fields) == 1 ? fields[1] : fields
+    function solve_eps_pullback(Δ)
+        Δs = length(fields) == 1 ? [Δ] : collect(unthunk(Δ))
+        g = pullback_eps(fields, fs, Δs, tol)
+        release!(fs)
+        return (NoTangent(), g, NoTangent())
+    end
+    return y, solve_eps_pullback
+end
+
+# rrule of solve(d::Device, TM) itself: the tangent of the Device carries ϵᵣ
+function ChainRulesCore.rrule(::typeof(solve), d::FDFD.Device,
+                              pol::FDFD.Polarization=FDFD.TM; precision::Symbol=:fp64,
+                              tol::Real=1e-12, handle::Handle=default_handle())
+    pol == FDFD.TM || throw(ArgumentError("the solve rrule is implemented for TM"))
+    fields, fs = solve_keep(d, precision, tol, handle)
+    y = length(fields) == 1 ? fields[1] : fields
+    function solve_pullback(Δ)
+        Δs = length(fields) == 1 ? [Δ] : collect(unthunk(Δ))
+        g = pullback_eps(fields, fs, Δs, tol)
+        release!(fs)
+        return (NoTangent(), Tangent{typeof(d)}(; ϵᵣ=g), NoTangent())
+    end
+    return y, solve_pullback
 end
 
 """

@@ -49,12 +49,17 @@ typedef struct {
 enum { FDFD_TM = 1, FDFD_TE = 2 };
 
 /* Factor precision.  FDFD_FP64: complex128 block factors (the plain path).
- * FDFD_FP32_IR: complex64 block factors on FP32 MFMA, accuracy restored by
- * FP64 flexible GMRES on the exact FP64 operator. */
+ * FDFD_FP32_IR: complex64 block factors, accuracy restored by FP64 flexible
+ * GMRES on the exact FP64 operator.  The complex64 products run by default as
+ * fp16x3 MFMA products of row-scaled split operands (about 22-bit products,
+ * FP32 accumulation); the environment switches FDFD_MIXED_GEMM=fp32 and
+ * FDFD_MIXED_TRAIL=fp32, read at fdfd_create, select FP32 MFMA products for
+ * the Schur GEMMs and the Gauss-Jordan trailing updates instead. */
 enum { FDFD_FP64 = 0, FDFD_FP32_IR = 1 };
 
-/* Eigenvalue selection for fdfd_eigenfrequency (Arpack `which`):
- * LM = largest magnitude of the shift-inverted operator = nearest the shift. */
+/* Eigenvalue selection for fdfd_eigenfrequency (Arpack `which`, applied to
+ * the eigenvalues theta = 1/(lambda - sigma) of the shift-inverted operator):
+ * LM = largest |theta| = nearest the shift; LR = largest Re(theta). */
 enum { FDFD_WHICH_LM = 0, FDFD_WHICH_LR = 1 };
 
 /* Status codes. */
@@ -105,9 +110,13 @@ int fdfd_set_eps_d(fdfd_handle* h, const void* d_eps);
 /* ---- factorization ------------------------------------------------------ */
 /* Assembles the TM (driven.jl:35) or TE (driven.jl:45) operator at angular
  * frequency omega (rad/s, normalised units c0/L0 as in device.jl:40) on the
- * device and computes its block-tridiagonal (twisted block-Thomas) factors.
- * Each line block is inverted by Gauss-Jordan elimination with partial
- * pivoting.  Replaces lu(A) of solver.jl:35. */
+ * device and factors it by block cyclic reduction of its ring of grid lines
+ * (the y-periodic corner blocks close the ring): every reduction level
+ * inverts every other line's dense block by Gauss-Jordan elimination with
+ * threshold partial pivoting (|pivot| >= 0.1 x the column's largest
+ * candidate) and folds its couplings into its neighbours with complex GEMMs.
+ * The factorization is a direct solver of A.  Replaces lu(A) of
+ * solver.jl:35. */
 int fdfd_factor(fdfd_handle* h, int pol, double omega, int precision,
                 int* factor_id, fdfd_stats* stats);
 int fdfd_release_factor(fdfd_handle* h, int factor_id);
@@ -146,12 +155,33 @@ int fdfd_residual(fdfd_handle* h, int factor_id, const fdfd_complex* x,
 int fdfd_adjoint_grad(fdfd_handle* h, int factor_id, const fdfd_complex* e_fwd,
                       const fdfd_complex* lam, fdfd_complex weight, double* grad);
 
+/* Pullback (reverse-mode derivative) of a TM driven solve with respect to
+ * eps_r: the body of the ChainRulesCore rrule of solve(d::Device, TM)
+ * (driven.jl:4-59; the reference has no derivative).  For the fields
+ * F = (Ez, Hx, Hy) of fdfd_solve on factor_id and a real loss L(F) whose
+ * cotangent is dfields = dL/dRe F + i dL/dIm F (Nx*Ny*3, the fdfd_solve
+ * layout), it returns grad[k] = dL/d eps_r[k] (Nx*Ny doubles):
+ *   bar e = bar Ez + Mx^H bar Hx + My^H bar Hy   (H = M Ez, driven.jl:40-41)
+ *   A^T lam = conj(bar e)                       (FGMRES on A^T, the factor's
+ *                                                transposed solve, to tol)
+ *   grad[k] = -Re(lam[k] w^2 eps0' Ez[k])       (dA/deps_k, driven.jl:21,35)
+ * ez is the Ez component of the forward solve (Nx*Ny).  The source is taken
+ * as independent of eps_r.  FDFD_ERR_INVALID for TE factors. */
+int fdfd_solve_pullback(fdfd_handle* h, int factor_id, const fdfd_complex* ez,
+                        const fdfd_complex* dfields, double tol, double* grad,
+                        fdfd_stats* stats);
+int fdfd_solve_pullback_d(fdfd_handle* h, int factor_id, const void* d_ez,
+                          const void* d_dfields, double tol, void* d_grad,
+                          fdfd_stats* stats);
+
 /* ---- eigenfrequencies --------------------------------------------------- */
 /* Shift-invert Arnoldi (eigen.jl:69-115): eigenvalues lambda of the TM
  * operator diag(1/eps) (dxf dxb + dyf dyb) nearest sigma = -w0^2 mu0' eps0'
  * (TE: dxf Tex^-1 dxb + dyf Tey^-1 dyb, sigma = -w0^2 mu0'), returned as
- * omega = sqrt(-lambda/(mu0' eps0')) (TE: sqrt(-lambda/mu0')), sorted by
- * |lambda - sigma|.  fields (may be NULL): nev * Nx*Ny*3 eigenfields with
+ * omega = sqrt(-lambda/(mu0' eps0')) (TE: sqrt(-lambda/mu0')), in the order
+ * of `which`: nearest sigma first (LM) or largest Re 1/(lambda - sigma) first
+ * (LR).  precision as for fdfd_factor; other values of precision or which
+ * are FDFD_ERR_INVALID.  fields (may be NULL): nev * Nx*Ny*3 eigenfields with
  * the forward-difference H recovery of eigen.jl:90-91 / 108-109, each
  * eigenvector normalised to unit 2-norm.  The shifted operator is factored
  * once and the factors are reused for every Arnoldi step. */
@@ -276,7 +306,11 @@ int fdfd_flux_d(fdfd_handle* h, int pol, int Nx, int Ny, double dx, double dy,
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source
  * is solved at every frequency).  fdfd_batch_run factors all (job, omega)
  * pairs with the batched block kernels and solves every source; results are
- * read back with fdfd_batch_result.  Uneven grid sizes may be mixed. */
+ * read back with fdfd_batch_result.  Uneven grid sizes may be mixed.
+ * fdfd_batch_run works on the handle: like fdfd_set_grid, it releases every
+ * factor the handle holds (their ids become FDFD_ERR_NOTFOUND) and leaves
+ * the handle without a grid; call fdfd_set_grid / fdfd_set_eps again before
+ * the next fdfd_factor. */
 typedef struct fdfd_batch fdfd_batch;
 int fdfd_batch_create(fdfd_handle* h, fdfd_batch** out);
 int fdfd_batch_add(fdfd_batch* b, int Nx, int Ny, double dx, double dy,
@@ -300,6 +334,29 @@ int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
 /* Panel width override (0 = default 32).  force_full_search != 0 makes every
  * panel search the whole column for its pivots (no candidate window). */
 int fdfd_set_tuning(fdfd_handle* h, int panel_width, int force_full_search);
+
+/* Failure injection for testing the error paths (no counterpart in the
+ * reference, whose only failure handling is the solver fallback of
+ * solver.jl:19-28).  One-shot: the next operation of the named kind on the
+ * handle fails as stated, then the handle behaves normally.
+ *   FDFD_INJECT_PIVOT_FALLBACK  next factorization: the windowed pivot pass is
+ *                               treated as failing its verification, so the
+ *                               full-column search runs (stats.pivot_fallback)
+ *   FDFD_INJECT_SINGULAR        next factorization reports a zero pivot in
+ *                               line block `arg` (FDFD_ERR_SINGULAR)
+ *   FDFD_INJECT_NOCONV          next refined solve runs at most `arg` FGMRES
+ *                               iterations and no restart (FDFD_ERR_NOCONV
+ *                               unless that reaches the tolerance)
+ *   FDFD_INJECT_OOM             next factorization fails its storage
+ *                               allocation (FDFD_ERR_OOM) */
+enum {
+    FDFD_INJECT_NONE = 0,
+    FDFD_INJECT_PIVOT_FALLBACK = 1,
+    FDFD_INJECT_SINGULAR = 2,
+    FDFD_INJECT_NOCONV = 3,
+    FDFD_INJECT_OOM = 4
+};
+int fdfd_debug_inject(fdfd_handle* h, int what, int arg);
 
 #ifdef __cplusplus
 }

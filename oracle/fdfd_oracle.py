@@ -18,14 +18,20 @@ see tests/test_oracle.py):
   constants / normalisation ........ literal values, c0*sqrt(eps0*mu0)==1
   Grid, dx, xc, x2ind .............. hand-computed cases, banker's rounding ties
   delta (periodic differences) ..... dense brute force; Dxf^T == -Dxb; FFT symbol
-  create_sfactor / S_create ........ closed-form sigma(l) at hand-picked cells
+  create_sfactor / S_create ........ design attenuation 2 eta0 int sigma = -lnR; grading
+                                     exponent m = 3.5 from sigma at two depths (ratio
+                                     (l1/l2)^m); literal sigma at the deepest cell
   grid_average ..................... np.roll restatement + hand case
   TM/TE operator ................... dense kron brute force; FFT eigenvalues for
                                      npml=0; Kronecker-sum separability with PML
   solve (SuperLU + FP64 refinement). residual certificate; FFT closed form for
                                      npml=0 uniform eps; dense zgesv tiny case
-  field recovery ................... closed-form differences of the solved field
-  eigenfrequency ................... Kronecker-sum closed form (uniform eps, PML)
+  field recovery ................... discrete Ampere (TM) and Faraday (TE) identities of
+                                     the solved fields; the TE one fails for a swapped
+                                     pairing of the averaged permittivities
+  eigenfrequency ................... FFT (npml=0) and Kronecker-sum (uniform eps, PML)
+                                     closed forms, TM and TE; TE eigenfield identity
+  workload generators (config_c*) .. hand-derived region bounds, seeds, source rows
   modulated (MF-FDFD) system ....... per-Fourier-mode closed form (uniform eps and
                                      deps, npml=0); A^T = W^-1 A W with W = Sxb Syb
   2-D full-vector eigenmode ........ closed-form spectrum (uniform eps); slabs uniform
@@ -674,19 +680,23 @@ def eigen_operator(g: Grid, eps_r: np.ndarray, omega0: float, pol: int):
 
 
 def eigenfrequency(d: Device, pol: int, nev: int, ncv: int | None = None,
-                   tol: float = 0.0):
+                   tol: float = 0.0, which: str = "LM"):
     """Shift-invert Arnoldi via ARPACK (scipy.sparse.linalg.eigs with sigma),
-    the same ARPACK family Arpack.jl wraps (eigen.jl:86,104).  Eigenvalues are
-    returned sorted by distance to sigma; fields use the forward stretched
-    differences (eigen.jl:90-91, 108-109)."""
+    the same ARPACK family Arpack.jl wraps (eigen.jl:86,104); `which` applies
+    to theta = 1/(lambda - sigma) in both.  Eigenvalues are returned nearest
+    sigma first (LM) or largest Re theta first (LR); fields use the forward
+    stretched differences (eigen.jl:90-91, 108-109)."""
     g = d.grid
     eps0, mu0, _ = normalize_parameters(g.L0)
     omega0 = float(d.omega[0])
     A, sigma = eigen_operator(g, d.eps_r, omega0, pol)
     if ncv is None:
         ncv = max(20, 2 * nev + 1)
-    lam, vec = spla.eigs(A, k=nev, sigma=sigma, which="LM", ncv=ncv, tol=tol)
-    order = np.argsort(np.abs(lam - sigma))
+    lam, vec = spla.eigs(A, k=nev, sigma=sigma, which=which, ncv=ncv, tol=tol)
+    if which == "LR":
+        order = np.argsort(-np.real(1 / (lam - sigma)))
+    else:
+        order = np.argsort(np.abs(lam - sigma))
     lam, vec = lam[order], vec[:, order]
     dxf, dxb, dyf, dyb = stretched_deltas(g, omega0)
     fields = []
@@ -847,3 +857,105 @@ def random_pixel_device(Nx: int, Ny: int, npml: int, lam_um: float, seed,
 
 def config_c2(n: int = 1024, npml: int = 20, seed: int = 2):
     return random_pixel_device(n, n, npml, 1.55, seed)
+
+
+def config_c3():
+    """Batched multi-frequency, multi-source TM workload: grids (Nx, Ny) in
+    {(256, 256), (384, 512), (512, 512)}, 20-cell PML, the random-pixel
+    permittivity of config_c2 scaled to each grid (design side
+    round(600/1024 * min(Nx, Ny)/8)*8 cells, 8x8 pixels, seed (3, grid
+    index)), 15-cell guides through the centre row, 64 wavelengths uniform
+    in [1.50, 1.60] um and 16 unit point sources (1im, device.jl:97) per grid
+    at ix ~ U{21..floor(0.2 Nx)}, iy ~ U{21..Ny-20} (1-based, seed (30, grid
+    index)), shared by every wavelength.  The reference would run one
+    solve() per (grid, source) with a factorization per wavelength
+    (driven.jl:11-55): 192 factorizations and 3072 right-hand sides.
+
+    Returns a list of (Device, omegas[64], srcs[Nx, Ny, 16])."""
+    lams = np.linspace(1.50, 1.60, 64)
+    oms = np.array([omega_of_wavelength(l) for l in lams])
+    out = []
+    for gi, (Nx, Ny) in enumerate(((256, 256), (384, 512), (512, 512))):
+        d = random_pixel_device(Nx, Ny, 20, 1.55, [3, gi])
+        d.src[:] = 0
+        rng = np.random.Generator(np.random.PCG64([30, gi]))
+        ix = rng.integers(21, int(0.2 * Nx) + 1, size=16)
+        iy = rng.integers(21, Ny - 20 + 1, size=16)
+        srcs = np.zeros((Nx, Ny, 16), dtype=np.complex128)
+        for r in range(16):
+            srcs[ix[r] - 1, iy[r] - 1, r] = 1j
+        out.append((d, oms, srcs))
+    return out
+
+
+def config_c4(Nx: int = 1024, Ny: int = 768, npml: int = 20, design: int = 256,
+              seed: int = 4, obs_from_pml: int = 50, waist_um: float = 0.25):
+    """Adjoint-gradient workload: Nx x Ny, 20-cell PML, centred design region
+    of design x design cells (1024 x 768: ix 385:640, iy 257:512, 1-based)
+    with eps_r ~ U[1, 12.25] per cell (seed 4, drawn x fastest), 15-cell
+    guides through the centre row left and right of it, the 25-cell line
+    source of config_c2 at ix = 51, lambda = 1.55 um.  The objective is
+    J = |sum_iy Ez[ix_obs, iy] conj(E_mode[iy])|^2 on the column
+    ix_obs = Nx - npml - 50 (1-based; 954 at Nx = 1024, 1 um before the right
+    PML) with E_mode(y) = exp(-((y - y_guide)/0.25 um)^2) at the cell centres
+    and y_guide the centre of the guide cells.  The 32 finite-difference
+    cells are rng(seed=40).choice(design^2, 32) over the design cells, x
+    fastest.
+
+    Returns (Device, m[N] (J = |m^T e|^2), design_cells[design^2] (0-based
+    linear indices, x fastest), fd_cells[32] (linear indices))."""
+    d = random_pixel_device(Nx, Ny, npml, 1.55, 0, design=8)
+    d.eps_r[:] = 1.0
+    guide = 15
+    x0, y0 = (Nx - design) // 2, (Ny - design) // 2
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vals = rng.uniform(1.0, 12.25, design * design).reshape((design, design), order="F")
+    d.eps_r[x0:x0 + design, y0:y0 + design] = vals
+    yg0 = Ny // 2 - guide // 2
+    d.eps_r[:x0, yg0:yg0 + guide] = 12.25
+    d.eps_r[x0 + design:, yg0:yg0 + guide] = 12.25
+    g = d.grid
+    yc = g.yc()
+    y_guide = float(np.mean(yc[yg0:yg0 + guide]))
+    emode = np.exp(-((yc - y_guide) / waist_um) ** 2)
+    ix_obs = Nx - npml - obs_from_pml - 1  # 0-based
+    mfield = np.zeros((Nx, Ny), dtype=np.complex128)
+    mfield[ix_obs, :] = np.conj(emode)
+    m = mfield.ravel(order="F")
+    X, Y = np.meshgrid(np.arange(x0, x0 + design), np.arange(y0, y0 + design), indexing="ij")
+    cells = (X + Y * Nx).ravel(order="F")
+    pick = np.random.Generator(np.random.PCG64(40)).choice(design * design, 32, replace=False)
+    return d, m, cells, cells[pick]
+
+
+def config_c5(n: int = 768, npml: int = 20, a: float = 0.5, r_over_a: float = 0.2,
+              eps_rod: float = 8.9, dl: float = 0.02):
+    """Eigenfrequency workload: n x n cells (dl 0.02 um, 20-cell PML, domain
+    centred on the origin), a triangular lattice (a = 0.5 um) of dielectric
+    rods eps_r = 8.9, radius 0.2a, centres (a(i + j/2), a j sqrt(3)/2) for
+    all integers i, j, with the rod at the origin removed.  A cell belongs to
+    a rod when its centre satisfies (xc - x0)^2 + (yc - y0)^2 <= r^2
+    (device.jl:63-67 evaluates regions at cell centres).  TM, omega0 =
+    2 pi c0 / 1.55 um; eigenfrequency nev = 8 nearest, ncv = 20.
+    Returns the Device (omega = omega0)."""
+    half = n * dl / 2
+    g = Grid.create(dl, (npml, npml), (-half, half), (-half, half))
+    assert g.N == (n, n), g.N
+    d = Device(g, omega_of_wavelength(1.55))
+    xc, yc = g.xc(), g.yc()
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    r = r_over_a * a
+    eps = np.ones((n, n))
+    jmax = int(math.ceil(half / (a * math.sqrt(3) / 2))) + 1
+    for j in range(-jmax, jmax + 1):
+        y0 = a * j * math.sqrt(3) / 2
+        imax = int(math.ceil(half / a + abs(j) / 2)) + 1
+        for i in range(-imax, imax + 1):
+            x0 = a * (i + j / 2)
+            if i == 0 and j == 0:
+                continue
+            if abs(x0) > half + r or abs(y0) > half + r:
+                continue
+            eps[(X - x0) ** 2 + (Y - y0) ** 2 <= r * r] = eps_rod
+    d.eps_r[:] = eps
+    return d
