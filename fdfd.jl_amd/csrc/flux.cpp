@@ -3,6 +3,7 @@
 // post-processing never touches the handle's grid or factors; the handle
 // only supplies the device and stream.
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -11,6 +12,21 @@
 using namespace fdfd;
 
 namespace {
+
+// Scratch of fdfd_flux_d_noalloc: the plane table (int4 per plane) followed
+// by the per-plane sums (double per plane), for at most FLUX_NOALLOC_MAX
+// planes (96 KiB).  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, so the
+// device address of each GPU's copy is looked up once per device
+// (hipGetSymbolAddress with that device current, from fdfd_create) and only
+// that address is used.
+constexpr int FLUX_NOALLOC_MAX = 4096;
+constexpr size_t FLUX_SCRATCH_BYTES = (size_t)FLUX_NOALLOC_MAX * (sizeof(int4) + sizeof(double));
+static __device__ __align__(256) unsigned char scratch[FLUX_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_ptr[MAX_DEVICES] = {};
+static_assert(sizeof(fdfd_flux_plane) == sizeof(int4), "a plane is copied as one int4");
 
 int check_grid(Handle* h, int pol, int Nx, int Ny) {
     if (pol != FDFD_TM && pol != FDFD_TE) return fail(h, FDFD_ERR_INVALID, "pol must be TM or TE");
@@ -53,6 +69,23 @@ int run_flux(Handle* h, int pol, int Nx, int Ny, double dx, double dy, const dou
 }
 
 }  // namespace
+
+namespace fdfd {
+// Device address of this device's flux scratch (called by fdfd_create with
+// the device current; once per device).
+int rt_flux_scratch_setup(Handle* h) {
+    if (h->device < 0 || h->device >= MAX_DEVICES)
+        return fail(h, FDFD_ERR_INVALID, "device ordinal beyond the flux scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[h->device]) return FDFD_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(h, FDFD_ERR_HIP, std::string("flux scratch lookup: ") + hipGetErrorString(e));
+    scratch_ptr[h->device] = p;
+    return FDFD_OK;
+}
+}  // namespace fdfd
 
 extern "C" int fdfd_poynting_d(fdfd_handle* hh, int pol, int Nx, int Ny, const void* d_fields,
                                void* d_S) {
@@ -118,4 +151,38 @@ extern "C" int fdfd_flux(fdfd_handle* hh, int pol, int Nx, int Ny, double dx, do
     if (!rc) rc = run_flux(h, pol, Nx, Ny, dx, dy, d_f, nplanes, planes, flux);
     (void)hipFree(d_f);
     return rc;
+}
+
+// fdfd_flux_d without any device allocation: the plane table and the sums go
+// through this device's static scratch instead of a per-call workspace.  At
+// most FLUX_NOALLOC_MAX planes.  Calls that overlap on one GPU (several
+// handles or threads) share the scratch and race; each call finishes (stream
+// synchronized) before it returns.  Every sum the kernel reads back is
+// written by the same call, so nothing needs clearing between calls.
+extern "C" int fdfd_flux_d_noalloc(fdfd_handle* hh, int pol, int Nx, int Ny, double dx, double dy,
+                                   const void* d_fields, int nplanes,
+                                   const fdfd_flux_plane* planes, double* flux) {
+    Handle* h = hh ? &hh->impl : nullptr;
+    if (!h || !d_fields || !flux) return FDFD_ERR_INVALID;
+    int rc = check_grid(h, pol, Nx, Ny);
+    if (!rc) rc = check_planes(h, Nx, Ny, nplanes, planes);
+    if (rc) return rc;
+    if (nplanes > FLUX_NOALLOC_MAX)
+        return fail(h, FDFD_ERR_INVALID, "fdfd_flux_d_noalloc takes at most " +
+                                             std::to_string(FLUX_NOALLOC_MAX) + " planes");
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        base = (h->device >= 0 && h->device < MAX_DEVICES) ? scratch_ptr[h->device] : nullptr;
+    }
+    if (!base) return fail(h, FDFD_ERR_STATE, "flux scratch of this device was not set up");
+    (void)hipSetDevice(h->device);
+    int4* d_p = (int4*)base;
+    double* d_o = (double*)((char*)base + (size_t)FLUX_NOALLOC_MAX * sizeof(int4));
+    HIPCHK(h, hipMemcpyAsync(d_p, planes, sizeof(int4) * nplanes, hipMemcpyHostToDevice, h->stream));
+    launch_flux(h->stream, d_o, (const double2*)d_fields, d_p, nplanes, Nx, Ny, pol, dx, dy);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(flux, d_o, sizeof(double) * nplanes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return FDFD_OK;
 }

@@ -427,6 +427,12 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
         delete hh;
         return FDFD_ERR_HIP;
     }
+    // per-device static scratch of the no-allocation entry points
+    if (rt_flux_scratch_setup(&hh->impl) != FDFD_OK) {
+        (void)hipStreamDestroy(hh->impl.stream);
+        delete hh;
+        return FDFD_ERR_HIP;
+    }
     *out = hh;
     return FDFD_OK;
 }

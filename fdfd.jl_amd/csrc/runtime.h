@@ -172,6 +172,7 @@ void rt_free_factor(Handle* h, Factor* F);
 // work buffers of the handle (grown on demand)
 int ensure_factor_work(Handle* h, int n, int nch);
 int ensure_solve_work(Handle* h, long N);
+int rt_flux_scratch_setup(Handle* h);
 
 // Right-preconditioned FGMRES(30) in FP64 on an operator given by callbacks:
 //   apply(x, y, b): y = b - A x if b != nullptr, else y = A x

@@ -87,6 +87,8 @@ PROTOTYPES = {
     "fdfd_poynting_d": (_I, [_P, _I, _I, _I, _P, _P]),
     "fdfd_flux": (_I, [_P, _I, _I, _I, _D, _D, _P, _I, C.POINTER(fdfd_flux_plane), _PD]),
     "fdfd_flux_d": (_I, [_P, _I, _I, _I, _D, _D, _P, _I, C.POINTER(fdfd_flux_plane), _PD]),
+    "fdfd_flux_d_noalloc": (_I, [_P, _I, _I, _I, _D, _D, _P, _I, C.POINTER(fdfd_flux_plane),
+                                 _PD]),
     "fdfd_vector_eigenmode": (_I, [_P, _D, fdfd_complex, _I, _I, _I, _D, _I, _P, _P, _PS]),
     "fdfd_batch_create": (_I, [_P, C.POINTER(_P)]),
     "fdfd_batch_add": (_I, [_P, _I, _I, _D, _D, _I, _I, _D, _I, _P, _I, _P, _I, _P, _PI]),

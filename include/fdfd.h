@@ -301,6 +301,16 @@ int fdfd_flux(fdfd_handle* h, int pol, int Nx, int Ny, double dx, double dy,
 int fdfd_flux_d(fdfd_handle* h, int pol, int Nx, int Ny, double dx, double dy,
                 const void* d_fields, int nplanes,
                 const fdfd_flux_plane* planes, double* flux);
+/* fdfd_flux_d without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * plane table and the per-plane sums go through a static per-GPU scratch of
+ * 96 KiB instead of a per-call workspace, so nplanes <= 4096
+ * (FDFD_ERR_INVALID above).  Same results as fdfd_flux_d.  Calls that overlap
+ * on one GPU share that scratch and race. */
+int fdfd_flux_d_noalloc(fdfd_handle* h, int pol, int Nx, int Ny, double dx,
+                        double dy, const void* d_fields, int nplanes,
+                        const fdfd_flux_plane* planes, double* flux);
 
 /* ---- batched driven solves (many grids x frequencies x sources) --------- */
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source
