@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 PEAK = {"hbm": 8.0e12, "fp64": 78.6e12, "fp32": 157.3e12, "fp16": 2.5e15}
-KERNELS = ["trailing", "panel", "form", "solve", "stencil", "zbuild", "gemm"]
+KERNELS = ["trailing", "panel", "form", "solve", "stencil", "zbuild", "gemm", "assembly"]
 
 
 class Hip:
@@ -64,38 +64,54 @@ class Hip:
         self.lib.hipDeviceSynchronize()
 
 
-def cpu_baseline(n_target: int, sizes=(256, 512)):
-    """SuperLU (scipy splu, COLAMD, single-threaded) on the same workload
-    generator at the sample sizes; 1024^2 time extrapolated with the measured
-    growth between the two samples."""
+def _cpu_count():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count()
+
+
+def cpu_baseline(n_target: int):
+    """The reference's CPU path restated: assemble the same sparse A and b
+    (driven.jl:21-36) and solve x = A \\ b with scipy's SuperLU (COLAMD, one
+    thread; the reference calls UMFPACK through Julia's lu).  Timed on the
+    host cores of this machine at config C1 (200 x 120) and at the metric's
+    workload (n_target^2, the same random-pixel coupler as the GPU steps):
+    assembly + factor + solve, one run each."""
+    import scipy
     import scipy.sparse.linalg as spla
 
     import fdfd_oracle as O
 
-    times = []
-    for n in sizes:
-        d = O.random_pixel_device(n, n, 20, 1.55, 2)
+    def timed(d):
         om = float(d.omega[0])
         t0 = time.perf_counter()
         A = O.assemble(d.grid, d.eps_r, om, O.TM)
         b = O.rhs(om, d.src)
+        t1 = time.perf_counter()
         lu = spla.splu(A.tocsc(), permc_spec="COLAMD")
+        t2 = time.perf_counter()
         x = lu.solve(b)
-        t = time.perf_counter() - t0
-        res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
-        times.append((n, t, res))
-    (n1, t1, _), (n2, t2, r2) = times
-    expo = math.log(t2 / t1) / math.log(n2 / n1)
-    t_target = t2 * (n_target / n2) ** expo
+        t3 = time.perf_counter()
+        res = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+        return {"assembly_s": t1 - t0, "factor_s": t2 - t1, "solve_s": t3 - t2,
+                "total_s": t3 - t0, "residual": res}
+
+    c1 = timed(O.config_c1())
+    big = timed(O.random_pixel_device(n_target, n_target, 20, 1.55, 2))
     return {
-        "value": 1.0 / t_target,
+        "value": 1.0 / big["total_s"],
         "unit": f"TM solves/s at {n_target}^2",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle sparse LU (scipy SuperLU, COLAMD, 1 thread) of the same random-pixel "
-                   f"TM workload at {n1}^2 ({t1:.2f} s) and {n2}^2 ({t2:.2f} s, residual "
-                   f"{r2:.1e}); {n_target}^2 time {t_target:.1f} s extrapolated with the "
-                   f"measured exponent {expo:.2f} in the side length"),
+        "host_cpus_available": _cpu_count(),
+        "library": f"scipy {scipy.__version__} SuperLU (splu, COLAMD), single-threaded",
+        "sample": (f"one TM solve of the {n_target}^2 random-pixel coupler (assembly "
+                   f"{big['assembly_s']:.1f} s + factor {big['factor_s']:.1f} s + solve "
+                   f"{big['solve_s']:.2f} s, residual {big['residual']:.1e}) and one of config C1 "
+                   f"200x120 ({c1['total_s'] * 1e3:.0f} ms)"),
+        "c1": c1,
+        "n_target": big,
     }
 
 
@@ -178,6 +194,116 @@ def modulated_leg(eng, with_cpu: bool):
     return out
 
 
+def _cgrid(g):
+    from fdfd.core import Grid
+    return Grid(g.dx, g.Npml, (g.bounds[0][0], g.bounds[1][0]), (g.bounds[0][1], g.bounds[1][1]),
+                g.L0)
+
+
+def te_leg(eng, hip, d, eps_d, src_d, out_d, steps: int):
+    """TE driven solves of the metric's workload (1024^2 coupler, Hz line
+    source), same step as the headline: device-resident eps/src -> assemble ->
+    factor -> refined solve to 1e-10 -> fields."""
+    from fdfd import _lib
+    from fdfd.solve import PRECISIONS
+    om = float(d.omega[0])
+    lib, h = eng.lib, eng.h
+    out = {"workload": "TE driven solve of the same 1024^2 coupler (Hz line source), tol 1e-10"}
+    for pname in ("mixed", "fp64"):
+        def step():
+            st = _lib.fdfd_stats()
+            fid = C.c_int()
+            _lib.check(lib, h, lib.fdfd_set_eps_d(h, eps_d))
+            _lib.check(lib, h, lib.fdfd_factor(h, 2, om, PRECISIONS[pname], C.byref(fid),
+                                               C.byref(st)))
+            _lib.check(lib, h, lib.fdfd_solve_d(h, fid.value, 1, src_d, out_d, 1e-10, C.byref(st)))
+            _lib.check(lib, h, lib.fdfd_release_factor(h, fid.value))
+            return st.as_dict()
+        step()
+        hip.sync()
+        t0 = time.perf_counter()
+        sts = [step() for _ in range(steps)]
+        hip.sync()
+        dt = (time.perf_counter() - t0) / steps
+        out[pname] = {"solves_per_s": 1 / dt, "ms_per_step": dt * 1e3,
+                      "gmres_its": int(np.mean([x["iterations"] for x in sts])),
+                      "residual": max(x["residual"] for x in sts)}
+    out["mixed_over_plain"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
+def c3_leg(eng):
+    """Batched sweep: 3 grids (256^2, 384x512, 512^2) x 64 wavelengths x 16
+    point sources = 192 factorizations and 3072 refined solves (mixed, to
+    1e-10) in one fdfd_batch_run.  Wall time of the call (eps and sources
+    uploaded inside it; results stay on the device)."""
+    import fdfd_oracle as O
+    jobs = O.config_c3()
+    spec = [(_cgrid(dd.grid), 1, dd.eps_r, oms, srcs) for dd, oms, srcs in jobs]
+    nsolve = sum(len(oms) * srcs.shape[2] for _, oms, srcs in jobs)
+    t0 = time.perf_counter()
+    _, st = eng.batch_solve(spec, precision="mixed", tol=1e-10, fetch=[])
+    wall = time.perf_counter() - t0
+    return {"workload": "3 grids (256^2, 384x512, 512^2) x 64 wavelengths (1.50-1.60 um) x 16 "
+                        "point sources, complex64 factors + FP64 FGMRES to 1e-10, one call",
+            "solves": nsolve, "factorizations": 192, "wall_s": wall, "solves_per_s": nsolve / wall,
+            "factor_ms": st["t_factor_ms"], "solve_ms": st["t_solve_ms"],
+            "fgmres_its": st["iterations"], "worst_residual": st["residual"]}
+
+
+def c4_leg(eng, reps: int = 3):
+    """Adjoint gradients of J = |m^T Ez|^2 on the 1024 x 768 coupler with a
+    256 x 256 continuous design: forward factor + refined solve + pullback
+    (transposed FGMRES on the same factor + gradient kernel) per gradient."""
+    import fdfd_oracle as O
+    from fdfd import adjoint, core
+    d, m, cells, _ = O.config_c4()
+    dev = core.Device(_cgrid(d.grid), float(d.omega[0]))
+    dev.eps_r[:] = d.eps_r
+    dev.src[:] = d.src
+    mf = m.reshape(d.grid.N, order="F")
+    out = {"workload": "dJ/deps, 1024x768, 65,536 design cells, tol 1e-10 (host arrays in/out)"}
+    grads = {}
+    for pname in ("mixed", "fp64"):
+        adjoint.objective_and_gradient(dev, mf, precision=pname, tol=1e-10, eng=eng)  # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            J, g, _ = adjoint.objective_and_gradient(dev, mf, precision=pname, tol=1e-10, eng=eng)
+        dt = (time.perf_counter() - t0) / reps
+        grads[pname] = g.ravel(order="F")[cells]
+        out[pname] = {"gradients_per_s": 1 / dt, "ms_per_gradient": dt * 1e3}
+    out["mixed_vs_plain_rel"] = float(np.linalg.norm(grads["mixed"] - grads["fp64"]) /
+                                      np.linalg.norm(grads["fp64"]))
+    return out
+
+
+def c5_leg(eng):
+    """8 eigenfrequencies nearest 2 pi c0/1.55 um of the 768^2 triangular rod
+    lattice with a missing rod (shift-invert Arnoldi on the factors, tol
+    1e-10), plain and mixed factors; omega and Q differences between them."""
+    import fdfd_oracle as O
+    d = O.config_c5()
+    eng.set_grid(_cgrid(d.grid))
+    eng.set_eps(d.eps_r)
+    out = {"workload": "TM eigenfrequencies, 768^2 rod lattice, nev=8, ncv=20, tol 1e-10"}
+    ws = {}
+    for pname in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        w, _, st = eng.eigenfrequency(1, float(d.omega[0]), 8, ncv=20, precision=pname, tol=1e-10,
+                                      want_fields=False)
+        dt = time.perf_counter() - t0
+        ws[pname] = w
+        out[pname] = {"seconds": dt, "factor_ms": st["t_factor_ms"], "fgmres_its": st["iterations"],
+                      "restarts": st["restarts"]}
+    wp = ws["fp64"]
+    wm = np.array([ws["mixed"][int(np.argmin(np.abs(ws["mixed"] - x)))] for x in wp])
+    qp, qm = wp.real / (2 * np.abs(wp.imag)), wm.real / (2 * np.abs(wm.imag))
+    out["omega_rel_diff"] = float(np.max(np.abs(wm - wp) / np.abs(wp)))
+    out["Q_rel_diff"] = float(np.max(np.abs(qm - qp) / qp))
+    out["Q"] = [float(q) for q in qp]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,11 +314,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-legs", action="store_true",
-                    help="skip the mode-source and modulated-solve legs (headline only)")
-    ap.add_argument("--precisions", default="mixed,fp64,mixed-fp32gemm",
-                    help="mixed: complex64 factors with fp16x3 GEMM products (default mixed "
-                         "path); mixed-fp32gemm: complex64 factors with FP32 MFMA products; "
-                         "fp64: complex128 factors")
+                    help="skip every leg but the headline TM solves (profiling runs)")
+    ap.add_argument("--precisions", default="mixed,fp64,mixed-fp32gemm,mixed-fp32",
+                    help="mixed: complex64 factors with fp16x3 GEMM and trailing products (default "
+                         "mixed path); mixed-fp32gemm: FP32 MFMA GEMMs, fp16x3 trailing updates; "
+                         "mixed-fp32: FP32 MFMA GEMMs and trailing updates; fp64: complex128 "
+                         "factors")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only run 4 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
@@ -215,24 +342,28 @@ def main():
 
     import fdfd_oracle as O
     from fdfd import _lib
-    from fdfd.core import Grid
     from fdfd.solve import PRECISIONS, Engine
 
     n = a.n
     d = O.random_pixel_device(n, n, 20, 1.55, 2)
-    g = d.grid
     om = float(d.omega[0])
     N = n * n
-    grid = Grid(0.02, g.Npml, (g.bounds[0][0], g.bounds[1][0]), (g.bounds[0][1], g.bounds[1][1]),
-                g.L0)
+    grid = _cgrid(d.grid)
     eng = Engine(0)
     eng.set_grid(grid)
-    eng_x = None  # handle with FP32 GEMM products on the mixed path (read at handle creation)
-    if "mixed-fp32gemm" in a.precisions.split(","):
-        os.environ["FDFD_MIXED_GEMM"] = "fp32"
-        eng_x = Engine(0)
-        eng_x.set_grid(grid)
-        del os.environ["FDFD_MIXED_GEMM"]
+    # handles whose mixed path uses FP32 MFMA products (switches read at creation)
+    variant_env = {"mixed-fp32gemm": {"FDFD_MIXED_GEMM": "fp32"},
+                   "mixed-fp32": {"FDFD_MIXED_GEMM": "fp32", "FDFD_MIXED_TRAIL": "fp32"}}
+    engines = {}
+    for pname in a.precisions.split(","):
+        if pname in variant_env:
+            os.environ.update(variant_env[pname])
+            engines[pname] = Engine(0)
+            engines[pname].set_grid(grid)
+            for k in variant_env[pname]:
+                del os.environ[k]
+        else:
+            engines[pname] = eng
     hip = Hip()
     eps_d = hip.upload(np.ascontiguousarray(d.eps_r.ravel(order="F")))
     src_d = hip.upload(np.ascontiguousarray(d.src.ravel(order="F")))
@@ -253,8 +384,8 @@ def main():
 
     results = {}
     for pname in a.precisions.split(","):
-        e = eng_x if pname == "mixed-fp32gemm" else eng
-        prec = PRECISIONS["mixed" if pname == "mixed-fp32gemm" else pname]
+        e = engines[pname]
+        prec = PRECISIONS["mixed" if pname.startswith("mixed") else pname]
         for _ in range(a.warmup):
             step(prec, e)
         hip.sync()
@@ -297,27 +428,33 @@ def main():
     if not kd["launches"]:  # --profile-steps 0: no per-kernel events were recorded
         kd = {"ms": 0.0, "launches": 1, "flops": 0.0, "bytes": 0.0}
     avg_s = kd["ms"] / kd["launches"] / 1e3
-    # roofline of the dominant kernel: the tighter of its MFMA ceiling
-    # (algorithmic flops / matrix peak) and its HBM ceiling (algorithmic
-    # bytes / 8 TB/s) is its bound; frac = achieved / that peak
+    # roofline of the dominant kernel against the published peak of the MFMA
+    # instructions it runs (fp16 for the split-operand GEMM, FP32 or FP64
+    # otherwise) or of HBM; achieved = algorithmic flops (8 n^3 per complex
+    # GEMM) or bytes per launch / average launch time
     if head == "mixed":
-        if dom == "gemm":
-            # each real product of the default mixed GEMM is formed from
-            # three fp16 MFMA products, so its ceiling is the fp16 peak / 3
-            mpeak, pnote = PEAK["fp16"] / 3, "fp16 dense MFMA peak / 3 (fp16x3 products)"
-        else:
-            mpeak, pnote = PEAK["fp32"], "FP32 dense MFMA peak"
+        ptype = "fp16" if (dom == "gemm" or dom == "trailing") else "fp32"
+    elif head.startswith("mixed"):
+        ptype = "fp32"
     else:
-        mpeak, pnote = PEAK["fp64"], "FP64 dense MFMA peak"
+        ptype = "fp64"
+    mpeak = PEAK[ptype]
+    # executed MFMA work per algorithmic flop: fp16x3 forms each real product
+    # from three fp16 products
+    exec_factor = 3.0 if ptype == "fp16" else 1.0
     per_launch_f = kd["flops"] / kd["launches"]
     per_launch_b = kd["bytes"] / kd["launches"]
     t_m = per_launch_f / mpeak
     t_b = per_launch_b / PEAK["hbm"]
-    if per_launch_f > 0 and t_m >= t_b:
+    if per_launch_f > 0 and t_m * exec_factor >= t_b:
         achieved = per_launch_f / avg_s
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": mpeak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / mpeak, "traffic": None, "kernel": dom,
-                "peak_note": pnote}
+                "peak_note": f"published dense {ptype} MFMA peak of MI355X",
+                "mfma_executed_tflops": exec_factor * achieved / 1e12,
+                "mfma_executed_frac": exec_factor * achieved / mpeak,
+                "executed_note": ("fp16x3 split products: 3 fp16 MFMA products per real "
+                                  "product" if exec_factor > 1 else "executed = algorithmic")}
         if per_launch_b > 0:
             roof["hbm_achieved_GBps"] = per_launch_b / avg_s / 1e9
             roof["flop_per_byte"] = per_launch_f / per_launch_b
@@ -336,10 +473,7 @@ def main():
     if os.path.exists(pmc):
         with open(pmc) as fh:
             P = json.load(fh)
-        rt = "float" if head == "mixed" else "double"
-        # kernel-name prefix of the dominant slot (the mixed path's GEMM is
-        # the split-operand fp16x3 kernel)
-        # (mixed lines with n % 4 == 0 use the fp16x3 row-strip trailing kernel)
+        rt = "float" if head.startswith("mixed") else "double"
         trail = "k_trailing_h3" if head == "mixed" and n % 4 == 0 else "k_trailing<" + rt
         prefix = {"gemm": "k_gemm_b3" if head == "mixed" else "k_gemm<" + rt,
                   "trailing": trail, "solve": "k_mv<" + rt}.get(dom)
@@ -350,10 +484,39 @@ def main():
                 roof["traffic_source"] = os.path.relpath(pmc, ROOT)
                 break
     roof["avg_launch_us"] = avg_s * 1e6
-    # whole-factorization MFMA figure: 8 n^3 m real flops per factorization
+    # whole-factorization figure: F = 8 n^3 m real flops per factorization
     roof["factor_tflops"] = 8.0 * n ** 3 * n / (R["factor_ms"] / 1e3) / 1e12
+    roof["factor_frac_fp32_peak"] = roof["factor_tflops"] * 1e12 / PEAK["fp32"]
+    # the stencil / residual kernel (matrix-free operator apply, 64 B/cell)
+    ks = R["kernels"]["stencil"]
+    stencil = None
+    if ks["launches"] and ks["bytes"] > 0:
+        gbps = ks["bytes"] / (ks["ms"] / 1e3) / 1e9
+        stencil = {"kernel": "k_apply (residual b - A'x / A'x)", "algorithmic_bytes_per_cell": 64,
+                   "launches": ks["launches"], "avg_launch_us": 1e3 * ks["ms"] / ks["launches"],
+                   "achieved_GBps": gbps, "frac_hbm": gbps * 1e9 / PEAK["hbm"]}
 
-    cpu = None if a.no_cpu else cpu_baseline(n)
+    legs = {}
+    if not a.no_legs:
+        print("# TE leg", file=sys.stderr, flush=True)
+        legs["te"] = te_leg(eng, hip, d, eps_d, src_d, out_d, max(2, a.steps // 4))
+    for p_ in (eps_d, src_d, out_d):
+        hip.free(p_)
+    if not a.no_legs:
+        print("# C3 leg", file=sys.stderr, flush=True)
+        legs["c3_batch"] = c3_leg(eng)
+        print("# C4 leg", file=sys.stderr, flush=True)
+        legs["c4_adjoint"] = c4_leg(eng)
+        print("# C5 leg", file=sys.stderr, flush=True)
+        legs["c5_eigen"] = c5_leg(eng)
+        print("# mode and modulated legs", file=sys.stderr, flush=True)
+        eng.set_grid(grid)
+        legs["modes"] = mode_leg(eng, d, not a.no_cpu)
+        legs["modulated"] = modulated_leg(eng, not a.no_cpu)
+    cpu = None
+    if not a.no_cpu:
+        print("# CPU baseline (SuperLU at C1 and the metric size)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(n)
     line = {
         "metric": "TM solves/s at 1024^2 to 1e-10 rel. residual, mixed vs plain FP64 GPU path",
         "value": R["solves_per_s"],
@@ -365,8 +528,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64 factors (GEMM products fp16x3 on row-scaled split operands, FP32 accumulation) + c128 refinement"
-                  if head == "mixed" else "c128"),
+        "dtype": ("c64 factors (GEMM and trailing products fp16x3 on row-scaled split operands, "
+                  "FP32 accumulation) + c128 refinement" if head == "mixed" else "c128"),
         "data": "synthetic (seeded random-pixel coupler, generated in-process)",
         "config": {
             "workload": (f"TM driven solve, {n}x{n} cells, dl=0.02 um, 20-cell SC-PML, 600x600 "
@@ -375,6 +538,7 @@ def main():
             "n": n, "tol": a.tol,
             "plain_fp64_solves_per_s": results.get("fp64", {}).get("solves_per_s"),
             "mixed_fp32_gemm_solves_per_s": results.get("mixed-fp32gemm", {}).get("solves_per_s"),
+            "mixed_fp32_solves_per_s": results.get("mixed-fp32", {}).get("solves_per_s"),
             "mixed_over_plain": (results["mixed"]["solves_per_s"] / results["fp64"]["solves_per_s"]
                                  if "mixed" in results and "fp64" in results else None),
             "factor_ms": {k: v["factor_ms"] for k, v in results.items()},
@@ -384,16 +548,14 @@ def main():
             "pivot_fallback": {k: v["pivot_fallback"] for k, v in results.items()},
         },
         "roofline": roof,
+        "stencil": stencil,
         "cpu_baseline": cpu,
     }
-    if not a.no_legs:
-        line["modes"] = mode_leg(eng, d, not a.no_cpu)
-        line["modulated"] = modulated_leg(eng, not a.no_cpu)
-    for p in (eps_d, src_d, out_d):
-        hip.free(p)
+    line.update(legs)
     eng.close()
-    if eng_x is not None:
-        eng_x.close()
+    for pname, e in engines.items():
+        if e is not eng:
+            e.close()
     print(json.dumps(line))
 
 

@@ -9,8 +9,12 @@
 // then solved with FP64 FGMRES on its factor; fields stay in device memory
 // until fdfd_batch_result copies one set out.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "../../include/fdfd.h"
@@ -91,21 +95,90 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     if (precision != FDFD_FP64 && precision != FDFD_FP32_IR)
         return fail(h, FDFD_ERR_INVALID, "precision must be FDFD_FP64 or FDFD_FP32_IR");
     if (!(tol > 0)) return fail(h, FDFD_ERR_INVALID, "tol must be > 0");
+    (void)hipSetDevice(h->device);
     free_results(b);
-    int iters = 0, pivot_fallback = 0;
-    double worst = 0, t_factor = 0, t_solve = 0, growth = 0;
+    // the batch replaces the handle's problem: its factors are released and
+    // it is left without a grid (fdfd_set_grid starts the next problem)
+    (void)hipStreamSynchronize(h->stream);
+    for (auto& kv : h->factors) rt_free_factor(h, kv.second);
+    h->factors.clear();
+    h->has_grid = h->has_eps = false;
+    const bool timing = std::getenv("FDFD_DEBUG_TIMING") != nullptr;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto secs = [](auto a, auto c) { return std::chrono::duration<double>(c - a).count(); };
+    // largest factors first (the slab below is sized by them)
+    std::vector<int> order(b->jobs.size());
+    std::iota(order.begin(), order.end(), 0);
+    auto fbytes = [&](const BatchJob& j) {
+        const int n = std::min(j.g.Nx, j.g.Ny), m = std::max(j.g.Nx, j.g.Ny);
+        return rt_factor_bytes(n, m, precision, h->gemm_b3 != 0);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return fbytes(b->jobs[a]) > fbytes(b->jobs[c]); });
+    // Memory plan: result buffers first, then one slab for the factor arenas
+    // of the largest group of frequencies, reused by every group of every
+    // grid (no factor-sized allocation or free inside the batch).
+    arena_clear(h);
     int rc = FDFD_OK;
     for (auto& job : b->jobs) {
-        const GridDesc& g = job.g;
-        const long N = (long)g.Nx * g.Ny;
-        if ((rc = fdfd_set_grid(hh, g.Nx, g.Ny, g.dx, g.dy, g.npx, g.npy, g.L0))) break;
-        if ((rc = fdfd_set_eps(hh, (const fdfd_complex*)job.eps.data()))) break;
-        const size_t nres = (size_t)job.omegas.size() * job.nsrc * 3 * N;
+        const size_t nres = (size_t)job.omegas.size() * job.nsrc * 3 * job.g.Nx * (size_t)job.g.Ny;
         if (hipMalloc(&job.result, sizeof(double2) * nres) != hipSuccess) {
             job.result = nullptr;
-            rc = fail(h, FDFD_ERR_OOM, "batch: result buffer allocation failed");
-            break;
+            free_results(b);
+            return fail(h, FDFD_ERR_OOM, "batch: result buffer allocation failed");
         }
+    }
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    std::vector<int> group(b->jobs.size(), 1);
+    std::vector<size_t> arena_bytes(b->jobs.size(), 0);
+    size_t slab_need = 0;
+    for (size_t ji = 0; ji < b->jobs.size(); ++ji) {
+        const BatchJob& job = b->jobs[ji];
+        const int n = std::min(job.g.Nx, job.g.Ny), m = std::max(job.g.Nx, job.g.Ny);
+        size_t sb = 0, ss = 0;
+        rt_factor_arena_sizes(n, m, precision, h->gemm_b3 != 0, &sb, &ss);
+        arena_bytes[ji] = ((sb + 255) & ~(size_t)255) + ((ss + 255) & ~(size_t)255);
+        // per frequency: the arenas plus the operator, Gauss-Jordan work and
+        // tables allocated beside them (about 96 B and 2 GJ channels per cell)
+        const double N = (double)job.g.Nx * job.g.Ny;
+        const double per = (double)arena_bytes[ji] + 96.0 * N + 64.0 * N * (double)VMAX / n;
+        int B = (int)std::floor(0.8 * (double)freeb / per);
+        B = std::max(1, std::min<int>({B, (int)job.omegas.size(), 64}));
+        group[ji] = B;
+        slab_need = std::max(slab_need, (size_t)B * arena_bytes[ji]);
+    }
+    if (hipMalloc((void**)&h->slab, slab_need) != hipSuccess) {
+        (void)hipGetLastError();
+        h->slab = nullptr;  // factors then allocate their arenas one by one
+        h->slab_size = 0;
+    } else {
+        h->slab_size = slab_need;
+    }
+    h->slab_used = 0;
+    int iters = 0, pivot_fallback = 0;
+    double worst = 0, t_factor = 0, t_solve = 0, growth = 0;
+    for (int ji : order) {
+        BatchJob& job = b->jobs[ji];
+        const auto tj0 = now();
+        const GridDesc& g = job.g;
+        const long N = (long)g.Nx * g.Ny;
+        h->g = g;
+        h->has_grid = true;
+        if (h->eps_n < N) {
+            if (h->eps) (void)hipFree(h->eps);
+            h->eps = nullptr;
+            h->eps_n = 0;
+            if (hipMalloc(&h->eps, sizeof(double2) * N) != hipSuccess) {
+                h->eps = nullptr;
+                rc = fail(h, FDFD_ERR_OOM, "batch: eps buffer allocation failed");
+                break;
+            }
+            h->eps_n = N;
+        }
+        if ((rc = stream_copy(h, h->eps, job.eps.data(), sizeof(double2) * N, hipMemcpyHostToDevice)))
+            break;
+        h->has_eps = true;
         double2* dsrc = nullptr;
         if (hipMalloc(&dsrc, sizeof(double2) * N * job.nsrc) != hipSuccess ||
             stream_copy(h, dsrc, job.srcs.data(), sizeof(double2) * N * job.nsrc,
@@ -114,13 +187,8 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
             rc = fail(h, FDFD_ERR_OOM, "batch: source upload failed");
             break;
         }
-        // frequencies per group: what the free HBM holds (factors + operator)
-        const int n = std::min(g.Nx, g.Ny), m = std::max(g.Nx, g.Ny);
-        const double per = (double)rt_factor_bytes(n, m, precision, h->gemm_b3 != 0) + 96.0 * N;
-        size_t freeb = 0, totalb = 0;
-        (void)hipMemGetInfo(&freeb, &totalb);
-        int B = (int)std::max(1.0, std::floor(0.8 * (double)freeb / per - 1.0));
-        B = std::min<int>({B, (int)job.omegas.size(), 64});
+        const int B = group[ji];
+        double tf = 0, ts = 0;
         for (size_t o0 = 0; o0 < job.omegas.size() && rc == FDFD_OK; o0 += B) {
             const int nb = (int)std::min<size_t>(B, job.omegas.size() - o0);
             std::vector<Factor*> Fs;
@@ -131,13 +199,17 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 F->precision = precision;
                 Fs.push_back(F);
             }
+            const auto tb0 = now();
+            h->slab_used = 0;  // the previous group's factors are freed
             rc = rt_build_factors(h, Fs);
+            tf += secs(tb0, now());
             for (Factor* F : Fs) {
                 pivot_fallback |= F->pivot_fallback;
                 growth = std::max(growth, F->pivot_growth);
             }
             if (rc == FDFD_OK) {
                 t_factor += Fs[0]->t_factor_ms;
+                const auto ts0 = now();
                 hipEvent_t e0, e1;
                 (void)hipEventCreate(&e0);
                 (void)hipEventCreate(&e1);
@@ -168,12 +240,24 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 t_solve += ms;
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
+                ts += secs(ts0, now());
             }
+            const auto tr0 = now();
             for (Factor* F : Fs) rt_free_factor(h, F);
+            tf += secs(tr0, now());
         }
         (void)hipFree(dsrc);
+        if (timing)
+            fprintf(stderr, "batch job %d (%d x %d, %zu omegas, %d sources, B=%d): %.3f s "
+                    "(factor incl. host %.3f s, solves %.3f s)\n", ji, g.Nx, g.Ny,
+                    job.omegas.size(), job.nsrc, B, secs(tj0, now()), tf, ts);
         if (rc) break;
     }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->slab);
+    h->slab = nullptr;
+    h->slab_size = h->slab_used = 0;
+    h->has_grid = h->has_eps = false;
     if (stats) {  // also on failure: what was reached
         memset(stats, 0, sizeof(*stats));
         stats->iterations = iters;

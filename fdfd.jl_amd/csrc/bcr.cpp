@@ -47,6 +47,14 @@ size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
     return bytes;
 }
 
+// the two arenas a factor takes: its blocks and (fp16x3 GEMMs) the split operands
+void rt_factor_arena_sizes(int n, int m, int precision, bool b3, size_t* blocks, size_t* split) {
+    const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
+    const Plan P = make_plan(m);
+    *blocks = (size_t)P.nblocks * n * n * elem;
+    *split = (b3 && precision != FDFD_FP64) ? (size_t)P.nblocks * b3_block_bytes(n) : 0;
+}
+
 // ---------------------------------------------------------------------------
 // device tables for a set of factors built together (same grid and shape)
 // ---------------------------------------------------------------------------
@@ -105,6 +113,8 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     const long N = (long)g.Nx * g.Ny;
     const int B = (int)Fs.size();
     if (B < 1) return fail(h, FDFD_ERR_INVALID, "no factor to build");
+    if (B > MAX_BUILD)
+        return fail(h, FDFD_ERR_INVALID, "at most " + std::to_string(MAX_BUILD) + " factors per build");
     const int prec = Fs[0]->precision;
     for (Factor* F : Fs) {
         if (F->precision != prec)
@@ -243,7 +253,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                     J.piv = h->piv + c * VMAX;
                     J.Z = (char*)h->Zb + (size_t)c * VMAX * n * elem;
                     J.win = h->win + c * WIN;
-                    J.verr = h->verr;
+                    J.verr = h->verr + f;  // the factor's own verification flag
                     J.pivstat = F->pivstat + 3 * (long)j;
                     J.n = n;
                     ljs.push_back(J);
@@ -418,31 +428,76 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
     // Pass 0: windowed pivot search with verification.  Pass 1 (only if a
-    // row outside a window would have won a GEPP step): full-column search.
+    // row outside a window would have won a GEPP step, or a window pivot was
+    // zero): full-column search, for the factors that failed.  When only some
+    // of a batch failed, those are rebuilt on their own (fresh factors swapped
+    // in), so the others keep their pass-0 result.
     int used_full = 0;
+    std::vector<int> redo;           // factors rebuilt by the nested full-search build
+    float ms = 0, ms_redo = 0;
     for (int pass = 0; pass < 2; ++pass) {
         const bool full = (pass == 1) || h->full_pivot_search;
         used_full = full ? 1 : 0;
-        HIPCHK(h, hipMemsetAsync(h->verr, 0, sizeof(int), h->stream));
+        HIPCHK(h, hipMemsetAsync(h->verr, 0, sizeof(int) * B, h->stream));
         HIPCHK(h, hipEventRecord(e0, h->stream));
         launch_all(full, h->prof.enabled);
         HIPCHK(h, hipGetLastError());
         HIPCHK(h, hipEventRecord(e1, h->stream));
         HIPCHK(h, hipEventSynchronize(e1));
-        int verr = 0;
-        COPY(h, &verr, h->verr, sizeof(int), hipMemcpyDeviceToHost);
-        if (pass == 0 && inject == FDFD_INJECT_PIVOT_FALLBACK) verr = 1;
-        if (full || !verr) break;
+        std::vector<int> verr(B, 0);
+        COPY(h, verr.data(), h->verr, sizeof(int) * B, hipMemcpyDeviceToHost);
+        if (pass == 0 && inject == FDFD_INJECT_PIVOT_FALLBACK) verr[0] = 1;
+        std::vector<int> bad;
+        for (int f = 0; f < B; ++f)
+            if (verr[f]) bad.push_back(f);
+        if (full || bad.empty()) break;
+        if ((int)bad.size() < B) {
+            HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+            std::vector<Factor*> sub;
+            for (int f : bad) {
+                Factor* G = new Factor();
+                G->pol = Fs[f]->pol;
+                G->omega = Fs[f]->omega;
+                G->omega_pml = Fs[f]->omega_pml;
+                G->shift = Fs[f]->shift;
+                G->form = Fs[f]->form;
+                G->precision = Fs[f]->precision;
+                sub.push_back(G);
+            }
+            const bool was_full = h->full_pivot_search;
+            h->full_pivot_search = true;
+            rc = rt_build_factors(h, sub);
+            h->full_pivot_search = was_full;
+            if (rc) {
+                for (Factor* G : sub) rt_free_factor(h, G);
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
+            }
+            ms_redo = sub[0]->t_factor_ms;
+            for (size_t q = 0; q < bad.size(); ++q) {
+                Factor* F = Fs[bad[q]];
+                const int id = F->id;
+                rt_release_factor_storage(h, F);
+                *F = *sub[q];           // takes over the rebuilt factor's storage
+                F->id = id;
+                F->pivot_fallback = 1;
+                delete sub[q];          // the struct only: its storage now belongs to F
+                redo.push_back(bad[q]);
+            }
+            break;
+        }
     }
-    float ms = 0;
-    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    if (redo.empty()) HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    ms += ms_redo;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    auto rebuilt = [&](int f) { return std::find(redo.begin(), redo.end(), f) != redo.end(); };
     int first_singular = -1, singular_factor = -1, nsingular = 0;
     for (int fi = 0; fi < B; ++fi) {
         Factor* F = Fs[fi];
         F->t_factor_ms = ms;
-        F->pivot_fallback = used_full;
+        if (!rebuilt(fi)) F->pivot_fallback = used_full;
         std::vector<double> ps(3 * (size_t)m);
         COPY(h, ps.data(), F->pivstat, sizeof(double) * ps.size(), hipMemcpyDeviceToHost);
         double mn = std::numeric_limits<double>::infinity(), mx = 0;
@@ -477,8 +532,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
         return fail(h, FDFD_ERR_SINGULAR, msg);
     }
 
-    // --- reduced-solve program of each factor
-    for (Factor* F : Fs) {
+    // --- reduced-solve program of each factor (the rebuilt ones have theirs)
+    for (int fi = 0; fi < B; ++fi) {
+        if (rebuilt(fi)) continue;
+        Factor* F = Fs[fi];
         std::vector<MvJob> mvs;
         std::vector<DiagJob> dgs;
         auto vec = [&](int v, int line) {

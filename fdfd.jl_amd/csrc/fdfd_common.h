@@ -271,6 +271,15 @@ void launch_vmode_fields(hipStream_t st, double2* out, const double2* h, const d
 void launch_poynting(hipStream_t s, double* S, const double2* f, int Nx, int Ny, int pol);
 void launch_flux(hipStream_t s, double* out, const double2* f, const int4* planes, int nplanes,
                  int Nx, int Ny, int pol, double dx, double dy);
+// matrix-free stencil of a driven (form 0, TM or TE) or sideband (form 1,
+// TM) operator: weights from the 1-D inverse stretch vectors and eps
+struct StencilOp {
+    int pol, form, Nx, Ny;
+    const double2 *eps, *sxf, *sxb, *syf, *syb;
+    double ry, kdiag;
+};
+void launch_stencil(hipStream_t s, double2* y, const double2* x, const StencilOp& op,
+                    int transpose, const double2* b);
 void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef,
                   int Nx, int Ny, int transpose, const double2* b /*nullable: r=b-Ax*/);
 void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n);

@@ -185,6 +185,150 @@ void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* co
     k_apply<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(y, x, coef, Nx, Ny, transpose, b);
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free form of the same operator (driven TM/TE, form 0, and the TM
+// sideband form 1): the stencil weights are formed in registers from the 1-D
+// inverse stretch vectors (a few KB, cache-resident) and eps, so the kernel
+// streams x, eps, b and y only (64 B per cell) instead of five coefficient
+// planes.  A workgroup owns a 64 x 16 tile: x (and eps for TE, whose weights
+// need the neighbours' averages) is staged with a one-cell periodic halo in
+// LDS, each of the 256 lanes computes one column of four cells, and the
+// loads and stores of a tile row are contiguous.  The weights are the ones
+// k_coeffs writes, in the same operation order:
+//   A'x  at (i,j) = cW x_W + cE x_E + cS x_S + cN x_N + cC x_C
+//   A'^T x at (i,j) = cE(i-1) x_W + cW(i+1) x_E + cN(j-1) x_S + cS(j+1) x_N + cC x_C
+// ---------------------------------------------------------------------------
+constexpr int ST_TX = 64, ST_TY = 16;
+
+namespace {
+struct StW { double2 w, e, s, n, c; };
+
+// weights of cell (i, j); ev(di, dj) returns eps at (i+di, j+dj) (TE only)
+template <int POL, int FORM, typename EV>
+__device__ inline StW st_weights(int i, int j, int Nx, int Ny, const double2* sxf,
+                                 const double2* sxb, const double2* syf, const double2* syb,
+                                 double ry, double kdiag, EV ev) {
+    const int ip = (i + 1 == Nx) ? 0 : i + 1, im = (i == 0) ? Nx - 1 : i - 1;
+    const int jp = (j + 1 == Ny) ? 0 : j + 1, jm = (j == 0) ? Ny - 1 : j - 1;
+    const double2 ryc = make_double2(ry, 0.0);
+    StW W;
+    if (POL == 1 && FORM == 1) {
+        W.e = cmul(sxb[i], sxf[i]);
+        W.w = cmul(sxb[i], sxf[im]);
+        W.n = cmul(ryc, cmul(syb[j], syf[j]));
+        W.s = cmul(ryc, cmul(syb[j], syf[jm]));
+        W.c = cadd(cneg(cadd(cadd(W.e, W.w), cadd(W.n, W.s))), cscale(ev(0, 0), kdiag));
+    } else if (POL == 1) {
+        W.e = cmul(sxf[i], sxb[ip]);
+        W.w = cmul(sxf[i], sxb[i]);
+        W.n = cmul(ryc, cmul(syf[j], syb[jp]));
+        W.s = cmul(ryc, cmul(syf[j], syb[j]));
+        W.c = cadd(cneg(cadd(cadd(W.e, W.w), cadd(W.n, W.s))), cscale(ev(0, 0), kdiag));
+    } else {
+        const double2 e0 = ev(0, 0);
+        const double2 two = make_double2(2.0, 0.0);
+        const double2 tx_i = cdiv(two, cadd(e0, ev(-1, 0)));
+        const double2 tx_ip = cdiv(two, cadd(ev(1, 0), e0));
+        const double2 ty_j = cdiv(two, cadd(e0, ev(0, -1)));
+        const double2 ty_jp = cdiv(two, cadd(ev(0, 1), e0));
+        W.e = cmul(sxf[i], cmul(tx_ip, sxb[ip]));
+        W.w = cmul(sxf[i], cmul(tx_i, sxb[i]));
+        W.n = cmul(ryc, cmul(syf[j], cmul(ty_jp, syb[jp])));
+        W.s = cmul(ryc, cmul(syf[j], cmul(ty_j, syb[j])));
+        W.c = cadd(cneg(cadd(cadd(W.e, W.w), cadd(W.n, W.s))), make_double2(kdiag, 0.0));
+    }
+    return W;
+}
+}  // namespace
+
+template <int POL, int FORM, bool TRANS>
+__global__ __launch_bounds__(256) void k_stencil(double2* __restrict__ y, const double2* __restrict__ x,
+                                                 const double2* __restrict__ eps,
+                                                 const double2* sxf, const double2* sxb,
+                                                 const double2* syf, const double2* syb, int Nx,
+                                                 int Ny, double ry, double kdiag,
+                                                 const double2* __restrict__ b) {
+    // TE weights (and every transposed weight) need eps one or two cells away
+    constexpr int EH = (POL == 2) ? (TRANS ? 2 : 1) : 0;
+    __shared__ double2 xs[ST_TY + 2][ST_TX + 2];
+    __shared__ double2 es[(POL == 2) ? ST_TY + 2 * EH : 1][(POL == 2) ? ST_TX + 2 * EH : 1];
+    const int i0 = blockIdx.x * ST_TX, j0 = blockIdx.y * ST_TY;
+    const int t = threadIdx.x;
+    auto wrapx = [&](int v) { v %= Nx; return v < 0 ? v + Nx : v; };
+    auto wrapy = [&](int v) { v %= Ny; return v < 0 ? v + Ny : v; };
+    for (int q = t; q < (ST_TY + 2) * (ST_TX + 2); q += 256) {
+        const int r = q / (ST_TX + 2), c = q % (ST_TX + 2);
+        const int gi = wrapx(i0 - 1 + c), gj = wrapy(j0 - 1 + r);
+        xs[r][c] = x[(long)gj * Nx + gi];
+    }
+    if (POL == 2) {
+        constexpr int EW = ST_TX + 2 * EH, EHt = ST_TY + 2 * EH;
+        for (int q = t; q < EHt * EW; q += 256) {
+            const int r = q / EW, c = q % EW;
+            const int gi = wrapx(i0 - EH + c), gj = wrapy(j0 - EH + r);
+            es[r][c] = eps[(long)gj * Nx + gi];
+        }
+    }
+    __syncthreads();
+    const int lx = t & (ST_TX - 1), ly0 = t >> 6;
+    const int i = i0 + lx;
+    if (i >= Nx) return;
+#pragma unroll
+    for (int q = 0; q < ST_TY / 4; ++q) {
+        const int ly = ly0 + 4 * q, j = j0 + ly;
+        if (j >= Ny) break;
+        const long k = (long)j * Nx + i;
+        // eps at (i + di, j + dj) relative to the weight's cell (ci, cj)
+        auto weights = [&](int ci, int cj, int oi, int oj) {
+            auto ev = [&](int di, int dj) -> double2 {
+                if (POL == 2) return es[ly + EH + oj + dj][lx + EH + oi + di];
+                return eps[k];
+            };
+            return st_weights<POL, FORM>(ci, cj, Nx, Ny, sxf, sxb, syf, syb, ry, kdiag, ev);
+        };
+        const double2 xc = xs[ly + 1][lx + 1];
+        const double2 xw = xs[ly + 1][lx], xe = xs[ly + 1][lx + 2];
+        const double2 xsn = xs[ly][lx + 1], xn = xs[ly + 2][lx + 1];
+        double2 acc;
+        const StW W = weights(i, j, 0, 0);
+        if (!TRANS) {
+            acc = cmul(W.c, xc);
+            acc = cadd(acc, cmul(W.w, xw));
+            acc = cadd(acc, cmul(W.e, xe));
+            acc = cadd(acc, cmul(W.s, xsn));
+            acc = cadd(acc, cmul(W.n, xn));
+        } else {
+            const int im = (i == 0) ? Nx - 1 : i - 1, ip = (i + 1 == Nx) ? 0 : i + 1;
+            const int jm = (j == 0) ? Ny - 1 : j - 1, jp = (j + 1 == Ny) ? 0 : j + 1;
+            acc = cmul(W.c, xc);
+            acc = cadd(acc, cmul(weights(im, j, -1, 0).e, xw));
+            acc = cadd(acc, cmul(weights(ip, j, 1, 0).w, xe));
+            acc = cadd(acc, cmul(weights(i, jm, 0, -1).n, xsn));
+            acc = cadd(acc, cmul(weights(i, jp, 0, 1).s, xn));
+        }
+        y[k] = b ? csub(b[k], acc) : acc;
+    }
+}
+
+void launch_stencil(hipStream_t s, double2* y, const double2* x, const StencilOp& op,
+                    int transpose, const double2* b) {
+    const dim3 grid((op.Nx + ST_TX - 1) / ST_TX, (op.Ny + ST_TY - 1) / ST_TY);
+#define ST_LAUNCH(P, F, T)                                                                     \
+    k_stencil<P, F, T><<<grid, 256, 0, s>>>(y, x, op.eps, op.sxf, op.sxb, op.syf, op.syb, op.Nx, \
+                                            op.Ny, op.ry, op.kdiag, b)
+    if (op.form == 1) {
+        if (transpose) ST_LAUNCH(1, 1, true);
+        else ST_LAUNCH(1, 1, false);
+    } else if (op.pol == 1) {
+        if (transpose) ST_LAUNCH(1, 0, true);
+        else ST_LAUNCH(1, 0, false);
+    } else {
+        if (transpose) ST_LAUNCH(2, 0, true);
+        else ST_LAUNCH(2, 0, false);
+    }
+#undef ST_LAUNCH
+}
+
 // y = a * x
 __global__ void k_scale_copy(double2* y, const double2* x, double2 a, long n) {
     long k = blockIdx.x * (long)blockDim.x + threadIdx.x;

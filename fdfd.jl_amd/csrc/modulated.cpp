@@ -115,8 +115,7 @@ extern "C" int fdfd_modulated_solve(fdfd_handle* hh, double omega, double Omega,
 
     auto apply = [&](const double2* xx, double2* yy, const double2* bb) -> int {
         for (int j = 0; j < nf; ++j)
-            launch_apply(h->stream, yy + (long)j * N, xx + (long)j * N, Fs[j]->coef, g.Nx, g.Ny, 0,
-                         bb ? bb + (long)j * N : nullptr);
+            rt_apply(h, Fs[j], yy + (long)j * N, xx + (long)j * N, 0, bb ? bb + (long)j * N : nullptr);
         if (K > 0) launch_mf_couple(h->stream, yy, xx, d_deps, cw, nf, N, bb ? -1.0 : 1.0);
         return FDFD_OK;
     };

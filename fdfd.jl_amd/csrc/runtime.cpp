@@ -78,7 +78,7 @@ int ensure_factor_work(Handle* h, int n, int nch) {
     HIPCHK(h, hipMalloc(&h->used, sizeof(int) * nch * n));
     HIPCHK(h, hipMalloc(&h->piv, sizeof(int) * nch * NBmax));
     HIPCHK(h, hipMalloc(&h->win, sizeof(int) * nch * WIN));
-    HIPCHK(h, hipMalloc(&h->verr, sizeof(int)));
+    HIPCHK(h, hipMalloc(&h->verr, sizeof(int) * MAX_BUILD));  // one flag per factor built together
     HIPCHK(h, hipMalloc(&h->Vbuf, sizeof(double2) * nch * (size_t)n * NBmax));
     HIPCHK(h, hipMalloc(&h->LU, sizeof(double2) * nch * 2 * PMAX * PMAX));
     HIPCHK(h, hipMalloc(&h->Zb, sizeof(double2) * nch * (size_t)NBmax * n));
@@ -115,6 +115,13 @@ int ensure_solve_work(Handle* h, long N) {
 }
 
 void* arena_take(Handle* h, size_t bytes) {
+    if (h->slab) {  // batch mode: carve from the slab while it lasts
+        const size_t off = (h->slab_used + 255) & ~(size_t)255;
+        if (off + bytes <= h->slab_size) {
+            h->slab_used = off + bytes;
+            return h->slab + off;
+        }
+    }
     int best = -1;
     for (int i = 0; i < (int)h->arenas.size(); ++i)
         if (h->arenas[i].second >= bytes && (best < 0 || h->arenas[i].second < h->arenas[best].second))
@@ -136,10 +143,8 @@ void* arena_take(Handle* h, size_t bytes) {
     return p;
 }
 
-void arena_give(Handle* h, void* p, size_t bytes) {
-    if (!p) return;
-    h->arenas.push_back({p, bytes});
-    while ((int)h->arenas.size() > ARENA_POOL) {  // drop the smallest
+void arena_trim(Handle* h) {
+    while ((int)h->arenas.size() > h->arena_cap) {  // drop the smallest
         int sm = 0;
         for (int i = 1; i < (int)h->arenas.size(); ++i)
             if (h->arenas[i].second < h->arenas[sm].second) sm = i;
@@ -148,12 +153,25 @@ void arena_give(Handle* h, void* p, size_t bytes) {
     }
 }
 
+size_t arena_pooled_bytes(const Handle* h) {
+    size_t t = 0;
+    for (const auto& a : h->arenas) t += a.second;
+    return t;
+}
+
+void arena_give(Handle* h, void* p, size_t bytes) {
+    if (!p) return;
+    if (h->slab && (char*)p >= h->slab && (char*)p < h->slab + h->slab_size) return;  // slab: reset per group
+    h->arenas.push_back({p, bytes});
+    arena_trim(h);
+}
+
 void arena_clear(Handle* h) {
     for (auto& a : h->arenas) (void)hipFree(a.first);
     h->arenas.clear();
 }
 
-static void free_factor(Handle* h, Factor* F) {
+void rt_release_factor_storage(Handle* h, Factor* F) {
     if (!F) return;
     (void)hipStreamSynchronize(h->stream);  // the arena may be reused by the next factor
     dfree(F->coef); dfree(F->sxf); dfree(F->sxb); dfree(F->syf); dfree(F->syb);
@@ -162,6 +180,20 @@ static void free_factor(Handle* h, Factor* F) {
     dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
     dfree(F->mv_tab); dfree(F->dg_tab); dfree(F->trs_tab); dfree(F->triw);
     if (F->solve_exec) (void)hipGraphExecDestroy(F->solve_exec);
+    F->coef = F->sxf = F->sxb = F->syf = F->syb = nullptr;
+    F->blocks = F->blocks_b = nullptr;
+    F->prow = F->iprow = nullptr;
+    F->pivstat = nullptr;
+    F->mv_tab = nullptr;
+    F->dg_tab = nullptr;
+    F->trs_tab = nullptr;
+    F->triw = nullptr;
+    F->solve_exec = nullptr;
+}
+
+static void free_factor(Handle* h, Factor* F) {
+    if (!F) return;
+    rt_release_factor_storage(h, F);
     delete F;
 }
 
@@ -187,7 +219,7 @@ int rt_assemble(Handle* h, Factor* F) {
         launch_sfactor(h->stream, F->sxf, F->sxb, F->syf, F->syb, g0, F->omega, eps0n, eta0);
         F->scale = g.dx * g.dx;
         const double k0sq = F->omega * F->omega * mu0n * eps0n;
-        ProfScope ps(h, P_STENCIL, 0, 0);
+        ProfScope ps(h, P_ASSEMBLE, 0, 96.0 * N);
         launch_vmode_coeffs(h->stream, F->coef, h->eps, g, k0sq, F->shift, F->scale, F->form - 2);
         HIPCHK(h, hipGetLastError());
         return FDFD_OK;
@@ -202,8 +234,10 @@ int rt_assemble(Handle* h, Factor* F) {
         F->scale = eps0n * g.dx * g.dx;
         kdiag = F->omega * F->omega * mu0n * eps0n * g.dx * g.dx;
     }
+    F->kdiag = kdiag;
     {
-        ProfScope ps(h, P_STENCIL, 0, 0);
+        // reads eps (16 B/cell), writes the five coefficient planes (80 B/cell)
+        ProfScope ps(h, P_ASSEMBLE, 0, 96.0 * N);
         launch_coeffs(h->stream, F->coef, h->eps, F->sxf, F->sxb, F->syf, F->syb, g, F->pol,
                       kdiag, F->form);
     }
@@ -211,6 +245,18 @@ int rt_assemble(Handle* h, Factor* F) {
     return FDFD_OK;
 }
 
+
+void rt_apply(Handle* h, const Factor* F, double2* y, const double2* x, int transpose,
+              const double2* b) {
+    const GridDesc& g = h->g;
+    if (F->form == 0 || F->form == 1) {
+        const StencilOp op{F->pol, F->form, g.Nx, g.Ny, h->eps, F->sxf, F->sxb, F->syf, F->syb,
+                           (g.dx * g.dx) / (g.dy * g.dy), F->kdiag};
+        launch_stencil(h->stream, y, x, op, transpose, b);
+    } else {
+        launch_apply(h->stream, y, x, F->coef, g.Nx, g.Ny, transpose, b);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // FP64 Krylov building blocks
@@ -358,8 +404,9 @@ static int gmres(Handle* h, Factor* F, int trans, const double2* b, double2* x, 
     const GridDesc& g = h->g;
     const long N = (long)g.Nx * g.Ny;
     auto apply = [&](const double2* xx, double2* yy, const double2* bb) -> int {
-        ProfScope ps(h, P_STENCIL, 0, 64.0 * N);
-        launch_apply(h->stream, yy, xx, F->coef, g.Nx, g.Ny, trans, bb);
+        // x, eps, y (and b) at 16 B per cell
+        ProfScope ps(h, P_STENCIL, 0, (bb ? 64.0 : 48.0) * N);
+        rt_apply(h, F, yy, xx, trans, bb);
         return FDFD_OK;
     };
     auto prec = [&](const double2* v, double2* z) -> int { return rt_prec_apply(h, F, trans, v, z); };
@@ -473,7 +520,9 @@ int fdfd_set_grid(fdfd_handle* hh, int Nx, int Ny, double dx, double dy, int npm
     h->has_grid = true;
     h->has_eps = false;
     const long N = (long)Nx * Ny;
+    h->eps_n = 0;
     HIPCHK(h, hipMalloc(&h->eps, sizeof(double2) * N));
+    h->eps_n = N;
     return FDFD_OK;
 }
 
@@ -671,8 +720,7 @@ int fdfd_apply(fdfd_handle* hh, int factor_id, int transpose, const fdfd_complex
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
     COPY(h, h->tmp, x, sizeof(double2) * N, hipMemcpyHostToDevice);
-    launch_apply(h->stream, h->tmp2, h->tmp, F->coef, h->g.Nx, h->g.Ny, transpose ? 1 : 0,
-                 nullptr);
+    rt_apply(h, F, h->tmp2, h->tmp, transpose ? 1 : 0, nullptr);
     launch_scale_copy(h->stream, h->tmp2, h->tmp2, make_double2(1.0 / F->scale, 0), N);
     HIPCHK(h, hipMemcpyAsync(y, h->tmp2, sizeof(double2) * N, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -693,7 +741,7 @@ int fdfd_residual(fdfd_handle* hh, int factor_id, const fdfd_complex* x, const f
     double bn = 0, rn = 0;
     int rc = dev_norm(h, h->tmp2, N, &bn);
     if (rc) return rc;
-    launch_apply(h->stream, h->bvec, h->tmp, F->coef, h->g.Nx, h->g.Ny, 0, h->tmp2);
+    rt_apply(h, F, h->bvec, h->tmp, 0, h->tmp2);
     if ((rc = dev_norm(h, h->bvec, N, &rn))) return rc;
     *rel = (bn > 0) ? rn / bn : rn;
     return FDFD_OK;

@@ -17,9 +17,10 @@ namespace fdfd {
 using cd = std::complex<double>;
 
 // per-kernel profile slots: Gauss-Jordan trailing update, panel, block
-// formation, reduced-solve steps, stencil/residual, Z build, complex GEMM
+// formation, reduced-solve steps, stencil/residual, Z build, complex GEMM,
+// operator assembly
 enum ProfId { P_TRAIL = 0, P_PANEL = 1, P_SCHUR = 2, P_SWEEP = 3, P_STENCIL = 4, P_ZBUILD = 5,
-              P_GEMM = 6, P_COUNT = 7 };
+              P_GEMM = 6, P_ASSEMBLE = 7, P_COUNT = 8 };
 
 struct Prof {
     bool enabled = false;
@@ -48,6 +49,7 @@ struct Factor {
     int n = 0, m = 0, orient = 0;  // line length, number of lines, 0: lines along x
     long si = 1, sj = 1;           // stride of a line's cells / of consecutive lines
     double scale = 1;              // A' = scale * A
+    double kdiag = 0;              // diagonal term of A' (times eps for TM), see k_coeffs
     double2* coef = nullptr;       // 5 planes of N
     double2 *sxf = nullptr, *sxb = nullptr, *syf = nullptr, *syb = nullptr;
     size_t elem = 16;              // bytes per complex factor entry
@@ -109,6 +111,12 @@ struct Handle {
     // released factor arenas kept for reuse (at most ARENA_POOL), freed by
     // fdfd_set_grid and fdfd_destroy
     std::vector<std::pair<void*, size_t>> arenas;
+    int arena_cap = 2;   // pool capacity (ARENA_POOL; raised while a batch runs)
+    // while a batch runs, factor arenas are carved from one slab allocated for
+    // the batch's largest group (bump pointer, reset per group)
+    char* slab = nullptr;
+    size_t slab_size = 0, slab_used = 0;
+    long eps_n = 0;      // cells the eps buffer holds
 };
 
 // error helper: record message, return code
@@ -164,15 +172,24 @@ struct ProfScope {
 // factor arenas: take one of at least `bytes` from the pool or allocate;
 // give one back (the pool keeps the largest ARENA_POOL); free them all
 constexpr int ARENA_POOL = 2;
+constexpr int MAX_BUILD = 128;   // factors one rt_build_factors call builds together
+void arena_trim(Handle* h);      // drop the smallest arenas beyond arena_cap
+size_t arena_pooled_bytes(const Handle* h);
 void* arena_take(Handle* h, size_t bytes);
 void arena_give(Handle* h, void* p, size_t bytes);
 void arena_clear(Handle* h);
 void rt_free_factor(Handle* h, Factor* F);
+void rt_release_factor_storage(Handle* h, Factor* F);  // device storage only, F stays
 
 // work buffers of the handle (grown on demand)
 int ensure_factor_work(Handle* h, int n, int nch);
 int ensure_solve_work(Handle* h, long N);
 int rt_flux_scratch_setup(Handle* h);
+// y = A' x, A'^T x (transpose) or b - A' x (b != nullptr) for factor F's
+// operator: the matrix-free stencil for driven and sideband operators, the
+// coefficient planes for the full-vector mode blocks
+void rt_apply(Handle* h, const Factor* F, double2* y, const double2* x, int transpose,
+              const double2* b);
 
 // Right-preconditioned FGMRES(30) in FP64 on an operator given by callbacks:
 //   apply(x, y, b): y = b - A x if b != nullptr, else y = A x
@@ -190,6 +207,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs);
 int rt_build_factor(Handle* h, Factor* F);
 int rt_prec_apply(Handle* h, Factor* F, int trans, const double2* in, double2* out);
 size_t rt_factor_bytes(int n, int m, int precision, bool b3);
+void rt_factor_arena_sizes(int n, int m, int precision, bool b3, size_t* blocks, size_t* split);
 
 }  // namespace fdfd
 

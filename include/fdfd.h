@@ -337,7 +337,7 @@ int fdfd_batch_destroy(fdfd_batch* b);
 /* Per-kernel device time accumulated with HIP events on the handle's stream
  * while profiling is enabled.  kernel: 0 = trailing GJE update, 1 = panel,
  * 2 = Schur formation, 3 = sweep GEMV, 4 = stencil/residual, 5 = Z build,
- * 6 = complex GEMM. */
+ * 6 = complex GEMM, 7 = operator assembly. */
 int fdfd_profile_enable(fdfd_handle* h, int enable);
 int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
                       long long* launches, double* flops, double* bytes);
@@ -349,9 +349,11 @@ int fdfd_set_tuning(fdfd_handle* h, int panel_width, int force_full_search);
  * reference, whose only failure handling is the solver fallback of
  * solver.jl:19-28).  One-shot: the next operation of the named kind on the
  * handle fails as stated, then the handle behaves normally.
- *   FDFD_INJECT_PIVOT_FALLBACK  next factorization: the windowed pivot pass is
- *                               treated as failing its verification, so the
- *                               full-column search runs (stats.pivot_fallback)
+ *   FDFD_INJECT_PIVOT_FALLBACK  next factorization: the windowed pivot pass of
+ *                               its first factor (of a batched build: the
+ *                               first frequency) is treated as failing its
+ *                               verification, so that factor is redone with
+ *                               the full-column search (stats.pivot_fallback)
  *   FDFD_INJECT_SINGULAR        next factorization reports a zero pivot in
  *                               line block `arg` (FDFD_ERR_SINGULAR)
  *   FDFD_INJECT_NOCONV          next refined solve runs at most `arg` FGMRES

@@ -63,6 +63,25 @@ def test_injected_pivot_fallback_redoes_factorization(engine, small, precision):
     assert st["pivot_fallback"] == 0
 
 
+def test_injected_fallback_rebuilds_one_factor_of_a_batch(engine, small):
+    """In a batched build only the factor that failed its window verification
+    is redone (nested full-search build swapped in); all results still match
+    the oracle and the batch reports the fallback."""
+    d, om, ref = small
+    oms = np.array([O.omega_of_wavelength(l) for l in (1.50, 1.55, 1.60)])
+    srcs = d.src[:, :, None]
+    engine.debug_inject(FDFD_INJECT_PIVOT_FALLBACK)
+    res, st = engine.batch_solve([(_cgrid(d.grid), O.TM, d.eps_r, oms, srcs)],
+                                 precision="mixed", tol=1e-12)
+    assert st["pivot_fallback"] == 1 and st["residual"] <= 1e-12
+    A0 = None
+    for o, w in enumerate(oms):
+        A0 = O.assemble(d.grid, d.eps_r, float(w), O.TM)
+        x, _ = O.linear_solve(A0, O.rhs(float(w), d.src))
+        ez = res[0][o, 0, :, :, 0].ravel(order="F")
+        assert np.linalg.norm(ez - x) / np.linalg.norm(x) <= 1e-10, o
+
+
 def test_injected_singular_pivot(engine, small):
     d, om, ref = small
     _setup(engine, d)
