@@ -19,7 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfdfd.so")
 ARCH = "gfx950"
 
-SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "k_mode.hip", "runtime.cpp", "bcr.cpp",
+SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "k_mode.hip", "k_multi.hip",
+           "runtime.cpp", "bcr.cpp", "multi.cpp",
            "eigen.cpp", "dense_eig.cpp", "batch.cpp", "mode.cpp", "modulated.cpp",
            "flux.cpp", "arnoldi.cpp", "vmode.cpp"]
 

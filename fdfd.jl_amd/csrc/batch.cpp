@@ -128,6 +128,18 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
             return fail(h, FDFD_ERR_OOM, "batch: result buffer allocation failed");
         }
     }
+    {
+        long Nmax = 0;
+        int kmax = 1;
+        for (const auto& job : b->jobs) {
+            Nmax = std::max(Nmax, (long)job.g.Nx * job.g.Ny);
+            kmax = std::max(kmax, std::min(job.nsrc, KMULTI));
+        }
+        if ((rc = ensure_solve_work(h, Nmax)) || (rc = ensure_multi_work(h, Nmax, kmax))) {
+            free_results(b);
+            return rc;
+        }
+    }
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     std::vector<int> group(b->jobs.size(), 1);
@@ -217,20 +229,25 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 const double eps0n = EPS0 * g.L0, mu0n = MU0 * g.L0;
                 for (int q = 0; q < nb && rc == FDFD_OK; ++q) {
                     Factor* F = Fs[q];
-                    for (int sidx = 0; sidx < job.nsrc && rc == FDFD_OK; ++sidx) {
-                        // b' = scale * (i omega) * src   (driven.jl:36,46)
-                        launch_scale_copy(h->stream, h->bvec, dsrc + (long)sidx * N,
-                                          make_double2(0.0, F->scale * F->omega), N);
-                        launch_zero(h->stream, h->xvec, N);
-                        double res = 0;
-                        rc = rt_gmres(h, F, 0, h->bvec, h->xvec, tol, &iters, &res);
-                        worst = std::max(worst, res);
+                    const double c = (F->pol == FDFD_TM) ? mu0n : eps0n;
+                    const double2 w = make_double2(0.0, -1.0 / (F->omega * c));
+                    // the sources of this frequency, KMULTI at a time, in one
+                    // lockstep FGMRES on the factor
+                    for (int s0 = 0; s0 < job.nsrc && rc == FDFD_OK; s0 += KMULTI) {
+                        const int ns = std::min(KMULTI, job.nsrc - s0);
+                        for (int k = 0; k < ns; ++k) {
+                            // b' = scale * (i omega) * src   (driven.jl:36,46)
+                            launch_scale_copy(h->stream, h->mB + (long)k * N, dsrc + (long)(s0 + k) * N,
+                                              make_double2(0.0, F->scale * F->omega), N);
+                            launch_zero(h->stream, h->mX + (long)k * N, N);
+                        }
+                        rc = rt_gmres_multi(h, F, ns, h->mB, h->mX, tol, &iters, &worst);
                         if (rc) break;
-                        const double c = (F->pol == FDFD_TM) ? mu0n : eps0n;
-                        double2 w = make_double2(0.0, -1.0 / (F->omega * c));
-                        double2* out = job.result + ((o0 + q) * job.nsrc + sidx) * 3 * N;
-                        launch_fields(h->stream, out, h->xvec, h->eps, F->sxf, F->sxb, F->syf,
-                                      F->syb, g, F->pol, w, 0);
+                        for (int k = 0; k < ns; ++k) {
+                            double2* out = job.result + ((o0 + q) * job.nsrc + s0 + k) * 3 * N;
+                            launch_fields(h->stream, out, h->mX + (long)k * N, h->eps, F->sxf,
+                                          F->sxb, F->syf, F->syb, g, F->pol, w, 0);
+                        }
                     }
                 }
                 (void)hipEventRecord(e1, h->stream);

@@ -279,7 +279,29 @@ struct StencilOp {
     double ry, kdiag;
 };
 void launch_stencil(hipStream_t s, double2* y, const double2* x, const StencilOp& op,
-                    int transpose, const double2* b);
+                    int transpose, const double2* b, long rs = 0, const int* ridx = nullptr,
+                    int na = 1);
+// several right-hand sides per launch (k_multi.hip): sources ridx[0..na) of
+// buffers whose source r starts at r * rs
+template <typename R> int mv_multi_prepare(int n);
+template <typename R>
+void launch_mv_multi(hipStream_t s, const MvJob* jobs_d, int njobs, int n, long rs, const int* ridx,
+                     int na);
+void launch_diag_multi(hipStream_t s, const DiagJob* jobs_d, int njobs, int n, long rs,
+                       const int* ridx, int na);
+int trisolve_multi_prepare(int n);
+void launch_trisolve_multi(hipStream_t s, const TrsJob* jobs_d, int njobs, int n, long rs,
+                           const int* ridx, int na);
+void launch_multidot_b(hipStream_t s, const double2* V, long ldv, long rs, int k, const double2* w,
+                       long wrs, long n, double2* partial, double2* out, int nblk,
+                       const int* ridx, int na);
+void launch_gemv_update_b(hipStream_t s, double2* w, long wrs, const double2* V, long ldv, long rs,
+                          const int* kcnt, const double2* h, int kstride, double sign, long n,
+                          const int* ridx, int na);
+void launch_norm2_b(hipStream_t s, const double2* x, long rs, long n, double* partial, double* out,
+                    int nblk, const int* ridx, int na);
+void launch_scale_b(hipStream_t s, double2* y, long yrs, const double2* x, long xrs, const double2* a,
+                    long n, const int* ridx, int na);
 void launch_apply(hipStream_t s, double2* y, const double2* x, const double2* coef,
                   int Nx, int Ny, int transpose, const double2* b /*nullable: r=b-Ax*/);
 void launch_scale_copy(hipStream_t s, double2* y, const double2* x, double2 a, long n);

@@ -1,0 +1,462 @@
+// Several right-hand sides per launch: the reduced-solve steps and the FP64
+// Krylov building blocks of a lockstep multi-RHS FGMRES.
+//
+// The reference solves one source per solve() call and refactors per
+// frequency (src/solver/driven.jl:11-55).  Here the sources that share a
+// factor are solved together: every step of the reduced solve streams a
+// factor block once for a chunk of KC right-hand sides (instead of once per
+// source), and each Arnoldi step does its dot products, updates and norms
+// for all sources in one launch and one host round trip.
+//
+// Vector layout: right-hand side r of a buffer starts at r * rs (rs >= N);
+// only the sources listed in `ridx` (na of them) are touched, so converged
+// sources drop out of the launches.
+#include <hip/hip_runtime.h>
+
+#include "fdfd_common.h"
+#include "k_device.h"
+#include "tri_qr.h"
+
+namespace fdfd {
+
+// ---------------------------------------------------------------------------
+// reduced solve, dense matrix-vector step for KC right-hand sides at once:
+// out_r[omap(i)] = [base_r] - sum_t M_t[i, :] x_{t,r}[imap_t(:)]
+// Each workgroup owns a row range of one job and one chunk of KC sources;
+// the sources' gathered inputs sit in LDS, and every matrix entry loaded is
+// used for all KC of them.
+// ---------------------------------------------------------------------------
+template <typename R, int KC>
+__global__ __launch_bounds__(256) void k_mv_multi(const MvJob* jobs, int n, long rs,
+                                                  const int* ridx, int na) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char mv_lds[];
+    C* w = (C*)mv_lds;  // [term][KC][n]
+    const MvJob J = jobs[blockIdx.y];
+    const int q0 = blockIdx.z * KC;
+    const int kc = min(KC, na - q0);
+    const long si = J.si;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        for (int t = 0; t < J.nt; ++t) {
+            const int s = J.imap[t] ? J.imap[t][c] : c;
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                if (q < kc) {
+                    const long r = ridx[q0 + q];
+                    w[((long)t * KC + q) * n + c] = from_d<C>(J.x[t][(long)s * si + r * rs]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rows_per_block = (n + gridDim.x - 1) / gridDim.x;
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = min(n, r0 + rows_per_block);
+    for (int row = r0 + 2 * wv; row < r1; row += 8) {
+        const bool two = (row + 1) < r1;
+        const int rb = two ? row + 1 : row;
+        R ar[KC], ai[KC], br[KC], bi[KC];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) ar[q] = ai[q] = br[q] = bi[q] = R(0);
+        for (int t = 0; t < J.nt; ++t) {
+            const C* Ma = (const C*)J.M[t] + (long)row * n;
+            const C* Mb = (const C*)J.M[t] + (long)rb * n;
+            const C* wt = w + (long)t * KC * n;
+            for (int c = lane; c < n; c += 64) {
+                const C a = gload(Ma + c), b = gload(Mb + c);
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const C x = wt[(long)q * n + c];
+                    ar[q] += a.x * x.x - a.y * x.y;
+                    ai[q] += a.x * x.y + a.y * x.x;
+                    br[q] += b.x * x.x - b.y * x.y;
+                    bi[q] += b.x * x.y + b.y * x.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+            for (int o = 32; o > 0; o >>= 1) {
+                ar[q] += shfl_x(ar[q], o);
+                ai[q] += shfl_x(ai[q], o);
+                br[q] += shfl_x(br[q], o);
+                bi[q] += shfl_x(bi[q], o);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < KC; ++q) {
+                if (q < kc) {
+                    const long ro = (long)ridx[q0 + q] * rs;
+                    long e = (long)(J.omap ? J.omap[row] : row) * si + ro;
+                    double2 z = make_double2((double)ar[q], (double)ai[q]);
+                    J.out[e] = J.base ? csub(J.base[e], z) : z;
+                    if (two) {
+                        e = (long)(J.omap ? J.omap[row + 1] : row + 1) * si + ro;
+                        z = make_double2((double)br[q], (double)bi[q]);
+                        J.out[e] = J.base ? csub(J.base[e], z) : z;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename R>
+static int mv_kc(int n) {
+    // LDS for two terms of KC gathered vectors: at most 128 KB
+    const int per = 2 * n * (int)sizeof(cplx<R>);
+    int kc = 8;
+    while (kc > 1 && kc * per > 128 * 1024) kc >>= 1;
+    return kc;
+}
+
+template <typename R>
+int mv_multi_prepare(int n) {
+    const int kc = mv_kc<R>(n);
+    const int lds = 2 * kc * n * (int)sizeof(cplx<R>);
+    hipError_t e = hipSuccess;
+    switch (kc) {
+        case 8: e = hipFuncSetAttribute((const void*)k_mv_multi<R, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); break;
+        case 4: e = hipFuncSetAttribute((const void*)k_mv_multi<R, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); break;
+        case 2: e = hipFuncSetAttribute((const void*)k_mv_multi<R, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); break;
+        default: e = hipFuncSetAttribute((const void*)k_mv_multi<R, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); break;
+    }
+    return (int)e;
+}
+template int mv_multi_prepare<double>(int);
+template int mv_multi_prepare<float>(int);
+
+template <typename R>
+void launch_mv_multi(hipStream_t s, const MvJob* jobs_d, int njobs, int n, long rs, const int* ridx,
+                     int na) {
+    const int kc = mv_kc<R>(n);
+    const int nch = (na + kc - 1) / kc;
+    int nblk = (4096 + njobs * nch - 1) / (njobs * nch);
+    nblk = std::max(1, std::min(nblk, (n + 7) / 8));
+    const dim3 grid(nblk, njobs, nch);
+    const size_t lds = 2 * (size_t)kc * n * sizeof(cplx<R>);
+    switch (kc) {
+        case 8: k_mv_multi<R, 8><<<grid, 256, lds, s>>>(jobs_d, n, rs, ridx, na); break;
+        case 4: k_mv_multi<R, 4><<<grid, 256, lds, s>>>(jobs_d, n, rs, ridx, na); break;
+        case 2: k_mv_multi<R, 2><<<grid, 256, lds, s>>>(jobs_d, n, rs, ridx, na); break;
+        default: k_mv_multi<R, 1><<<grid, 256, lds, s>>>(jobs_d, n, rs, ridx, na); break;
+    }
+}
+template void launch_mv_multi<double>(hipStream_t, const MvJob*, int, int, long, const int*, int);
+template void launch_mv_multi<float>(hipStream_t, const MvJob*, int, int, long, const int*, int);
+
+// first-level (diagonal) step, one source per grid row
+__global__ void k_diag_multi(const DiagJob* jobs, int n, long rs, const int* ridx) {
+    const DiagJob J = jobs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long e = (long)i * J.si;
+    const long ro = (long)ridx[blockIdx.z] * rs;
+    double2 v = J.x0[e + ro];
+    if (J.nt > 0) {
+        double2 d = J.d[0][0][e];
+        if (J.d[0][1]) d = cadd(d, J.d[0][1][e]);
+        v = csub(v, cmul(d, J.x[0][e + ro]));
+    }
+    if (J.nt > 1) {
+        double2 d = J.d[1][0][e];
+        if (J.d[1][1]) d = cadd(d, J.d[1][1][e]);
+        v = csub(v, cmul(d, J.x[1][e + ro]));
+    }
+    J.out[e + ro] = v;
+}
+
+void launch_diag_multi(hipStream_t s, const DiagJob* jobs_d, int njobs, int n, long rs,
+                       const int* ridx, int na) {
+    k_diag_multi<<<dim3((n + 255) / 256, njobs, na), 256, 0, s>>>(jobs_d, n, rs, ridx);
+}
+
+// ---------------------------------------------------------------------------
+// First-level line solves for several sources: the recurrences of k_trisolve
+// (y = Q in, then out = R^-1 y, tri_qr.h) run by lanes 0 .. kt-1 of wave 0,
+// one source each, in the same arithmetic order; wave 1 stages the
+// rotations and rows of R once for all of them (broadcast LDS reads).  Each
+// source's vector sits in LDS with a padded stride (no bank conflicts).
+// ---------------------------------------------------------------------------
+constexpr int TM_CH = 128;   // steps per staged chunk
+constexpr int TM_STEP = 10;  // doubles per step (6 rotation, 10 row of R)
+
+static int tm_kt(int n) {
+    // vectors (kt x (n + 1) complex) + staging (2 x TM_CH x TM_STEP doubles) <= 150 KB
+    const size_t stage = 2 * TM_CH * TM_STEP * sizeof(double);
+    int kt = 16;
+    while (kt > 1 && kt * (size_t)(n + 1) * sizeof(double2) + stage > 150 * 1024) --kt;
+    return kt;
+}
+static size_t tm_lds(int n, int kt) {
+    return kt * (size_t)(n + 1) * sizeof(double2) + 2 * TM_CH * TM_STEP * sizeof(double);
+}
+
+__global__ __launch_bounds__(128) void k_trisolve_multi(const TrsJob* jobs, int n, long rs,
+                                                        const int* ridx, int na, int kt) {
+    extern __shared__ __align__(16) double tmm[];
+    const int ldl = n + 1;
+    double2* vall = (double2*)tmm;
+    double* buf = tmm + 2 * (size_t)kt * ldl;
+    const TrsJob J = jobs[blockIdx.x];
+    const int q0 = blockIdx.y * kt;
+    const int nk = min(kt, na - q0);
+    const int t = threadIdx.x;
+    const bool comp = t < nk, loader = t >= 64;
+    const int ll = t - 64;
+    auto stage = [&](int b, long src0, int cnt) {
+        constexpr int PER = TM_CH * TM_STEP / 64;
+        double r[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 64 + ll;
+            r[q] = gload_s(J.w + src0 + (e < cnt ? e : 0));
+        }
+        double* d = buf + b * TM_CH * TM_STEP;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 64 + ll;
+            if (e < cnt) d[e] = r[q];
+        }
+    };
+    // gather the nk input vectors
+    for (int q = 0; q < nk; ++q) {
+        const long ro = (long)ridx[q0 + q] * rs;
+        for (int k = t; k < n; k += 128) vall[(long)q * ldl + k] = gload(J.in + (long)k * J.si + ro);
+    }
+    double2* v = vall + (long)(comp ? t : 0) * ldl;
+    using tri::cz;
+    auto ld = [&](int k) { return tri::mkz(v[k].x, v[k].y); };
+    auto st = [&](int k, cz z) { v[k] = make_double2(z.re, z.im); };
+    const int n1 = n - 1;
+    const int nc1 = (n1 + TM_CH - 1) / TM_CH;
+    if (loader && n1 > 0) stage(0, 0, tri::ROT * min(TM_CH, n1));
+    __syncthreads();
+    cz carry = tri::mkz(0, 0), tail = tri::mkz(0, 0);
+    if (comp && n > 1) {
+        carry = ld(0);
+        tail = ld(n - 1);
+    }
+    for (int c = 0; c < nc1; ++c) {
+        const int k0 = c * TM_CH, len = min(TM_CH, n1 - k0);
+        if (loader && c + 1 < nc1) {
+            const int k1 = k0 + TM_CH;
+            stage((c + 1) & 1, (long)tri::ROT * k1, tri::ROT * min(TM_CH, n1 - k1));
+        }
+        if (comp) {
+            const double* rb = buf + (c & 1) * TM_CH * TM_STEP;
+            for (int i = 0; i < len; ++i) {
+                const int k = k0 + i;
+                const double* r = rb + tri::ROT * i;
+                if (k + 2 < n) {
+                    cz nxt = ld(k + 1);
+                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
+                    tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
+                    st(k, carry);
+                    carry = nxt;
+                } else {
+                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
+                    st(n - 2, carry);
+                    st(n - 1, tail);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const long rbase = (long)tri::ROT * n;
+    const int nc2 = (n + TM_CH - 1) / TM_CH;
+    if (loader) {
+        const int lo = max(0, n - TM_CH);
+        stage(0, rbase + (long)tri::RROW * lo, tri::RROW * (n - lo));
+    }
+    __syncthreads();
+    cz x1 = tri::mkz(0, 0), x2 = tri::mkz(0, 0), xA = tri::mkz(0, 0), xB = tri::mkz(0, 0);
+    for (int c = 0; c < nc2; ++c) {
+        const int hi = n - c * TM_CH, lo = max(0, hi - TM_CH);
+        if (loader && c + 1 < nc2) {
+            const int lo2 = max(0, lo - TM_CH);
+            stage((c + 1) & 1, rbase + (long)tri::RROW * lo2, tri::RROW * (lo - lo2));
+        }
+        if (comp) {
+            const double* rb = buf + (c & 1) * TM_CH * TM_STEP;
+            for (int k = hi - 1; k >= lo; --k) {
+                const double* r = rb + tri::RROW * (k - lo);
+                const cz yk = ld(k);
+                const cz dinv = tri::mkz(r[0], r[1]);
+                if (k == n - 1) {
+                    xB = tri::zmul(yk, dinv);
+                    st(k, xB);
+                } else if (k == n - 2) {
+                    xA = tri::zmul(tri::zsub(yk, tri::zmul(tri::mkz(r[8], r[9]), xB)), dinv);
+                    st(k, xA);
+                    x1 = xA;
+                    x2 = xB;
+                } else {
+                    cz y = yk;
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
+                    y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
+                    const cz x = tri::zmul(y, dinv);
+                    st(k, x);
+                    x2 = x1;
+                    x1 = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < nk; ++q) {
+        const long ro = (long)ridx[q0 + q] * rs;
+        for (int k = t; k < n; k += 128) gstore(J.out + (long)k * J.si + ro, vall[(long)q * ldl + k]);
+    }
+}
+
+int trisolve_multi_prepare(int n) {
+    return (int)hipFuncSetAttribute((const void*)k_trisolve_multi,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)tm_lds(n, tm_kt(n)));
+}
+
+void launch_trisolve_multi(hipStream_t s, const TrsJob* jobs_d, int njobs, int n, long rs,
+                           const int* ridx, int na) {
+    const int kt = tm_kt(n);
+    const dim3 grid(njobs, (na + kt - 1) / kt);
+    k_trisolve_multi<<<grid, 128, tm_lds(n, kt), s>>>(jobs_d, n, rs, ridx, na, kt);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov building blocks over the active sources (grid row y -> source
+// ridx[y]).  Basis vector j of source r sits at V + j * ldv + r * rs.
+// ---------------------------------------------------------------------------
+constexpr int MB_MAXK = 32;
+
+// partial[y][blk][i] = sum over this block's cells of conj(V_{i,r}) w_r
+__global__ __launch_bounds__(256) void k_multidot_b(const double2* V, long ldv, long rs, int k,
+                                                    const double2* w, long wrs, long n,
+                                                    double2* partial, const int* ridx) {
+    __shared__ double2 red[4][MB_MAXK];
+    const long r = ridx[blockIdx.y];
+    const double2* Vr = V + r * rs;
+    const double2* wr = w + r * wrs;
+    double2 acc[MB_MAXK];
+#pragma unroll
+    for (int i = 0; i < MB_MAXK; ++i) acc[i] = make_double2(0, 0);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        const double2 we = wr[e];
+#pragma unroll
+        for (int i = 0; i < MB_MAXK; ++i)
+            if (i < k) acc[i] = cadd(acc[i], cconjmul(Vr[i * ldv + e], we));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < MB_MAXK; ++i) {
+        if (i < k) {
+            double re = acc[i].x, im = acc[i].y;
+            for (int o = 32; o > 0; o >>= 1) {
+                re += __shfl_xor(re, o, 64);
+                im += __shfl_xor(im, o, 64);
+            }
+            if (lane == 0) red[wv][i] = make_double2(re, im);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double2 s = red[0][threadIdx.x];
+        for (int q = 1; q < 4; ++q) s = cadd(s, red[q][threadIdx.x]);
+        partial[((long)blockIdx.y * gridDim.x + blockIdx.x) * k + threadIdx.x] = s;
+    }
+}
+
+// out[y][i] = sum_blk partial[y][blk][i], fixed order
+__global__ void k_reduce_partials_b(const double2* partial, int nblk, int k, double2* out) {
+    const int y = blockIdx.x;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        double2 s = make_double2(0, 0);
+        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[((long)y * nblk + b) * k + i]);
+        out[(long)y * k + i] = s;
+    }
+}
+
+void launch_multidot_b(hipStream_t s, const double2* V, long ldv, long rs, int k, const double2* w,
+                       long wrs, long n, double2* partial, double2* out, int nblk,
+                       const int* ridx, int na) {
+    k_multidot_b<<<dim3(nblk, na), 256, 0, s>>>(V, ldv, rs, k, w, wrs, n, partial, ridx);
+    k_reduce_partials_b<<<na, 64, 0, s>>>(partial, nblk, k, out);
+}
+
+// w_r += sign * sum_{i < kcnt[y]} h[y][i] V_{i,r}   (h has kstride entries per source)
+__global__ void k_gemv_update_b(double2* w, long wrs, const double2* V, long ldv, long rs,
+                                const int* kcnt, const double2* h, int kstride, double sign,
+                                long n, const int* ridx) {
+    __shared__ double2 hs[64];
+    const int y = blockIdx.y;
+    const long r = ridx[y];
+    const int k = kcnt[y];
+    if (threadIdx.x < k) hs[threadIdx.x] = cscale(h[(long)y * kstride + threadIdx.x], sign);
+    __syncthreads();
+    double2* wr = w + r * wrs;
+    const double2* Vr = V + r * rs;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        double2 acc = wr[e];
+        for (int i = 0; i < k; ++i) acc = cadd(acc, cmul(hs[i], Vr[i * ldv + e]));
+        wr[e] = acc;
+    }
+}
+
+void launch_gemv_update_b(hipStream_t s, double2* w, long wrs, const double2* V, long ldv, long rs,
+                          const int* kcnt, const double2* h, int kstride, double sign, long n,
+                          const int* ridx, int na) {
+    const unsigned nb = (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 2048 / na + 1));
+    k_gemv_update_b<<<dim3(nb, na), 256, 0, s>>>(w, wrs, V, ldv, rs, kcnt, h, kstride, sign, n, ridx);
+}
+
+// out[y] = ||x_r||^2
+__global__ __launch_bounds__(256) void k_norm2_b(const double2* x, long rs, long n, double* partial,
+                                                 const int* ridx) {
+    __shared__ double red[4];
+    const double2* xr = x + (long)ridx[blockIdx.y] * rs;
+    double acc = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        acc += cnorm2(xr[e]);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void k_reduce_real_b(const double* partial, int nblk, double* out) {
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int b = 0; b < nblk; ++b) s += partial[(long)blockIdx.x * nblk + b];
+        out[blockIdx.x] = s;
+    }
+}
+void launch_norm2_b(hipStream_t s, const double2* x, long rs, long n, double* partial, double* out,
+                    int nblk, const int* ridx, int na) {
+    k_norm2_b<<<dim3(nblk, na), 256, 0, s>>>(x, rs, n, partial, ridx);
+    k_reduce_real_b<<<na, 64, 0, s>>>(partial, nblk, out);
+}
+
+// y_r = a[y] * x_r
+__global__ void k_scale_b(double2* y, long yrs, const double2* x, long xrs, const double2* a, long n,
+                          const int* ridx) {
+    const long r = ridx[blockIdx.y];
+    const double2 s = a[blockIdx.y];
+    double2* yr = y + r * yrs;
+    const double2* xr = x + r * xrs;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x)
+        yr[e] = cmul(s, xr[e]);
+}
+void launch_scale_b(hipStream_t s, double2* y, long yrs, const double2* x, long xrs, const double2* a,
+                    long n, const int* ridx, int na) {
+    const unsigned nb = (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 2048 / na + 1));
+    k_scale_b<<<dim3(nb, na), 256, 0, s>>>(y, yrs, x, xrs, a, n, ridx);
+}
+
+}  // namespace fdfd

@@ -247,7 +247,15 @@ __global__ __launch_bounds__(256) void k_stencil(double2* __restrict__ y, const 
                                                  const double2* sxf, const double2* sxb,
                                                  const double2* syf, const double2* syb, int Nx,
                                                  int Ny, double ry, double kdiag,
-                                                 const double2* __restrict__ b) {
+                                                 const double2* __restrict__ b, long rs,
+                                                 const int* ridx) {
+    // several right-hand sides: grid z -> source ridx[z], vectors at r * rs
+    if (ridx) {
+        const long ro = (long)ridx[blockIdx.z] * rs;
+        y += ro;
+        x += ro;
+        if (b) b += ro;
+    }
     // TE weights (and every transposed weight) need eps one or two cells away
     constexpr int EH = (POL == 2) ? (TRANS ? 2 : 1) : 0;
     __shared__ double2 xs[ST_TY + 2][ST_TX + 2];
@@ -311,11 +319,11 @@ __global__ __launch_bounds__(256) void k_stencil(double2* __restrict__ y, const 
 }
 
 void launch_stencil(hipStream_t s, double2* y, const double2* x, const StencilOp& op,
-                    int transpose, const double2* b) {
-    const dim3 grid((op.Nx + ST_TX - 1) / ST_TX, (op.Ny + ST_TY - 1) / ST_TY);
+                    int transpose, const double2* b, long rs, const int* ridx, int na) {
+    const dim3 grid((op.Nx + ST_TX - 1) / ST_TX, (op.Ny + ST_TY - 1) / ST_TY, ridx ? na : 1);
 #define ST_LAUNCH(P, F, T)                                                                     \
     k_stencil<P, F, T><<<grid, 256, 0, s>>>(y, x, op.eps, op.sxf, op.sxb, op.syf, op.syb, op.Nx, \
-                                            op.Ny, op.ry, op.kdiag, b)
+                                            op.Ny, op.ry, op.kdiag, b, rs, ridx)
     if (op.form == 1) {
         if (transpose) ST_LAUNCH(1, 1, true);
         else ST_LAUNCH(1, 1, false);

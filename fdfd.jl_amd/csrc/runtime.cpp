@@ -95,8 +95,8 @@ int ensure_solve_work(Handle* h, long N) {
                      (void**)&h->Zk, (void**)&h->tmp, (void**)&h->tmp2, (void**)&h->bvec,
                      (void**)&h->xvec};
     for (auto b : bufs) { dfree(*b); *b = nullptr; }
-    HIPCHK(h, hipMalloc(&h->g_mid, sizeof(double2) * N));
-    HIPCHK(h, hipMalloc(&h->x_out, sizeof(double2) * N));
+    HIPCHK(h, hipMalloc(&h->g_mid, sizeof(double2) * N * KMULTI));
+    HIPCHK(h, hipMalloc(&h->x_out, sizeof(double2) * N * KMULTI));
     HIPCHK(h, hipMalloc(&h->V, sizeof(double2) * N * (KDIM + 1)));
     HIPCHK(h, hipMalloc(&h->Zk, sizeof(double2) * N * KDIM));
     HIPCHK(h, hipMalloc(&h->tmp, sizeof(double2) * N));
@@ -491,6 +491,7 @@ int fdfd_destroy(fdfd_handle* hh) {
     (void)hipStreamSynchronize(h->stream);
     for (auto& kv : h->factors) free_factor(h, kv.second);
     arena_clear(h);
+    free_multi_work(h);
     void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
                     h->dpart, h->rpart, h->hdev};
@@ -615,7 +616,27 @@ int fdfd_solve_d(fdfd_handle* hh, int factor_id, int nrhs, const void* d_src, vo
     HIPCHK(h, hipEventCreate(&e1));
     HIPCHK(h, hipEventRecord(e0, h->stream));
     int rc = FDFD_OK;
-    for (int r = 0; r < nrhs && rc == FDFD_OK; ++r) {
+    // field recovery: TM w = 1/(i omega mu0'), TE w = 1/(i omega eps0')
+    const double2 wf = make_double2(0.0, -1.0 / (F->omega * ((F->pol == FDFD_TM) ? mu0n : eps0n)));
+    if (nrhs > 1) {
+        // several sources: lockstep FGMRES, KMULTI at a time, one pass over
+        // the factor per iteration for all of them
+        for (int s0 = 0; s0 < nrhs && rc == FDFD_OK; s0 += KMULTI) {
+            const int ns = std::min(KMULTI, nrhs - s0);
+            if ((rc = ensure_multi_work(h, N, ns))) break;
+            for (int k = 0; k < ns; ++k) {
+                launch_scale_copy(h->stream, h->mB + (long)k * N, (const double2*)d_src + (long)(s0 + k) * N,
+                                  make_double2(0.0, F->scale * F->omega), N);
+                launch_zero(h->stream, h->mX + (long)k * N, N);
+            }
+            rc = rt_gmres_multi(h, F, ns, h->mB, h->mX, tol, &st.iters, &worst);
+            if (rc) break;
+            for (int k = 0; k < ns; ++k)
+                launch_fields(h->stream, (double2*)d_out + (long)(s0 + k) * 3 * N, h->mX + (long)k * N,
+                              h->eps, F->sxf, F->sxb, F->syf, F->syb, g, F->pol, wf, 0);
+        }
+    }
+    for (int r = 0; r < nrhs && nrhs == 1 && rc == FDFD_OK; ++r) {
         const double2* src = (const double2*)d_src + (long)r * N;
         double2* out = (double2*)d_out + (long)r * 3 * N;
         // b' = scale * (i omega) * src   (driven.jl:36,46)

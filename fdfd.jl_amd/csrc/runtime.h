@@ -99,7 +99,14 @@ struct Handle {
     int fuse_panels = 1;               // two Gauss-Jordan panels per trailing pass (FDFD_FUSE_PANELS=0)
     // sweep / Krylov work
     long work_N = 0;
-    double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors
+    double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors (KMULTI sources each)
+    // multi-source FGMRES work (multi.cpp): bases for up to multi_k sources of
+    // multi_N cells, small per-source device arrays and pinned staging
+    long multi_N = 0;
+    int multi_k = 0;
+    double2 *mV = nullptr, *mZ = nullptr, *mB = nullptr, *mX = nullptr;
+    void* msmall = nullptr;
+    void* mpin = nullptr;
     double2 *V = nullptr, *Zk = nullptr;  // Krylov bases
     int kdim = 0;
     double2 *tmp = nullptr, *tmp2 = nullptr, *bvec = nullptr, *xvec = nullptr;
@@ -173,6 +180,7 @@ struct ProfScope {
 // give one back (the pool keeps the largest ARENA_POOL); free them all
 constexpr int ARENA_POOL = 2;
 constexpr int MAX_BUILD = 128;   // factors one rt_build_factors call builds together
+constexpr int KMULTI = 16;       // sources one multi-source FGMRES solves together
 void arena_trim(Handle* h);      // drop the smallest arenas beyond arena_cap
 size_t arena_pooled_bytes(const Handle* h);
 void* arena_take(Handle* h, size_t bytes);
@@ -180,6 +188,12 @@ void arena_give(Handle* h, void* p, size_t bytes);
 void arena_clear(Handle* h);
 void rt_free_factor(Handle* h, Factor* F);
 void rt_release_factor_storage(Handle* h, Factor* F);  // device storage only, F stays
+// FGMRES for nr <= KMULTI sources on one factor in lockstep (multi.cpp):
+// X_r solves A' X_r = B_r (vectors at r * N), each to ||B_r - A' X_r|| <= tol ||B_r||
+int rt_gmres_multi(Handle* h, Factor* F, int nr, const double2* B, double2* X, double tol,
+                   int* iters, double* worst);
+int ensure_multi_work(Handle* h, long N, int k);
+void free_multi_work(Handle* h);
 
 // work buffers of the handle (grown on demand)
 int ensure_factor_work(Handle* h, int n, int nch);
