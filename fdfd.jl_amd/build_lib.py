@@ -22,7 +22,7 @@ ARCH = "gfx950"
 SOURCES = ["k_operator.hip", "k_factor.hip", "k_bcr.hip", "k_mode.hip", "k_multi.hip",
            "runtime.cpp", "bcr.cpp", "multi.cpp",
            "eigen.cpp", "dense_eig.cpp", "batch.cpp", "mode.cpp", "modulated.cpp",
-           "flux.cpp", "arnoldi.cpp", "vmode.cpp"]
+           "flux.cpp", "arnoldi.cpp", "vmode.cpp", "tune.cpp"]
 
 # All device code is built without packed FP32 instructions.
 HIPFLAGS = [

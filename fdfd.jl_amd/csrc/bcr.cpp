@@ -125,6 +125,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     }
     const int n = Fs[0]->n, m = Fs[0]->m;
     const size_t elem = Fs[0]->elem;
+    rt_tune_lookup(h, n, m, prec, Fs[0]->pol);
     // mixed path with three-product GEMMs: operands are split once per step
     const bool b3 = (prec != FDFD_FP64) && h->gemm_b3;
     if (n > MAX_LINE)

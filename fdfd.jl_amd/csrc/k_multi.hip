@@ -371,13 +371,27 @@ __global__ __launch_bounds__(256) void k_multidot_b(const double2* V, long ldv, 
     }
 }
 
-// out[y][i] = sum_blk partial[y][blk][i], fixed order
-__global__ void k_reduce_partials_b(const double2* partial, int nblk, int k, double2* out) {
+// out[y][i] = sum_blk partial[y][blk][i]: thread t adds column t % 32 over
+// rows t / 32 + 32 q (fixed order), then the 32 row groups in LDS
+__global__ __launch_bounds__(1024) void k_reduce_partials_b(const double2* partial, int nblk, int k,
+                                                            double2* out) {
+    __shared__ double2 acc[32][32];
     const int y = blockIdx.x;
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    const double2* P = partial + (long)y * nblk * k;
+    const int c = threadIdx.x % 32, grp = threadIdx.x / 32;
+    for (int c0 = 0; c0 < k; c0 += 32) {
+        const int i = c0 + c;
         double2 s = make_double2(0, 0);
-        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[((long)y * nblk + b) * k + i]);
-        out[(long)y * k + i] = s;
+        if (i < k)
+            for (int b = grp; b < nblk; b += 32) s = cadd(s, P[b * (long)k + i]);
+        acc[grp][c] = s;
+        __syncthreads();
+        if (grp == 0 && i < k) {
+            double2 t = acc[0][c];
+            for (int g2 = 1; g2 < 32; ++g2) t = cadd(t, acc[g2][c]);
+            out[(long)y * k + i] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -385,7 +399,7 @@ void launch_multidot_b(hipStream_t s, const double2* V, long ldv, long rs, int k
                        long wrs, long n, double2* partial, double2* out, int nblk,
                        const int* ridx, int na) {
     k_multidot_b<<<dim3(nblk, na), 256, 0, s>>>(V, ldv, rs, k, w, wrs, n, partial, ridx);
-    k_reduce_partials_b<<<na, 64, 0, s>>>(partial, nblk, k, out);
+    k_reduce_partials_b<<<na, 1024, 0, s>>>(partial, nblk, k, out);
 }
 
 // w_r += sign * sum_{i < kcnt[y]} h[y][i] V_{i,r}   (h has kstride entries per source)
@@ -429,17 +443,23 @@ __global__ __launch_bounds__(256) void k_norm2_b(const double2* x, long rs, long
     __syncthreads();
     if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
-__global__ void k_reduce_real_b(const double* partial, int nblk, double* out) {
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int b = 0; b < nblk; ++b) s += partial[(long)blockIdx.x * nblk + b];
-        out[blockIdx.x] = s;
+__global__ __launch_bounds__(256) void k_reduce_real_b(const double* partial, int nblk, double* out) {
+    __shared__ double acc[256];
+    const double* P = partial + (long)blockIdx.x * nblk;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += P[b];
+    acc[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
 }
 void launch_norm2_b(hipStream_t s, const double2* x, long rs, long n, double* partial, double* out,
                     int nblk, const int* ridx, int na) {
     k_norm2_b<<<dim3(nblk, na), 256, 0, s>>>(x, rs, n, partial, ridx);
-    k_reduce_real_b<<<na, 64, 0, s>>>(partial, nblk, out);
+    k_reduce_real_b<<<na, 256, 0, s>>>(partial, nblk, out);
 }
 
 // y_r = a[y] * x_r

@@ -412,17 +412,37 @@ void launch_multidot(hipStream_t s, const double2* V, long ldv, int k, const dou
     k_multidot<<<nblk, 256, 0, s>>>(V, ldv, k, w, n, partial);
 }
 
-__global__ void k_reduce_partials(const double2* partial, int nblk, int k, double2* out) {
-    // one block; thread i sums column i in a fixed order
-    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+// Final sums of per-block partials, one workgroup of 1024 threads: thread t
+// adds column t % 32 over the rows t / 32, t / 32 + 32, ... (a fixed order, so
+// results are reproducible), then the 32 row groups of a column are added in
+// LDS.  (One thread per column walking all rows waited out one load latency
+// per row: 77 us for 512 rows.)
+constexpr int RP_COLS = 32, RP_GROUPS = 32;
+__device__ inline void reduce_columns(const double2* partial, int nblk, int k, double2* out) {
+    __shared__ double2 acc[RP_GROUPS][RP_COLS];
+    const int c = threadIdx.x % RP_COLS, grp = threadIdx.x / RP_COLS;
+    for (int c0 = 0; c0 < k; c0 += RP_COLS) {
+        const int i = c0 + c;
         double2 s = make_double2(0, 0);
-        for (int b = 0; b < nblk; ++b) s = cadd(s, partial[b * (long)k + i]);
-        out[i] = s;
+        if (i < k)
+            for (int b = grp; b < nblk; b += RP_GROUPS) s = cadd(s, partial[b * (long)k + i]);
+        acc[grp][c] = s;
+        __syncthreads();
+        if (grp == 0 && i < k) {
+            double2 t = acc[0][c];
+            for (int g2 = 1; g2 < RP_GROUPS; ++g2) t = cadd(t, acc[g2][c]);
+            out[i] = t;
+        }
+        __syncthreads();
     }
+}
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double2* partial, int nblk, int k,
+                                                          double2* out) {
+    reduce_columns(partial, nblk, k, out);
 }
 void launch_reduce_partials(hipStream_t s, const double2* partial, int nblk, int k,
                             double2* out) {
-    k_reduce_partials<<<1, 64, 0, s>>>(partial, nblk, k, out);
+    k_reduce_partials<<<1, RP_COLS * RP_GROUPS, 0, s>>>(partial, nblk, k, out);
 }
 
 // w += sign * sum_i h_i V_i
@@ -458,15 +478,24 @@ __global__ __launch_bounds__(256) void k_norm2(const double2* x, long n, double*
 void launch_norm2(hipStream_t s, const double2* x, long n, double* partial, int nblk) {
     k_norm2<<<nblk, 256, 0, s>>>(x, n, partial);
 }
-__global__ void k_reduce_real(const double* partial, int nblk, double* out) {
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int b = 0; b < nblk; ++b) s += partial[b];
-        *out = s;
+// sum of nblk partials: 256 threads, fixed strided order, then a fixed tree
+__device__ inline void reduce_real(const double* partial, int nblk, double* out) {
+    __shared__ double acc[256];
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[b];
+    acc[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = acc[0];
+}
+__global__ __launch_bounds__(256) void k_reduce_real(const double* partial, int nblk, double* out) {
+    reduce_real(partial, nblk, out);
 }
 void launch_reduce_real(hipStream_t s, const double* partial, int nblk, double* out) {
-    k_reduce_real<<<1, 64, 0, s>>>(partial, nblk, out);
+    k_reduce_real<<<1, 256, 0, s>>>(partial, nblk, out);
 }
 
 // ---------------------------------------------------------------------------

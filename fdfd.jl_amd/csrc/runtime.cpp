@@ -470,6 +470,7 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     hh->impl.trail_h3 = (et && std::strcmp(et, "fp32") == 0) ? 0 : 1;
     const char* ef = std::getenv("FDFD_FUSE_PANELS");
     hh->impl.fuse_panels = (ef && ef[0] == '0') ? 0 : 1;
+    hh->impl.env_knobs = e3 || ew || eb || et || ef;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;

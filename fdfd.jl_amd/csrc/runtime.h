@@ -97,6 +97,8 @@ struct Handle {
     int gemm_b3 = 1;                   // mixed path: three-product fp16 GEMMs (FDFD_MIXED_GEMM=fp32: FP32)
     int trail_h3 = 1;                  // mixed path: three-product fp16 trailing updates (FDFD_MIXED_TRAIL=fp32: FP32)
     int fuse_panels = 1;               // two Gauss-Jordan panels per trailing pass (FDFD_FUSE_PANELS=0)
+    bool env_knobs = false;            // a variant switch above came from the environment
+    bool tuned_explicitly = false;     // fdfd_autotune chose the variants (no cache lookup)
     // sweep / Krylov work
     long work_N = 0;
     double2 *g_mid = nullptr, *x_out = nullptr;  // reduced-solve vectors (KMULTI sources each)
@@ -193,6 +195,8 @@ void rt_release_factor_storage(Handle* h, Factor* F);  // device storage only, F
 int rt_gmres_multi(Handle* h, Factor* F, int nr, const double2* B, double2* X, double tol,
                    int* iters, double* worst);
 int ensure_multi_work(Handle* h, long N, int k);
+// variants from the FDFD_TUNE_CACHE file for this shape (tune.cpp)
+void rt_tune_lookup(Handle* h, int n, int m, int precision, int pol);
 void free_multi_work(Handle* h);
 
 // work buffers of the handle (grown on demand)

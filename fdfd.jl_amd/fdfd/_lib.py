@@ -99,6 +99,8 @@ PROTOTYPES = {
     "fdfd_profile_read": (_I, [_P, _I, _PD, C.POINTER(C.c_longlong), _PD, _PD]),
     "fdfd_set_tuning": (_I, [_P, _I, _I]),
     "fdfd_debug_inject": (_I, [_P, _I, _I]),
+    "fdfd_autotune": (_I, [_P, _I, _D, _I, _PS]),
+    "fdfd_get_tuning": (_I, [_P, _PI, _I]),
     "fdfd_solve_pullback": (_I, [_P, _I, _P, _P, _D, _P, _PS]),
     "fdfd_solve_pullback_d": (_I, [_P, _I, _P, _P, _D, _P, _PS]),
 }

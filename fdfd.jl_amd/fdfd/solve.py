@@ -205,6 +205,22 @@ class Engine:
     def set_tuning(self, panel_width: int = 0, force_full_search: bool = False):
         self._chk(self.lib.fdfd_set_tuning(self.h, panel_width, int(force_full_search)))
 
+    def autotune(self, pol: int, omega: float, precision="mixed"):
+        """Time every factorization variant on the current grid and eps and keep
+        the fastest (fdfd_autotune); recorded in FDFD_TUNE_CACHE if set.
+        Returns (knobs, winning step ms)."""
+        st = fdfd_stats()
+        self._chk(self.lib.fdfd_autotune(self.h, int(pol), float(omega), PRECISIONS[precision],
+                                         C.byref(st)))
+        return self.tuning(), st.t_factor_ms
+
+    def tuning(self):
+        """The handle's factorization variants: gemm_b3, trail_h3, fuse_panels,
+        gemm_waves, gemm3m."""
+        k = (C.c_int * 5)()
+        self._chk(self.lib.fdfd_get_tuning(self.h, k, 5))
+        return dict(zip(("gemm_b3", "trail_h3", "fuse_panels", "gemm_waves", "gemm3m"), list(k)))
+
     def debug_inject(self, what: int, arg: int = 0):
         """One-shot failure injection (fdfd_debug_inject) for error-path tests."""
         self._chk(self.lib.fdfd_debug_inject(self.h, int(what), int(arg)))

@@ -345,6 +345,22 @@ int fdfd_profile_read(fdfd_handle* h, int kernel, double* total_ms,
  * panel search the whole column for its pivots (no candidate window). */
 int fdfd_set_tuning(fdfd_handle* h, int panel_width, int force_full_search);
 
+/* Autotuner (no counterpart in the reference).  Times a whole step (assemble
+ * + factor + refined solve of a point source to 1e-10) on the handle's grid
+ * and eps for every variant of the factorization kernels -- fp16x3 or FP32
+ * products for complex64 GEMMs and trailing updates, one or two panels per
+ * trailing pass, 8 or 4 waves and three- or four-multiplication products in
+ * the FP32/FP64 GEMM -- and keeps the fastest on the handle.  With the
+ * environment variable FDFD_TUNE_CACHE naming a file, the choice is recorded
+ * there per (line length, line count, precision, pol), and handles read it
+ * at factorization time (unless variant switches are set in the
+ * environment).  stats.t_factor_ms: the winning step time.
+ * fdfd_get_tuning: knobs[0..4] = gemm fp16x3, trailing fp16x3, fused panels,
+ * GEMM waves, GEMM three-multiplication. */
+int fdfd_autotune(fdfd_handle* h, int pol, double omega, int precision,
+                  fdfd_stats* stats);
+int fdfd_get_tuning(fdfd_handle* h, int* knobs, int nknobs);
+
 /* Failure injection for testing the error paths (no counterpart in the
  * reference, whose only failure handling is the solver fallback of
  * solver.jl:19-28).  One-shot: the next operation of the named kind on the
