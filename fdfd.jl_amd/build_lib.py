@@ -65,6 +65,48 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     return LIB
 
 
+# host-instrumented build for the sanitizer run (device code is not instrumented)
+ASAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+ASAN_DIR = os.path.join(BUILD, "asan")
+ASAN_EXE = os.path.join(LIBDIR, "asan_driver")
+
+
+def _compile_asan(src: str) -> str:
+    obj = os.path.join(ASAN_DIR, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *HIPFLAGS, *ASAN_HOST, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_asan(jobs: int = 4, verbose: bool = False) -> str:
+    """The library's sources with the host code instrumented by
+    AddressSanitizer and UndefinedBehaviorSanitizer, linked into
+    tests/asan_driver.cpp (run on a GPU by tests/test_gpu_asan.py)."""
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(_compile_asan, SOURCES))
+    driver = os.path.join(HERE, "..", "tests", "asan_driver.cpp")
+    dobj = os.path.join(ASAN_DIR, "asan_driver.o")
+    cmd = [hipcc(), "-O1", "-std=c++17", *ASAN_HOST, "-I" + os.path.join(HERE, "..", "include"),
+           "-x", "c++", "-c", driver, "-o", dobj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    tmp = ASAN_EXE + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *ASAN_HOST, dobj, *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ASAN_EXE)
+    if verbose:
+        print("built", ASAN_EXE)
+    return ASAN_EXE
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)

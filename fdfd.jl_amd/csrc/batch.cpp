@@ -120,6 +120,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     // grid (no factor-sized allocation or free inside the batch).
     arena_clear(h);
     int rc = FDFD_OK;
+    const auto ta0 = now();
     for (auto& job : b->jobs) {
         const size_t nres = (size_t)job.omegas.size() * job.nsrc * 3 * job.g.Nx * (size_t)job.g.Ny;
         if (hipMalloc(&job.result, sizeof(double2) * nres) != hipSuccess) {
@@ -142,6 +143,11 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     }
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
+    // factor memory of a group: at most FDFD_BATCH_MEMORY_GB (default 64) GB;
+    // larger groups gain little once a level's launches fill the chip
+    double budget = 64.0;
+    if (const char* eb = std::getenv("FDFD_BATCH_MEMORY_GB")) budget = std::max(1.0, std::atof(eb));
+    const double usable = std::min(0.8 * (double)freeb, budget * 1073741824.0);
     std::vector<int> group(b->jobs.size(), 1);
     std::vector<size_t> arena_bytes(b->jobs.size(), 0);
     size_t slab_need = 0;
@@ -155,7 +161,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         // tables allocated beside them (about 96 B and 2 GJ channels per cell)
         const double N = (double)job.g.Nx * job.g.Ny;
         const double per = (double)arena_bytes[ji] + 96.0 * N + 64.0 * N * (double)VMAX / n;
-        int B = (int)std::floor(0.8 * (double)freeb / per);
+        int B = (int)std::floor(usable / per);
         B = std::max(1, std::min<int>({B, (int)job.omegas.size(), 64}));
         group[ji] = B;
         slab_need = std::max(slab_need, (size_t)B * arena_bytes[ji]);
@@ -168,6 +174,10 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         h->slab_size = slab_need;
     }
     h->slab_used = 0;
+    if (timing)
+        fprintf(stderr, "batch setup (results %.1f GB, slab %.1f GB): %.3f s\n",
+                [&] { double t = 0; for (auto& j : b->jobs) t += (double)j.omegas.size() * j.nsrc * 48.0 * j.g.Nx * j.g.Ny; return t / 1e9; }(),
+                slab_need / 1e9, secs(ta0, now()));
     int iters = 0, pivot_fallback = 0;
     double worst = 0, t_factor = 0, t_solve = 0, growth = 0;
     for (int ji : order) {
