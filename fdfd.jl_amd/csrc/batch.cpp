@@ -96,6 +96,10 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
         return fail(h, FDFD_ERR_INVALID, "precision must be FDFD_FP64 or FDFD_FP32_IR");
     if (!(tol > 0)) return fail(h, FDFD_ERR_INVALID, "tol must be > 0");
     (void)hipSetDevice(h->device);
+    const bool timing = std::getenv("FDFD_DEBUG_TIMING") != nullptr;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto secs = [](auto a, auto c) { return std::chrono::duration<double>(c - a).count(); };
+    const auto tr0 = now();
     free_results(b);
     // the batch replaces the handle's problem: its factors are released and
     // it is left without a grid (fdfd_set_grid starts the next problem)
@@ -103,9 +107,6 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     for (auto& kv : h->factors) rt_free_factor(h, kv.second);
     h->factors.clear();
     h->has_grid = h->has_eps = false;
-    const bool timing = std::getenv("FDFD_DEBUG_TIMING") != nullptr;
-    auto now = [] { return std::chrono::steady_clock::now(); };
-    auto secs = [](auto a, auto c) { return std::chrono::duration<double>(c - a).count(); };
     // largest factors first (the slab below is sized by them)
     std::vector<int> order(b->jobs.size());
     std::iota(order.begin(), order.end(), 0);
@@ -121,6 +122,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     arena_clear(h);
     int rc = FDFD_OK;
     const auto ta0 = now();
+    if (timing) fprintf(stderr, "batch release of the handle's factors and pool: %.3f s\n", secs(tr0, ta0));
     for (auto& job : b->jobs) {
         const size_t nres = (size_t)job.omegas.size() * job.nsrc * 3 * job.g.Nx * (size_t)job.g.Ny;
         if (hipMalloc(&job.result, sizeof(double2) * nres) != hipSuccess) {

@@ -282,11 +282,14 @@ void launch_stencil(hipStream_t s, double2* y, const double2* x, const StencilOp
                     int transpose, const double2* b, long rs = 0, const int* ridx = nullptr,
                     int na = 1);
 // several right-hand sides per launch (k_multi.hip): sources ridx[0..na) of
-// buffers whose source r starts at r * rs
+// buffers whose source r starts at r * rs; the matrix-vector step runs on the
+// matrix cores (16 sources per chunk, inputs packed into `pack`, at least
+// mv_pack_bytes(njobs, n, fp64) bytes) when mfma is set, else on the vector ALUs
 template <typename R> int mv_multi_prepare(int n);
+size_t mv_pack_bytes(int njobs, int n, int fp64);
 template <typename R>
 void launch_mv_multi(hipStream_t s, const MvJob* jobs_d, int njobs, int n, long rs, const int* ridx,
-                     int na);
+                     int na, bool mfma, void* pack);
 void launch_diag_multi(hipStream_t s, const DiagJob* jobs_d, int njobs, int n, long rs,
                        const int* ridx, int na);
 int trisolve_multi_prepare(int n);

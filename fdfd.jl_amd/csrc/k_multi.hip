@@ -103,6 +103,157 @@ __global__ __launch_bounds__(256) void k_mv_multi(const MvJob* jobs, int n, long
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same step on the matrix cores, for a chunk of up to MVQ = 16
+// right-hand sides: Y (n x 16) = M (n x n) X (n x 16) is a skinny complex
+// GEMM, run as v_mfma 16x16x4 products (f32 for complex64 factors, f64 for
+// complex128).  Two launches:
+//  - k_mv_pack gathers each job's inputs once (through imap, converted to the
+//    factor's precision) into a packed panel Xp[job][term][k][16], zero past
+//    n and past the chunk's sources, so that
+//  - k_mv_mfma streams the factor blocks with no gather, barrier or LDS in its
+//    loop: a workgroup of 4 waves owns 16 RW rows of one job; wave w takes row
+//    tile w % RW and K part w / RW of KS = 4 / RW parts (chunks of MVK
+//    columns, chunk c in part c % KS), issuing the next chunk's A and B
+//    loads before the current chunk's products.  The lane's A operand for
+//    K-step j is M[row = lane % 16][k0 + lane / 16 + 4 j] (four lanes of a
+//    row read 4 adjacent entries per load), its B operand the matching
+//    packed row (a wave reads 512 contiguous bytes per K-step).  Re and Im go
+//    to separate accumulators for each of the four real products; the K
+//    parts are summed through LDS at the end.
+// Every factor entry is loaded once per chunk of 16 sources, and the packed
+// panel (16 sources x 8 or 16 B per column) is read from cache by each row
+// tile.  RW = 4 is used for steps whose jobs fill the chip, RW = 2 or 1 split
+// the K range for short steps, whose time is latency.
+// ---------------------------------------------------------------------------
+constexpr int MVQ = 16;   // right-hand sides per chunk of sources (the MFMA's N)
+constexpr int MVP = 64;   // packed panels are padded to a multiple of this many columns
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_mv_pack(const MvJob* jobs, int n, int npad, long rs,
+                                                 const int* ridx, int na, cplx<R>* Xp) {
+    using C = cplx<R>;
+    __shared__ C tile[MVP][MVQ + 1];
+    const MvJob J = jobs[blockIdx.y];
+    const int t = blockIdx.z;
+    if (t >= J.nt) return;
+    const int k0 = blockIdx.x * MVP;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int kc = min(MVQ, na);
+    const double2* x = t ? J.x[1] : J.x[0];
+    const int* im = t ? J.imap[1] : J.imap[0];
+    const int k = k0 + lane;
+    const long sk = k < n ? (long)(im ? gload_s(im + k) : k) * J.si : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int q = 4 * i + wv;
+        tile[lane][q] = (k < n && q < kc) ? from_d<C>(gload(x + sk + (long)gload_s(ridx + q) * rs))
+                                          : mk<C>(0, 0);
+    }
+    __syncthreads();
+    C* dst = Xp + ((long)(2 * blockIdx.y + t) * npad + k0) * MVQ;
+#pragma unroll
+    for (int i = 0; i < MVP * MVQ / 256; ++i) {
+        const int e = i * 256 + threadIdx.x;
+        gstore(dst + e, tile[e / MVQ][e % MVQ]);
+    }
+}
+
+template <typename R, int RW>
+__global__ __launch_bounds__(256) void k_mv_mfma(const MvJob* jobs, int n, int npad, long rs,
+                                                 const int* ridx, int na, const cplx<R>* Xp) {
+    using C = cplx<R>;
+    using MF = mfma_t<R>;
+    using acc_t = typename MF::acc;
+    constexpr int KS = 4 / RW;
+    constexpr int MVK = sizeof(R) == 4 ? 64 : 32;  // columns per chunk
+    constexpr int KJ = MVK / 4;                    // K-steps per chunk
+    __shared__ R red[KS > 1 ? (KS - 1) * RW * 64 * 8 : 1];
+    const int job = blockIdx.y;
+    const MvJob J = jobs[job];
+    const int kc = min(MVQ, na);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rt = wv % RW, kp = wv / RW;
+    const int kq = lane >> 4;
+    const int row_base = blockIdx.x * 16 * RW + rt * 16;
+    const int rowA = row_base + (lane & 15);
+    const bool rowok = rowA < n;
+    const int nkb = npad / MVK;
+    const int nch = J.nt * nkb;
+    const C* xb = Xp + (long)2 * job * npad * MVQ + (lane & 15);
+    auto load = [&](int ch, C* A, C* B) {
+        const int t = ch / nkb, k0 = (ch - t * nkb) * MVK + kq;
+        const C* row = (const C*)(t ? J.M[1] : J.M[0]) + (long)(rowok ? rowA : 0) * n;
+        const C* xc = xb + ((long)t * npad + k0) * MVQ;
+#pragma unroll
+        for (int j = 0; j < KJ; ++j) {
+            const int k = k0 + 4 * j;
+            A[j] = (rowok && k < n) ? gload(row + k) : mk<C>(0, 0);
+            B[j] = gload(xc + 4 * j * MVQ);
+        }
+    };
+    C a[KJ], b[KJ], an[KJ], bn[KJ];
+    acc_t crr = {0, 0, 0, 0}, cii = {0, 0, 0, 0}, cri = {0, 0, 0, 0}, cir = {0, 0, 0, 0};
+    if (kp < nch) load(kp, a, b);
+    for (int ch = kp; ch < nch; ch += KS) {
+        const bool more = ch + KS < nch;
+        if (more) load(ch + KS, an, bn);
+#pragma unroll
+        for (int j = 0; j < KJ; ++j) {
+            crr = MF::run(a[j].x, b[j].x, crr);
+            cii = MF::run(-a[j].y, b[j].y, cii);
+            cri = MF::run(a[j].x, b[j].y, cri);
+            cir = MF::run(a[j].y, b[j].x, cir);
+        }
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < KJ; ++j) {
+                a[j] = an[j];
+                b[j] = bn[j];
+            }
+        }
+    }
+    acc_t yr = crr + cii, yi = cri + cir;
+    if constexpr (KS > 1) {
+        // parts 1 .. KS-1 leave their sums in LDS, part 0 adds them
+        if (kp > 0) {
+            R* d = red + ((long)((kp - 1) * RW + rt) * 64 + lane) * 8;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                d[v] = yr[v];
+                d[4 + v] = yi[v];
+            }
+        }
+        __syncthreads();
+        if (kp > 0) return;
+#pragma unroll
+        for (int p = 1; p < KS; ++p) {
+            const R* d = red + ((long)((p - 1) * RW + rt) * 64 + lane) * 8;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                yr[v] += d[v];
+                yi[v] += d[4 + v];
+            }
+        }
+    }
+    const int q = lane & 15;
+    if (q >= kc) return;
+    const long ro = (long)gload_s(ridx + q) * rs;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = row_base + MF::row(lane, v);
+        if (row >= n) continue;
+        const long e = (long)(J.omap ? gload_s(J.omap + row) : row) * J.si + ro;
+        const double2 z = make_double2((double)yr[v], (double)yi[v]);
+        J.out[e] = J.base ? csub(gload(J.base + e), z) : z;
+    }
+}
+
+size_t mv_pack_bytes(int njobs, int n, int fp64) {
+    const size_t npad = (size_t)(n + MVP - 1) / MVP * MVP;
+    return (size_t)njobs * 2 * npad * MVQ * (fp64 ? sizeof(double2) : sizeof(float2));
+}
+
 template <typename R>
 static int mv_kc(int n) {
     // LDS for two terms of KC gathered vectors: at most 128 KB
@@ -130,7 +281,28 @@ template int mv_multi_prepare<float>(int);
 
 template <typename R>
 void launch_mv_multi(hipStream_t s, const MvJob* jobs_d, int njobs, int n, long rs, const int* ridx,
-                     int na) {
+                     int na, bool mfma, void* pack) {
+    if (mfma) {
+        // sources in chunks of MVQ; 64-row workgroups when they give at
+        // least two per CU, else the K range split two or four ways
+        const int npad = (n + MVP - 1) / MVP * MVP;
+        const long w64 = (long)njobs * ((n + 63) / 64), w32 = (long)njobs * ((n + 31) / 32);
+        const int rw = w64 >= 512 ? 4 : w32 >= 512 ? 2 : 1;
+        cplx<R>* Xp = (cplx<R>*)pack;
+        for (int q0 = 0; q0 < na; q0 += MVQ) {
+            const int nc = std::min(MVQ, na - q0);
+            k_mv_pack<R><<<dim3(npad / MVP, njobs, 2), 256, 0, s>>>(jobs_d, n, npad, rs, ridx + q0,
+                                                                   nc, Xp);
+            const dim3 grid((n + 16 * rw - 1) / (16 * rw), njobs);
+            if (rw == 4)
+                k_mv_mfma<R, 4><<<grid, 256, 0, s>>>(jobs_d, n, npad, rs, ridx + q0, nc, Xp);
+            else if (rw == 2)
+                k_mv_mfma<R, 2><<<grid, 256, 0, s>>>(jobs_d, n, npad, rs, ridx + q0, nc, Xp);
+            else
+                k_mv_mfma<R, 1><<<grid, 256, 0, s>>>(jobs_d, n, npad, rs, ridx + q0, nc, Xp);
+        }
+        return;
+    }
     const int kc = mv_kc<R>(n);
     const int nch = (na + kc - 1) / kc;
     int nblk = (4096 + njobs * nch - 1) / (njobs * nch);
@@ -144,8 +316,10 @@ void launch_mv_multi(hipStream_t s, const MvJob* jobs_d, int njobs, int n, long 
         default: k_mv_multi<R, 1><<<grid, 256, lds, s>>>(jobs_d, n, rs, ridx, na); break;
     }
 }
-template void launch_mv_multi<double>(hipStream_t, const MvJob*, int, int, long, const int*, int);
-template void launch_mv_multi<float>(hipStream_t, const MvJob*, int, int, long, const int*, int);
+template void launch_mv_multi<double>(hipStream_t, const MvJob*, int, int, long, const int*, int,
+                                      bool, void*);
+template void launch_mv_multi<float>(hipStream_t, const MvJob*, int, int, long, const int*, int,
+                                     bool, void*);
 
 // first-level (diagonal) step, one source per grid row
 __global__ void k_diag_multi(const DiagJob* jobs, int n, long rs, const int* ridx) {

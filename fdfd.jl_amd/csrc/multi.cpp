@@ -110,7 +110,9 @@ void free_multi_work(Handle* h) {
     }
     if (h->msmall) (void)hipFree(h->msmall);
     if (h->mpin) (void)hipHostFree(h->mpin);
-    h->msmall = h->mpin = nullptr;
+    if (h->mpack) (void)hipFree(h->mpack);
+    h->msmall = h->mpin = h->mpack = nullptr;
+    h->mpack_bytes = 0;
     h->multi_N = 0;
     h->multi_k = 0;
 }
@@ -128,8 +130,13 @@ static void launch_solve_multi(Handle* h, Factor* F, long rs, const int* ridx, i
             ps = new ProfScope(h, P_SWEEP, s.kind == 0 ? 8.0 * n * n * s.mats * na : 0, bytes);
         }
         if (s.kind == 0) {
-            if (fp64) launch_mv_multi<double>(h->stream, F->mv_tab + s.first, s.count, n, rs, ridx, na);
-            else launch_mv_multi<float>(h->stream, F->mv_tab + s.first, s.count, n, rs, ridx, na);
+            const bool mf = h->mv_mfma != 0;
+            if (fp64)
+                launch_mv_multi<double>(h->stream, F->mv_tab + s.first, s.count, n, rs, ridx, na, mf,
+                                        h->mpack);
+            else
+                launch_mv_multi<float>(h->stream, F->mv_tab + s.first, s.count, n, rs, ridx, na, mf,
+                                       h->mpack);
         } else if (s.kind == 2) {
             launch_trisolve_multi(h->stream, F->trs_tab + s.first, s.count, n, rs, ridx, na);
         } else {
@@ -148,6 +155,21 @@ int rt_gmres_multi(Handle* h, Factor* F, int nr, const double2* B, double2* X, d
     const long N = (long)g.Nx * g.Ny;
     int rc;
     if ((rc = ensure_multi_work(h, N, nr))) return rc;
+    // packed-input panel of the widest matrix-vector step (MFMA path)
+    int maxjobs = 0;
+    for (const auto& s : F->ssteps)
+        if (s.kind == 0) maxjobs = std::max(maxjobs, s.count);
+    const size_t pack = mv_pack_bytes(maxjobs, F->n, F->precision == FDFD_FP64);
+    if (h->mv_mfma && pack > h->mpack_bytes) {
+        if (h->mpack) (void)hipFree(h->mpack);
+        h->mpack = nullptr;
+        h->mpack_bytes = 0;
+        if (hipMalloc(&h->mpack, pack) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(h, FDFD_ERR_OOM, "multi-source packed inputs do not fit in device memory");
+        }
+        h->mpack_bytes = pack;
+    }
     if (mv_multi_prepare<float>(F->n) || mv_multi_prepare<double>(F->n) ||
         trisolve_multi_prepare(F->n))
         return fail(h, FDFD_ERR_HIP, "multi-source solve kernels: LDS limit could not be raised");

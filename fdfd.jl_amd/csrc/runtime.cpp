@@ -471,6 +471,8 @@ int fdfd_create(int device_ordinal, fdfd_handle** out) {
     const char* ef = std::getenv("FDFD_FUSE_PANELS");
     hh->impl.fuse_panels = (ef && ef[0] == '0') ? 0 : 1;
     hh->impl.env_knobs = e3 || ew || eb || et || ef;
+    const char* em = std::getenv("FDFD_MV_MFMA");
+    hh->impl.mv_mfma = (em && em[0] == '0') ? 0 : 1;
     if (hipStreamCreateWithFlags(&hh->impl.stream, hipStreamNonBlocking) != hipSuccess) {
         delete hh;
         return FDFD_ERR_HIP;

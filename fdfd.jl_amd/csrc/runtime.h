@@ -93,6 +93,7 @@ struct Handle {
     int inject = 0, inject_arg = 0;
     bool use_graphs = true;            // FDFD_GRAPHS=0 disables graph replay
     int gemm3m = 1;                    // three-multiplication complex GEMM (FDFD_GEMM3M=0: four)
+    int mv_mfma = 1;                   // multi-source reduced-solve products on MFMA (FDFD_MV_MFMA=0: VALU)
     int gemm_waves = 8;                // waves per GEMM workgroup (FDFD_GEMM_WAVES=4: four)
     int gemm_b3 = 1;                   // mixed path: three-product fp16 GEMMs (FDFD_MIXED_GEMM=fp32: FP32)
     int trail_h3 = 1;                  // mixed path: three-product fp16 trailing updates (FDFD_MIXED_TRAIL=fp32: FP32)
@@ -109,6 +110,8 @@ struct Handle {
     double2 *mV = nullptr, *mZ = nullptr, *mB = nullptr, *mX = nullptr;
     void* msmall = nullptr;
     void* mpin = nullptr;
+    void* mpack = nullptr;             // packed inputs of the MFMA reduced-solve products
+    size_t mpack_bytes = 0;
     double2 *V = nullptr, *Zk = nullptr;  // Krylov bases
     int kdim = 0;
     double2 *tmp = nullptr, *tmp2 = nullptr, *bvec = nullptr, *xvec = nullptr;

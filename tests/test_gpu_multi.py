@@ -53,3 +53,41 @@ def test_multi_source_solve_matches_single(engine, pol, precision):
         x, _ = O.linear_solve(A, O.rhs(om, srcs[:, :, r]))
         ez = fm[:, :, 0, r].ravel(order="F")
         assert np.linalg.norm(ez - x) / np.linalg.norm(x) <= 1e-10
+
+
+@pytest.mark.parametrize("shape", [(120, 104), (72, 136)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mfma_and_vector_products_agree(monkeypatch, shape, precision):
+    """The reduced solve's multi-source matrix-vector step on the matrix
+    cores (default) and on the vector ALUs (FDFD_MV_MFMA=0) give the same
+    fields, on a grid whose lines are rows (120 x 104: n = 104, strided
+    vectors) and one whose lines are columns (72 x 136: n = 72); neither n is
+    a multiple of the 64-row tile or the 64-column stage."""
+    from fdfd.solve import Engine
+    Nx, Ny = shape
+    d = O.random_pixel_device(Nx, Ny, 10, 1.55, 21)
+    om = float(d.omega[0])
+    rng = np.random.default_rng(5)
+    ns = 21  # a full chunk of 16 and a partial one
+    srcs = np.zeros((Nx, Ny, ns), dtype=complex)
+    for r in range(ns):
+        srcs[rng.integers(12, Nx - 12), rng.integers(12, Ny - 12), r] = 1 + 1j * r
+    out = {}
+    for mf in ("1", "0"):
+        monkeypatch.setenv("FDFD_MV_MFMA", mf)
+        eng = Engine(0)
+        try:
+            eng.set_grid(_cgrid(d.grid))
+            eng.set_eps(d.eps_r)
+            fid, _ = eng.factor(O.TM, om, precision)
+            f, st = eng.solve_fields(fid, srcs, tol=1e-12)
+            eng.release(fid)
+        finally:
+            eng.close()
+        print(f"{shape} {precision} mfma={mf}: {st['iterations']} its, residual "
+              f"{st['residual']:.1e}", flush=True)
+        assert st["residual"] <= 1e-12
+        out[mf] = f
+    for r in range(ns):
+        a, b = out["1"][:, :, :, r], out["0"][:, :, :, r]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-10, r
