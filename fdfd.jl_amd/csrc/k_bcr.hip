@@ -867,15 +867,35 @@ __global__ __launch_bounds__(512) void k_gemm_b3(const GemmJob* jobs, int n, int
         }
 #undef B3_MM
     };
+    // Ping-pong stages.  Between two barriers a wave multiplies the current
+    // stage from one LDS buffer and stores the next stage (loaded into
+    // registers one interval earlier) into the other, then issues the global
+    // loads of the stage after it.  The two waves of a SIMD belong to the two
+    // row halves (wr 0 and 1) and do these in opposite orders: wr = 1 stores
+    // and loads first, so its memory work overlaps the MFMAs of its wr = 0
+    // partner, whose memory work then overlaps wr = 1's MFMAs.  Every wave
+    // passes the same barriers.
     zero();
     load(0);
     store(0);
+    if (nkb > 1) load(1);
     __syncthreads();
+    const bool mem_first = wr == 1;
     for (int s = 0; s < nkb; ++s) {
         const int cur = s & 1;
-        if (s + 1 < nkb) load(s + 1);
-        compute(cur);
-        if (s + 1 < nkb) store(cur ^ 1);
+        if (mem_first) {
+            if (s + 1 < nkb) {
+                store(cur ^ 1);
+                if (s + 2 < nkb) load(s + 2);
+            }
+            compute(cur);
+        } else {
+            compute(cur);
+            if (s + 1 < nkb) {
+                store(cur ^ 1);
+                if (s + 2 < nkb) load(s + 2);
+            }
+        }
         __syncthreads();
     }
     epilogue();
