@@ -315,8 +315,11 @@ int fdfd_flux_d_noalloc(fdfd_handle* h, int pol, int Nx, int Ny, double dx,
 /* ---- batched driven solves (many grids x frequencies x sources) --------- */
 /* One job = one grid + eps + nomega frequencies + nsrc sources (each source
  * is solved at every frequency).  fdfd_batch_run factors all (job, omega)
- * pairs with the batched block kernels and solves every source; results are
- * read back with fdfd_batch_result.  Uneven grid sizes may be mixed.
+ * pairs with the batched block kernels, in groups of frequencies (at most
+ * FDFD_BATCH_MEMORY_GB, default 32, GB of factors per group), and solves the
+ * sources of each frequency together; the next group is factored on a
+ * second stream while the current one is solved.  Results are read back
+ * with fdfd_batch_result.  Uneven grid sizes may be mixed.
  * fdfd_batch_run works on the handle: like fdfd_set_grid, it releases every
  * factor the handle holds (their ids become FDFD_ERR_NOTFOUND) and leaves
  * the handle without a grid; call fdfd_set_grid / fdfd_set_eps again before
