@@ -235,19 +235,24 @@ def te_leg(eng, hip, d, eps_d, src_d, out_d, steps: int):
 def c3_leg(eng):
     """Batched sweep: 3 grids (256^2, 384x512, 512^2) x 64 wavelengths x 16
     point sources = 192 factorizations and 3072 refined solves (mixed, to
-    1e-10) in one fdfd_batch_run.  Wall time of the call (eps and sources
-    uploaded inside it; results stay on the device)."""
+    1e-10) in one fdfd_batch_run.  Wall time of the second of two identical
+    calls (eps and sources uploaded inside it; results stay on the device)."""
     import fdfd_oracle as O
     jobs = O.config_c3()
     spec = [(_cgrid(dd.grid), 1, dd.eps_r, oms, srcs) for dd, oms, srcs in jobs]
     nsolve = sum(len(oms) * srcs.shape[2] for _, oms, srcs in jobs)
+    # the first call also maps ~90 GB of fresh device memory (results and
+    # factor slabs); it is reported as cold_wall_s, the second call is timed
+    t0 = time.perf_counter()
+    eng.batch_solve(spec, precision="mixed", tol=1e-10, fetch=[])
+    cold = time.perf_counter() - t0
     t0 = time.perf_counter()
     _, st = eng.batch_solve(spec, precision="mixed", tol=1e-10, fetch=[])
     wall = time.perf_counter() - t0
     return {"workload": "3 grids (256^2, 384x512, 512^2) x 64 wavelengths (1.50-1.60 um) x 16 "
                         "point sources, complex64 factors + FP64 FGMRES to 1e-10, one call",
             "solves": nsolve, "factorizations": 192, "wall_s": wall, "solves_per_s": nsolve / wall,
-            "factor_ms": st["t_factor_ms"], "solve_ms": st["t_solve_ms"],
+            "cold_wall_s": cold, "factor_ms": st["t_factor_ms"], "solve_ms": st["t_solve_ms"],
             "fgmres_its": st["iterations"], "worst_residual": st["residual"]}
 
 
@@ -288,6 +293,9 @@ def c5_leg(eng):
     out = {"workload": "TM eigenfrequencies, 768^2 rod lattice, nev=8, ncv=20, tol 1e-10"}
     ws = {}
     for pname in ("mixed", "fp64"):
+        # untimed first call: device buffers for the new grid size
+        eng.eigenfrequency(1, float(d.omega[0]), 8, ncv=20, precision=pname, tol=1e-10,
+                           want_fields=False)
         t0 = time.perf_counter()
         w, _, st = eng.eigenfrequency(1, float(d.omega[0]), 8, ncv=20, precision=pname, tol=1e-10,
                                       want_fields=False)
