@@ -195,7 +195,9 @@ template void launch_triinv<float>(hipStream_t, const TriJob*, int, int);
 // into the other half of a double buffer, so each step reads its operands
 // from LDS and the memory latency is hidden behind the recurrence.  Lane 0
 // loads the LDS operands of TS_U steps at a time before running them, so one
-// LDS round trip serves a batch instead of every step.
+// LDS round trip serves a batch instead of every step; the steps of a batch
+// are straight-line code (the last two cells' special steps are peeled off),
+// so the independent parts of neighbouring steps interleave.
 constexpr int TS_CH = 128;     // steps per staged chunk
 constexpr int TS_STEP = 10;    // doubles per step: 6 (rotations) or 10 (row of R)
 constexpr int TS_U = 8;        // steps whose LDS operands are loaded together
@@ -259,37 +261,39 @@ __global__ __launch_bounds__(128) void k_trisolve(const TrsJob* jobs, int n) {
         }
         if (comp) {
             const double* rb = buf + (c & 1) * TS_CH * TS_STEP;
-            // batches of TS_U steps: every operand of a batch (rotations and
-            // v[k + 1]; v[k + 1] is stored at step k + 1, after it is read)
-            // is loaded before the first step, from clamped indices
-            for (int i0 = 0; i0 < len; i0 += TS_U) {
+            // general steps k < n - 2 (rot1 on rows (k, k+1), rot2 on rows
+            // (k, n-1)); the final rotation (k = n - 2) is peeled off
+            const int lenG = max(0, min(len, n - 2 - k0));
+            auto gstep = [&](const double* r, cz nxt, int k) {
+                tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
+                tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
+                st(k, carry);
+                carry = nxt;
+            };
+            // full batches of TS_U steps as straight-line code (no branch
+            // between steps, so the independent parts of neighbouring steps
+            // interleave): every operand of a batch (rotations and v[k + 1];
+            // v[k + 1] is stored at step k + 1, after it is read) is loaded
+            // before the first step
+            int i0 = 0;
+            for (; i0 + TS_U <= lenG; i0 += TS_U) {
                 double rr[TS_U][tri::ROT];
                 cz nx[TS_U];
 #pragma unroll
                 for (int u = 0; u < TS_U; ++u) {
-                    const int i = min(i0 + u, len - 1);
 #pragma unroll
-                    for (int j = 0; j < tri::ROT; ++j) rr[u][j] = rb[tri::ROT * i + j];
-                    nx[u] = ld(min(k0 + i + 1, n - 1));
+                    for (int j = 0; j < tri::ROT; ++j) rr[u][j] = rb[tri::ROT * (i0 + u) + j];
+                    nx[u] = ld(k0 + i0 + u + 1);
                 }
 #pragma unroll
-                for (int u = 0; u < TS_U; ++u) {
-                    if (i0 + u < len) {
-                        const int k = k0 + i0 + u;
-                        const double* r = rr[u];
-                        if (k + 2 < n) {  // rot1 on rows (k, k+1), rot2 on rows (k, n-1)
-                            cz nxt = nx[u];
-                            tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
-                            tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
-                            st(k, carry);
-                            carry = nxt;
-                        } else {          // k = n-2: rot1 on rows (n-2, n-1)
-                            tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
-                            st(n - 2, carry);
-                            st(n - 1, tail);
-                        }
-                    }
-                }
+                for (int u = 0; u < TS_U; ++u) gstep(rr[u], nx[u], k0 + i0 + u);
+            }
+            for (; i0 < lenG; ++i0) gstep(rb + tri::ROT * i0, ld(k0 + i0 + 1), k0 + i0);
+            if (k0 + len - 1 == n - 2) {  // k = n-2: rot1 on rows (n-2, n-1)
+                const double* r = rb + tri::ROT * (len - 1);
+                tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
+                st(n - 2, carry);
+                st(n - 1, tail);
             }
         }
         __syncthreads();
@@ -312,47 +316,47 @@ __global__ __launch_bounds__(128) void k_trisolve(const TrsJob* jobs, int n) {
         }
         if (comp) {
             const double* rb = buf + (c & 1) * TS_CH * TS_STEP;
-            // batches of TS_U rows going down: rows of R and y[k] (stored
-            // only at step k) loaded before the first step of the batch
-            for (int kt = hi - 1; kt >= lo; kt -= TS_U) {
+            auto row = [&](int k) { return rb + tri::RROW * (k - lo); };
+            int kt = hi - 1;
+            if (c == 0) {  // rows n-1 and n-2 give xB and xA
+                xB = tri::zmul(ld(n - 1), tri::mkz(row(n - 1)[0], row(n - 1)[1]));
+                st(n - 1, xB);
+                if (n >= 2) {
+                    const double* r = row(n - 2);
+                    xA = tri::zmul(tri::zsub(ld(n - 2), tri::zmul(tri::mkz(r[8], r[9]), xB)),
+                                   tri::mkz(r[0], r[1]));
+                    st(n - 2, xA);
+                    x1 = xA;
+                    x2 = xB;
+                }
+                kt = n - 3;
+            }
+            auto gstep = [&](const double* r, cz yk, int k) {
+                cz y = yk;
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
+                const cz x = tri::zmul(y, tri::mkz(r[0], r[1]));
+                st(k, x);
+                x2 = x1;
+                x1 = x;
+            };
+            // full batches of TS_U rows going down as straight-line code:
+            // rows of R and y[k] (stored only at step k) loaded first
+            for (; kt - (TS_U - 1) >= lo; kt -= TS_U) {
                 double rr[TS_U][tri::RROW];
                 cz yy[TS_U];
 #pragma unroll
                 for (int u = 0; u < TS_U; ++u) {
-                    const int k = max(kt - u, lo);
 #pragma unroll
-                    for (int j = 0; j < tri::RROW; ++j) rr[u][j] = rb[tri::RROW * (k - lo) + j];
-                    yy[u] = ld(k);
+                    for (int j = 0; j < tri::RROW; ++j) rr[u][j] = row(kt - u)[j];
+                    yy[u] = ld(kt - u);
                 }
 #pragma unroll
-                for (int u = 0; u < TS_U; ++u) {
-                    const int k = kt - u;
-                    if (k >= lo) {
-                        const double* r = rr[u];
-                        const cz yk = yy[u];
-                        const cz dinv = tri::mkz(r[0], r[1]);
-                        if (k == n - 1) {
-                            xB = tri::zmul(yk, dinv);
-                            st(k, xB);
-                        } else if (k == n - 2) {
-                            xA = tri::zmul(tri::zsub(yk, tri::zmul(tri::mkz(r[8], r[9]), xB)), dinv);
-                            st(k, xA);
-                            x1 = xA;
-                            x2 = xB;
-                        } else {
-                            cz y = yk;
-                            y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
-                            y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
-                            y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
-                            y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
-                            const cz x = tri::zmul(y, dinv);
-                            st(k, x);
-                            x2 = x1;
-                            x1 = x;
-                        }
-                    }
-                }
+                for (int u = 0; u < TS_U; ++u) gstep(rr[u], yy[u], kt - u);
             }
+            for (; kt >= lo; --kt) gstep(row(kt), ld(kt), kt);
         }
         __syncthreads();
     }

@@ -356,6 +356,7 @@ void launch_diag_multi(hipStream_t s, const DiagJob* jobs_d, int njobs, int n, l
 // ---------------------------------------------------------------------------
 constexpr int TM_CH = 128;   // steps per staged chunk
 constexpr int TM_STEP = 10;  // doubles per step (6 rotation, 10 row of R)
+constexpr int TM_U = 8;      // steps whose LDS operands are loaded together
 
 static int tm_kt(int n) {
     // vectors (kt x (n + 1) complex) + staging (2 x TM_CH x TM_STEP doubles) <= 150 KB
@@ -421,20 +422,35 @@ __global__ __launch_bounds__(128) void k_trisolve_multi(const TrsJob* jobs, int 
         }
         if (comp) {
             const double* rb = buf + (c & 1) * TM_CH * TM_STEP;
-            for (int i = 0; i < len; ++i) {
-                const int k = k0 + i;
-                const double* r = rb + tri::ROT * i;
-                if (k + 2 < n) {
-                    cz nxt = ld(k + 1);
-                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
-                    tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
-                    st(k, carry);
-                    carry = nxt;
-                } else {
-                    tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
-                    st(n - 2, carry);
-                    st(n - 1, tail);
+            // general steps k < n - 2; the final rotation (k = n - 2) is
+            // peeled off, so batches of TM_U steps run as straight-line code
+            // with their operands loaded first
+            const int lenG = max(0, min(len, n - 2 - k0));
+            auto gstep = [&](const double* r, cz nxt, int k) {
+                tri::rot(r[0], tri::mkz(r[1], r[2]), carry, nxt);
+                tri::rot(r[3], tri::mkz(r[4], r[5]), carry, tail);
+                st(k, carry);
+                carry = nxt;
+            };
+            int i0 = 0;
+            for (; i0 + TM_U <= lenG; i0 += TM_U) {
+                double rr[TM_U][tri::ROT];
+                cz nx[TM_U];
+#pragma unroll
+                for (int u = 0; u < TM_U; ++u) {
+#pragma unroll
+                    for (int j = 0; j < tri::ROT; ++j) rr[u][j] = rb[tri::ROT * (i0 + u) + j];
+                    nx[u] = ld(k0 + i0 + u + 1);
                 }
+#pragma unroll
+                for (int u = 0; u < TM_U; ++u) gstep(rr[u], nx[u], k0 + i0 + u);
+            }
+            for (; i0 < lenG; ++i0) gstep(rb + tri::ROT * i0, ld(k0 + i0 + 1), k0 + i0);
+            if (k0 + len - 1 == n - 2) {
+                const double* r = rb + tri::ROT * (len - 1);
+                tri::rot(r[0], tri::mkz(r[1], r[2]), carry, tail);
+                st(n - 2, carry);
+                st(n - 1, tail);
             }
         }
         __syncthreads();
@@ -455,30 +471,45 @@ __global__ __launch_bounds__(128) void k_trisolve_multi(const TrsJob* jobs, int 
         }
         if (comp) {
             const double* rb = buf + (c & 1) * TM_CH * TM_STEP;
-            for (int k = hi - 1; k >= lo; --k) {
-                const double* r = rb + tri::RROW * (k - lo);
-                const cz yk = ld(k);
-                const cz dinv = tri::mkz(r[0], r[1]);
-                if (k == n - 1) {
-                    xB = tri::zmul(yk, dinv);
-                    st(k, xB);
-                } else if (k == n - 2) {
-                    xA = tri::zmul(tri::zsub(yk, tri::zmul(tri::mkz(r[8], r[9]), xB)), dinv);
-                    st(k, xA);
+            auto row = [&](int k) { return rb + tri::RROW * (k - lo); };
+            int kt = hi - 1;
+            if (c == 0) {  // rows n-1 and n-2 give xB and xA
+                xB = tri::zmul(ld(n - 1), tri::mkz(row(n - 1)[0], row(n - 1)[1]));
+                st(n - 1, xB);
+                if (n >= 2) {
+                    const double* r = row(n - 2);
+                    xA = tri::zmul(tri::zsub(ld(n - 2), tri::zmul(tri::mkz(r[8], r[9]), xB)),
+                                   tri::mkz(r[0], r[1]));
+                    st(n - 2, xA);
                     x1 = xA;
                     x2 = xB;
-                } else {
-                    cz y = yk;
-                    y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
-                    y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
-                    y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
-                    y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
-                    const cz x = tri::zmul(y, dinv);
-                    st(k, x);
-                    x2 = x1;
-                    x1 = x;
                 }
+                kt = n - 3;
             }
+            auto gstep = [&](const double* r, cz yk, int k) {
+                cz y = yk;
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[2], r[3]), x1));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[4], r[5]), x2));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[6], r[7]), xA));
+                y = tri::zsub(y, tri::zmul(tri::mkz(r[8], r[9]), xB));
+                const cz x = tri::zmul(y, tri::mkz(r[0], r[1]));
+                st(k, x);
+                x2 = x1;
+                x1 = x;
+            };
+            for (; kt - (TM_U - 1) >= lo; kt -= TM_U) {
+                double rr[TM_U][tri::RROW];
+                cz yy[TM_U];
+#pragma unroll
+                for (int u = 0; u < TM_U; ++u) {
+#pragma unroll
+                    for (int j = 0; j < tri::RROW; ++j) rr[u][j] = row(kt - u)[j];
+                    yy[u] = ld(kt - u);
+                }
+#pragma unroll
+                for (int u = 0; u < TM_U; ++u) gstep(rr[u], yy[u], kt - u);
+            }
+            for (; kt >= lo; --kt) gstep(row(kt), ld(kt), kt);
         }
         __syncthreads();
     }
