@@ -257,6 +257,127 @@ function solve(d::FDFD.Device, pol::FDFD.Polarization=FDFD.TM; precision::Symbol
     return length(fields) == 1 ? fields[1] : fields
 end
 
+"""
+    solve_sources(d::Device, srcs::AbstractArray{<:Number,3}, pol=TM;
+                  precision=:mixed, tol=1e-10)
+
+Several sources per frequency on one factorization: `fdfd_solve` with
+`nrhs = size(srcs, 3)` runs a lockstep multi-source FGMRES (16 sources per
+pass over the factor blocks).  Returns, per frequency of `d.ω`, a vector of
+`FieldTM`/`FieldTE`, one per source.  The reference solves one source per
+`solve` call (src/solver/driven.jl:4-59).
+"""
+function solve_sources(d::FDFD.Device, srcs::AbstractArray{<:Number,3},
+                       pol::FDFD.Polarization=FDFD.TM; precision::Symbol=:mixed, tol::Real=1e-10,
+                       handle::Handle=default_handle())
+    h = handle
+    (Nx, Ny) = set_problem!(h, d)
+    (size(srcs, 1), size(srcs, 2)) == (Nx, Ny) ||
+        throw(DimensionMismatch("srcs must be Nx × Ny × nsrc"))
+    ns = size(srcs, 3)
+    src = Array{ComplexF64,3}(srcs)
+    F = pol == FDFD.TM ? FDFD.FieldTM : FDFD.FieldTE
+    fields = Vector{Vector{F}}()
+    for ω in d.ω
+        fid = Ref{Cint}(0)
+        st = Ref(Stats())
+        check(h, ccall((:fdfd_factor, libfdfd), Cint,
+                       (Ptr{Cvoid}, Cint, Cdouble, Cint, Ref{Cint}, Ref{Stats}),
+                       h.ptr, Int(pol), ω, precision_code(precision), fid, st))
+        out = Array{ComplexF64}(undef, Nx, Ny, 3, ns)
+        rc = GC.@preserve src out ccall((:fdfd_solve, libfdfd), Cint,
+                                         (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64},
+                                          Ptr{ComplexF64}, Cdouble, Ref{Stats}),
+                                         h.ptr, fid[], ns, src, out, tol, st)
+        ccall((:fdfd_release_factor, libfdfd), Cint, (Ptr{Cvoid}, Cint), h.ptr, fid[])
+        check(h, rc)
+        push!(fields, [F(d.grid, ω, out[:, :, :, r]) for r in 1:ns])
+    end
+    return fields
+end
+
+"""
+    solve_batch(jobs; precision=:mixed, tol=1e-10) -> (results, stats)
+
+Many devices × frequencies × sources in one `fdfd_batch_run`.  `jobs` is a
+vector of `(d::Device, pol, srcs::AbstractArray{<:Number,3})`: each device's
+grid, `ϵᵣ` and frequency list `d.ω`, with `size(srcs, 3)` sources solved at
+every frequency.  Grids may differ.  `results[j][iω][isrc]` is a
+`FieldTM`/`FieldTE`.  The handle's own factors are released and its grid is
+reset (`set_problem!` before the next `solve`).  Replaces a loop of
+`solve` calls over devices, frequencies and sources (driven.jl:11).
+"""
+function solve_batch(jobs; precision::Symbol=:mixed, tol::Real=1e-10,
+                     handle::Handle=default_handle())
+    h = handle
+    bref = Ref{Ptr{Cvoid}}(C_NULL)
+    check(h, ccall((:fdfd_batch_create, libfdfd), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                   h.ptr, bref))
+    b = bref[]
+    try
+        ids = Cint[]
+        for (d, pol, srcs) in jobs
+            g = d.grid
+            (Nx, Ny) = size(g)
+            eps = Array{ComplexF64}(d.ϵᵣ)
+            om = Float64.(collect(d.ω))
+            src = Array{ComplexF64,3}(srcs)
+            jid = Ref{Cint}(0)
+            GC.@preserve eps om src check(h, ccall((:fdfd_batch_add, libfdfd), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Cint, Cint, Cdouble, Cint,
+                 Ptr{ComplexF64}, Cint, Ptr{Cdouble}, Cint, Ptr{ComplexF64}, Ref{Cint}),
+                b, Nx, Ny, FDFD.dx(g), FDFD.dy(g), g.Npml[1], g.Npml[2], g.L₀, Int(pol),
+                eps, length(om), om, size(src, 3), src, jid))
+            push!(ids, jid[])
+        end
+        st = Ref(Stats())
+        check(h, ccall((:fdfd_batch_run, libfdfd), Cint, (Ptr{Cvoid}, Cint, Cdouble, Ref{Stats}),
+                       b, precision_code(precision), tol, st))
+        results = Any[]
+        for ((d, pol, srcs), jid) in zip(jobs, ids)
+            (Nx, Ny) = size(d.grid)
+            F = pol == FDFD.TM ? FDFD.FieldTM : FDFD.FieldTE
+            perω = Vector{Vector{F}}()
+            for (io, ω) in enumerate(d.ω)
+                fs = F[]
+                for r in 1:size(srcs, 3)
+                    out = Array{ComplexF64}(undef, Nx, Ny, 3)
+                    GC.@preserve out check(h, ccall((:fdfd_batch_result, libfdfd), Cint,
+                        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{ComplexF64}), b, jid, io - 1, r - 1, out))
+                    push!(fs, F(d.grid, ω, out))
+                end
+                push!(perω, fs)
+            end
+            push!(results, perω)
+        end
+        return results, st[]
+    finally
+        ccall((:fdfd_batch_destroy, libfdfd), Cint, (Ptr{Cvoid},), b)
+    end
+end
+
+"""
+    autotune!(d::Device, pol=TM; precision=:mixed) -> NamedTuple
+
+Times every variant of the factorization kernels on `d`'s grid and `ϵᵣ` at
+`d.ω[1]` (`fdfd_autotune`) and keeps the fastest on the handle; with
+`ENV["FDFD_TUNE_CACHE"]` naming a file the choice is recorded there and read
+by later handles.  No counterpart in the reference.
+"""
+function autotune!(d::FDFD.Device, pol::FDFD.Polarization=FDFD.TM; precision::Symbol=:mixed,
+                   handle::Handle=default_handle())
+    h = handle
+    set_problem!(h, d)
+    st = Ref(Stats())
+    check(h, ccall((:fdfd_autotune, libfdfd), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cint, Ref{Stats}),
+                   h.ptr, Int(pol), Float64(d.ω[1]), precision_code(precision), st))
+    knobs = Vector{Cint}(undef, 5)
+    check(h, ccall((:fdfd_get_tuning, libfdfd), Cint, (Ptr{Cvoid}, Ptr{Cint}, Cint),
+                   h.ptr, knobs, 5))
+    return (step_ms=st[].t_factor_ms, gemm_fp16x3=knobs[1] == 1, trailing_fp16x3=knobs[2] == 1,
+            fused_panels=knobs[3] == 1, gemm_waves=Int(knobs[4]), gemm3m=knobs[5] == 1)
+end
+
 # ---- reverse-mode derivative of solve with respect to ϵᵣ (ChainRulesCore) --
 # The reference has no derivative.  The forward pass is `solve`, keeping every
 # frequency's factor alive until the pullback has run; the pullback is one
