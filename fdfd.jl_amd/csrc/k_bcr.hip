@@ -409,8 +409,18 @@ void launch_gjreset(hipStream_t s, const LineJob* jobs_d, int njobs, int n) {
 //   re = t1 - t2, im = t3 - t1 - t2.
 // ---------------------------------------------------------------------------
 template <typename R> struct gemm_cfg;
-template <> struct gemm_cfg<float> { static constexpr int TM = 128, TN = 128; };
-template <> struct gemm_cfg<double> { static constexpr int TM = 128, TN = 64; };
+template <> struct gemm_cfg<float> {
+    static constexpr int TM = 128, TN = 128;
+    static constexpr int PA = TM + 16, ROT = 0;
+};
+// complex128: the A loader's four threads of a row write k-rows 4 apart in the
+// same cycle; rotating each group of four k-rows by 8 columns (ROT) spreads
+// them over the banks, and a row pitch of TM + 48 doubles keeps the fragment
+// reads of neighbouring k-rows in different bank halves
+template <> struct gemm_cfg<double> {
+    static constexpr int TM = 128, TN = 64;
+    static constexpr int PA = TM + 48, ROT = 8;
+};
 constexpr int GKT = 16;
 
 // WR x WC waves per workgroup, each owning a (TM/WR) x (TN/WC) sub-tile.
@@ -423,7 +433,9 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm(const GemmJob* jobs, int 
     constexpr int TM = gemm_cfg<R>::TM, TN = gemm_cfg<R>::TN;
     constexpr int WM = TM / WR, WN = TN / WC;
     constexpr int BM = WM / 16, BN = WN / 16;
-    constexpr int PA = TM + 16, PB = TN + 16;
+    constexpr int PA = gemm_cfg<R>::PA, PB = TN + 16, ROT = gemm_cfg<R>::ROT;
+    // column of A row i at k-row k of a stage
+    auto acol = [](int k, int i) { return i + ROT * ((k >> 2) & 3); };
     constexpr int AE = TM * GKT / NT;   // A elements per thread per stage (consecutive k)
     constexpr int TPR = GKT / AE;       // threads per A row
     constexpr int TPK = NT / GKT;       // threads per B k-row
@@ -476,8 +488,8 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm(const GemmJob* jobs, int 
     auto store = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < AE; ++e) {
-            sAr[buf][ak + e][arow] = ra[e].x;
-            sAi[buf][ak + e][arow] = ra[e].y;
+            sAr[buf][ak + e][acol(ak + e, arow)] = ra[e].x;
+            sAi[buf][ak + e][acol(ak + e, arow)] = ra[e].y;
         }
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
@@ -495,19 +507,15 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm(const GemmJob* jobs, int 
             cim[bi][bj] = acc_t{0, 0, 0, 0};
             if constexpr (G3) c2[bi][bj] = acc_t{0, 0, 0, 0};
         }
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int s = 0; s < ns; ++s) {
-        const int cur = s & 1;
-        if (s + 1 < ns) load(s + 1);
+    // the products of one stage (operands in LDS buffer cur)
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < GKT; kk += 4) {
             R ar[BM], an[BM], ai[BM], br[BN], bi[BN];
 #pragma unroll
             for (int q = 0; q < BM; ++q) {
-                ar[q] = sAr[cur][kk + lk][wr * WM + q * 16 + lr];
-                ai[q] = sAi[cur][kk + lk][wr * WM + q * 16 + lr];
+                ar[q] = sAr[cur][kk + lk][acol(kk + lk, wr * WM + q * 16 + lr)];
+                ai[q] = sAi[cur][kk + lk][acol(kk + lk, wr * WM + q * 16 + lr)];
                 an[q] = -ai[q];
             }
 #pragma unroll
@@ -554,7 +562,33 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm(const GemmJob* jobs, int 
                     for (int q = 0; q < BN; ++q) cim[p][q] = M::run(ai[p], br[q], cim[p][q]);
             }
         }
-        if (s + 1 < ns) store(cur ^ 1);
+    };
+    // Ping-pong stages (as in k_gemm_b3): between two barriers a wave
+    // multiplies the current stage and stores the next one (loaded into
+    // registers one interval earlier) into the other buffer, then issues the
+    // loads of the stage after it; the two waves of each SIMD (wv and wv + 4
+    // with 8 waves) do these in opposite orders, so one's memory work
+    // overlaps the other's MFMAs.  Every wave passes the same barriers.
+    load(0);
+    store(0);
+    if (ns > 1) load(1);
+    __syncthreads();
+    const bool mem_first = WR * WC == 8 && wv >= 4;
+    for (int s = 0; s < ns; ++s) {
+        const int cur = s & 1;
+        if (mem_first) {
+            if (s + 1 < ns) {
+                store(cur ^ 1);
+                if (s + 2 < ns) load(s + 2);
+            }
+            compute(cur);
+        } else {
+            compute(cur);
+            if (s + 1 < ns) {
+                store(cur ^ 1);
+                if (s + 2 < ns) load(s + 2);
+            }
+        }
         __syncthreads();
     }
     C* out = (C*)J.out;
