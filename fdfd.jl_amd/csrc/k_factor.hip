@@ -765,7 +765,7 @@ template void launch_thin<float>(hipStream_t, const LineJob*, int, int, int, int
 // The rank-(nb1 + nb2) product runs in chunks of PMAX columns of V through
 // the same LDS tiles.
 template <typename R>
-__global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, int nb1, int nb2) {
+__global__ __launch_bounds__(256, 3) void k_trailing(const LineJob* jobs, int k0, int nb1, int nb2) {
     using C = cplx<R>;
     using M = mfma_t<R>;
     using acc_t = typename M::acc;
@@ -779,14 +779,85 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
     const bool fused = nb2 > 0;
     const int kend = fused ? k0 + PMAX + nb2 : k0 + nb1;  // end of the updated columns
     const int k2 = k0 + PMAX;                               // start of K2 (fused)
-    __shared__ R Vr[64][NB + 1], Vi[64][NB + 1];
-    __shared__ R Zr[NB][64 + 1], Zi[NB][64 + 1];
+    // V and Z pass through LDS KH columns at a time (a quarter of the LDS of a
+    // whole 32-column chunk, so more workgroups share a CU)
+    constexpr int KH = 16;
+    __shared__ R Vr[64][KH + 1], Vi[64][KH + 1];
+    __shared__ R Zr[KH][64 + 1], Zi[KH][64 + 1];
     __shared__ int rslot[64];
     C* A = (C*)J.Y;
     const C* Vb = (const C*)J.Vbuf;
     const C* Zg = (const C*)J.Z;
     if (t < 64) rslot[t] = (i0 + t < n) ? gload_s(J.iprow + i0 + t) - k0 : -1;
-    // old tile values and the pivot-row slots
+    acc_t are[2][2], aim[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            are[bi][bj] = acc_t{0, 0, 0, 0};
+            aim[bi][bj] = acc_t{0, 0, 0, 0};
+        }
+    __syncthreads();  // rslot
+    const int nchunk = fused ? 2 : 1;
+    bool first = true;
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int koff = ch * PMAX, nbc = (ch == 0) ? nb1 : nb2;
+        for (int h0 = 0; h0 < nbc; h0 += KH) {
+            if (!first) __syncthreads();  // the previous columns' tiles are consumed
+            first = false;
+            for (int e = t; e < 64 * KH; e += 256) {
+                const int ii = e / KH, k = e % KH, i = i0 + ii;
+                C v = mk<C>(R(0), R(0));
+                // the first panel's pivot rows take no V1 part in a fused pair
+                const bool skip = fused && ch == 0 && rslot[ii] >= 0 && rslot[ii] < PMAX;
+                if (i < n && h0 + k < nbc && !skip) v = gload(Vb + (long)i * VMAX + koff + h0 + k);
+                Vr[ii][k] = v.x;
+                Vi[ii][k] = v.y;
+            }
+            for (int e = t; e < 64 * KH; e += 256) {
+                const int k = e / 64, jj = e % 64, j = j0 + jj;
+                C v = mk<C>(R(0), R(0));
+                // Z1' : Z1 with the K2 columns zeroed
+                const bool skip = fused && ch == 0 && j >= k2 && j < kend;
+                if (j < n && h0 + k < nbc && !skip) v = gload(Zg + (long)(koff + h0 + k) * n + j);
+                Zr[k][jj] = v.x;
+                Zi[k][jj] = v.y;
+            }
+            __syncthreads();
+            const int nbr = (min(KH, nbc - h0) + 3) & ~3;
+            for (int kk = 0; kk < nbr; kk += 4) {
+                R ar[2], an[2], ai[2], br[2], bi_[2];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    ar[q] = Vr[wr * 32 + q * 16 + lr][kk + lk];
+                    ai[q] = Vi[wr * 32 + q * 16 + lr][kk + lk];
+                    an[q] = -ai[q];
+                    br[q] = Zr[kk + lk][wc * 32 + q * 16 + lr];
+                    bi_[q] = Zi[kk + lk][wc * 32 + q * 16 + lr];
+                }
+                // passes over the four blocks, so consecutive MFMAs use
+                // different accumulators (each accumulator's order unchanged)
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) are[bi][bj] = M::run(ar[bi], br[bj], are[bi][bj]);
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) aim[bi][bj] = M::run(ar[bi], bi_[bj], aim[bi][bj]);
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) are[bi][bj] = M::run(an[bi], bi_[bj], are[bi][bj]);
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) aim[bi][bj] = M::run(ai[bi], br[bj], aim[bi][bj]);
+            }
+        }
+    }
+    // old tile values and the pivot-row slots (loaded after the products, so
+    // they hold no registers during them; other workgroups cover the latency)
     C old[2][4][2];
     int slot[2][4];
 #pragma unroll
@@ -811,59 +882,6 @@ __global__ __launch_bounds__(256) void k_trailing(const LineJob* jobs, int k0, i
                 }
             }
         }
-    acc_t are[2][2], aim[2][2];
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            are[bi][bj] = acc_t{0, 0, 0, 0};
-            aim[bi][bj] = acc_t{0, 0, 0, 0};
-        }
-    __syncthreads();  // rslot
-    const int nchunk = fused ? 2 : 1;
-    for (int ch = 0; ch < nchunk; ++ch) {
-        const int koff = ch * PMAX, nbc = (ch == 0) ? nb1 : nb2;
-        if (ch > 0) __syncthreads();  // the previous chunk's tiles are consumed
-        for (int e = t; e < 64 * NB; e += 256) {
-            const int ii = e / NB, k = e % NB, i = i0 + ii;
-            C v = mk<C>(R(0), R(0));
-            // the first panel's pivot rows take no V1 part in a fused pair
-            const bool skip = fused && ch == 0 && rslot[ii] >= 0 && rslot[ii] < PMAX;
-            if (i < n && k < nbc && !skip) v = gload(Vb + (long)i * VMAX + koff + k);
-            Vr[ii][k] = v.x;
-            Vi[ii][k] = v.y;
-        }
-        for (int e = t; e < 64 * NB; e += 256) {
-            const int k = e / 64, jj = e % 64, j = j0 + jj;
-            C v = mk<C>(R(0), R(0));
-            // Z1' : Z1 with the K2 columns zeroed
-            const bool skip = fused && ch == 0 && j >= k2 && j < kend;
-            if (j < n && k < nbc && !skip) v = gload(Zg + (long)(koff + k) * n + j);
-            Zr[k][jj] = v.x;
-            Zi[k][jj] = v.y;
-        }
-        __syncthreads();
-        const int nbr = (nbc + 3) & ~3;
-        for (int kk = 0; kk < nbr; kk += 4) {
-            R ar[2], ai[2], br[2], bi_[2];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                ar[q] = Vr[wr * 32 + q * 16 + lr][kk + lk];
-                ai[q] = Vi[wr * 32 + q * 16 + lr][kk + lk];
-                br[q] = Zr[kk + lk][wc * 32 + q * 16 + lr];
-                bi_[q] = Zi[kk + lk][wc * 32 + q * 16 + lr];
-            }
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    are[bi][bj] = M::run(ar[bi], br[bj], are[bi][bj]);
-                    are[bi][bj] = M::run(-ai[bi], bi_[bj], are[bi][bj]);
-                    aim[bi][bj] = M::run(ar[bi], bi_[bj], aim[bi][bj]);
-                    aim[bi][bj] = M::run(ai[bi], br[bj], aim[bi][bj]);
-                }
-        }
-    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
