@@ -511,8 +511,6 @@ def main():
     for p_ in (eps_d, src_d, out_d):
         hip.free(p_)
     if not a.no_legs:
-        print("# C3 leg", file=sys.stderr, flush=True)
-        legs["c3_batch"] = c3_leg(eng)
         print("# C4 leg", file=sys.stderr, flush=True)
         legs["c4_adjoint"] = c4_leg(eng)
         print("# C5 leg", file=sys.stderr, flush=True)
@@ -521,6 +519,10 @@ def main():
         eng.set_grid(grid)
         legs["modes"] = mode_leg(eng, d, not a.no_cpu)
         legs["modulated"] = modulated_leg(eng, not a.no_cpu)
+        # last: the batch frees ~26 GB of results when it ends, and the
+        # driver's clearing of freed memory slows the GPU work right after
+        print("# C3 leg", file=sys.stderr, flush=True)
+        legs["c3_batch"] = c3_leg(eng)
     cpu = None
     if not a.no_cpu:
         print("# CPU baseline (SuperLU at C1 and the metric size)", file=sys.stderr, flush=True)

@@ -259,16 +259,27 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     }
     // two slab halves (factor the next group while solving the current one);
     // one slab: no overlap; none: the factors allocate their arenas one by one
-    int halves = 2;
-    if (hipMalloc((void**)&h->slab, 2 * slab_need) != hipSuccess) {
-        (void)hipGetLastError();
-        halves = 1;
-        if (hipMalloc((void**)&h->slab, slab_need) != hipSuccess) {
-            (void)hipGetLastError();
-            h->slab = nullptr;
-            halves = 0;
+    // (the slab of an earlier batch on this handle is reused when it is large
+    // enough, and kept for the next one)
+    int halves = 0;
+    for (int want : {2, 1}) {
+        const size_t bytes = (size_t)want * slab_need;
+        if (h->bslab_size >= bytes) {
+            halves = want;
+            break;
         }
+        if (h->bslab) (void)hipFree(h->bslab);
+        h->bslab = nullptr;
+        h->bslab_size = 0;
+        if (hipMalloc((void**)&h->bslab, bytes) == hipSuccess) {
+            h->bslab_size = bytes;
+            halves = want;
+            break;
+        }
+        (void)hipGetLastError();
+        h->bslab = nullptr;
     }
+    h->slab = halves ? h->bslab : nullptr;
     // the batch's handle only uses the slab's range (its arena release skips
     // slab pointers); the builder carves the halves
     h->slab_size = (size_t)halves * slab_need;
@@ -283,7 +294,6 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
                 halves, slab_need / 1e9, groups.size(), secs(ta0, now()));
     Builder bld;
     if ((rc = builder_init(bld, h))) {
-        (void)hipFree(h->slab);
         h->slab = nullptr;
         h->slab_size = h->slab_used = 0;
         free_results(b);
@@ -442,8 +452,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     if (dsrc) (void)hipFree(dsrc);
     builder_release_borrowed(bld);
     (void)hipStreamSynchronize(h->stream);
-    (void)hipFree(h->slab);
-    h->slab = nullptr;
+    h->slab = nullptr;  // the slab itself stays with the handle (bslab)
     h->slab_size = h->slab_used = 0;
     h->has_grid = h->has_eps = false;
     if (stats) {  // also on failure: what was reached

@@ -497,7 +497,7 @@ int fdfd_destroy(fdfd_handle* hh) {
     free_multi_work(h);
     void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
-                    h->dpart, h->rpart, h->hdev};
+                    h->dpart, h->rpart, h->hdev, h->bslab};
     for (void* b : bufs) dfree(b);
     prof_resolve(h->prof);
     for (auto e : h->prof.pool) (void)hipEventDestroy(e);
@@ -517,16 +517,23 @@ int fdfd_set_grid(fdfd_handle* hh, int Nx, int Ny, double dx, double dy, int npm
     (void)hipSetDevice(h->device);
     for (auto& kv : h->factors) free_factor(h, kv.second);
     h->factors.clear();
-    arena_clear(h);
-    dfree(h->eps);
-    h->eps = nullptr;
+    // The pooled factor arenas and the eps buffer are kept when the grid keeps
+    // its shape (a new design or frequency sweep on the same grid): freeing
+    // and reallocating gigabytes makes the driver clear the memory again,
+    // which slows the GPU work that follows.
+    const bool same_shape = h->has_grid && h->g.Nx == Nx && h->g.Ny == Ny;
+    if (!same_shape) arena_clear(h);
     h->g = GridDesc{Nx, Ny, dx, dy, npml_x, npml_y, L0};
     h->has_grid = true;
     h->has_eps = false;
     const long N = (long)Nx * Ny;
-    h->eps_n = 0;
-    HIPCHK(h, hipMalloc(&h->eps, sizeof(double2) * N));
-    h->eps_n = N;
+    if (h->eps_n < N) {
+        dfree(h->eps);
+        h->eps = nullptr;
+        h->eps_n = 0;
+        HIPCHK(h, hipMalloc(&h->eps, sizeof(double2) * N));
+        h->eps_n = N;
+    }
     return FDFD_OK;
 }
 

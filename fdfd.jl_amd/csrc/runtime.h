@@ -128,6 +128,11 @@ struct Handle {
     // the batch's largest group (bump pointer, reset per group)
     char* slab = nullptr;
     size_t slab_size = 0, slab_used = 0;
+    // the batch slab stays allocated between batches (freeing and
+    // reallocating tens of GB makes the driver clear the memory again, which
+    // slows the GPU work that follows for seconds); freed by fdfd_destroy
+    char* bslab = nullptr;
+    size_t bslab_size = 0;
     long eps_n = 0;      // cells the eps buffer holds
 };
 

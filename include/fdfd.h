@@ -319,7 +319,10 @@ int fdfd_flux_d_noalloc(fdfd_handle* h, int pol, int Nx, int Ny, double dx,
  * FDFD_BATCH_MEMORY_GB, default 32, GB of factors per group), and solves the
  * sources of each frequency together; the next group is factored on a
  * second stream while the current one is solved.  Results are read back
- * with fdfd_batch_result.  Uneven grid sizes may be mixed.
+ * with fdfd_batch_result.  Uneven grid sizes may be mixed.  The factor
+ * memory of the run (two groups) stays allocated on the handle for later
+ * batches until fdfd_destroy: freeing and reallocating tens of GB costs
+ * seconds of slower GPU work while the driver clears the memory.
  * fdfd_batch_run works on the handle: like fdfd_set_grid, it releases every
  * factor the handle holds (their ids become FDFD_ERR_NOTFOUND) and leaves
  * the handle without a grid; call fdfd_set_grid / fdfd_set_eps again before
