@@ -213,54 +213,85 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
         }
     }
     __syncthreads();
-    if (w0) {
-        C W[NB];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            W[c] = mk<C>(R(0), R(0));
-            if (act && c < nb) W[c] = Ws[lane][c];
+    {
+        // GEPP on the window by all waves: lane l of every wave holds window
+        // row wrow[l], wave w its columns CPW w .. CPW w + CPW - 1.  At step
+        // cc the wave owning column cc finds the pivot (argmax of |.|_1 over
+        // the active rows; the lowest lane among exact ties, i.e. the
+        // smallest row index, GEPP's tie rule) and publishes the column and
+        // the pivot lane through LDS (double-buffered, one barrier per step);
+        // every wave then forms the multipliers and updates its own columns
+        // with the pivot row's entries (readlane within the wave).  The
+        // arithmetic per entry is that of a single-wave elimination.
+        constexpr int NW = WP_THREADS / 64, CPW = NB / NW;
+        static_assert(CPW * NW == NB, "columns per wave");
+        __shared__ C colb[2][WIN];
+        __shared__ int plb[2];
+        const int wv = t >> 6;
+        if (!w0) {
+            row = wrow[lane];
+            act = row >= 0;
         }
-        static_for<0, NB>([&](auto ic) {
-            constexpr int cc = decltype(ic)::value;
-            if (cc < nb) {
-                // argmax of |.|_1 over the active window rows; lanes hold the
-                // window in ascending row order, so the lowest lane among exact
-                // ties is the smallest row index (GEPP's tie rule)
-                const R mag = cabs1(W[cc]);
+        C W[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            const int c = CPW * wv + j;
+            W[j] = mk<C>(R(0), R(0));
+            if (act && c < nb) W[j] = Ws[lane][c];
+        }
+        for (int cc = 0; cc < nb; ++cc) {
+            const int b = cc & 1;
+            if (wv == cc / CPW) {
+                const int jj = cc % CPW;
+                C x = W[0];
+#pragma unroll
+                for (int j = 1; j < CPW; ++j)
+                    if (j == jj) x = W[j];
+                const R mag = cabs1(x);
                 const unsigned long long key = act ? mag_key(mag) : 0ULL;
                 const unsigned long long kmax = wave_max_u64(key);
                 unsigned long long hit = __ballot(act && key == kmax);
                 if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
                 const int pl = (int)__ffsll((long long)hit) - 1;  // wave-uniform
-                const R bm = act ? mag : R(-1);
+                colb[b][lane] = x;
+                if (lane == 0) plb[b] = pl;
                 if (lane == pl) {
                     prs[cc] = row;
-                    pmags[cc] = bm;
-                    act = false;
-                    mystep = cc;
-                }
-                // pivot-row entries are broadcast one at a time (uniform control
-                // flow); the pivot lane's registers are not modified in this step
-                const C ucc = mk<C>(bcast(W[cc].x, pl), bcast(W[cc].y, pl));
-                const R pivmag = cabs1(ucc);
-                const C m = csel(act && pivmag > R(0), cmul(W[cc], crcp(ucc)), mk<C>(R(0), R(0)));
-                if (act) W[cc] = m;
-#pragma unroll
-                for (int c = cc + 1; c < NB; ++c) {
-                    const C uc = mk<C>(bcast(W[c].x, pl), bcast(W[c].y, pl));
-                    if (act) W[c] = csub(W[c], cmul(m, uc));
+                    pmags[cc] = act ? mag : R(-1);
                 }
             }
-        });
-        // A pivot lane's registers are frozen from its step on: they hold its row
-        // of L (multipliers, c < step) and of U (c >= step).
+            __syncthreads();
+            const int pl = plb[b];
+            if (lane == pl) {
+                act = false;
+                mystep = cc;
+            }
+            const C ucc = colb[b][pl];
+            const R pivmag = cabs1(ucc);
+            const C m = csel(act && pivmag > R(0), cmul(colb[b][lane], crcp(ucc)), mk<C>(R(0), R(0)));
+            // pivot-row entries are broadcast one at a time (uniform control
+            // flow); the pivot lane's registers are not modified in this step
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) {
+                const int c = CPW * wv + j;
+                if (c == cc) {
+                    if (act) W[j] = m;
+                } else if (c > cc) {
+                    const C uc = mk<C>(bcast(W[j].x, pl), bcast(W[j].y, pl));
+                    if (act) W[j] = csub(W[j], cmul(m, uc));
+                }
+            }
+        }
+        // A pivot lane's registers are frozen from its step on: they hold its
+        // row of L (multipliers, c < step) and of U (c >= step).
         if (mystep >= 0) {
 #pragma unroll
-            for (int c = 0; c < NB; ++c) {
+            for (int j = 0; j < CPW; ++j) {
+                const int c = CPW * wv + j;
                 if (c < nb) {
                     const bool lo = c < mystep;
-                    Ls[mystep][c] = csel(lo, W[c], mk<C>(c == mystep ? R(1) : R(0), R(0)));
-                    Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[c]);
+                    Ls[mystep][c] = csel(lo, W[j], mk<C>(c == mystep ? R(1) : R(0), R(0)));
+                    Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[j]);
                 }
             }
         }

@@ -497,7 +497,7 @@ int fdfd_destroy(fdfd_handle* hh) {
     free_multi_work(h);
     void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
-                    h->dpart, h->rpart, h->hdev, h->bslab};
+                    h->dpart, h->rpart, h->hdev, h->bslab, h->io};
     for (void* b : bufs) dfree(b);
     prof_resolve(h->prof);
     for (auto e : h->prof.pool) (void)hipEventDestroy(e);
@@ -692,18 +692,21 @@ int fdfd_solve(fdfd_handle* hh, int factor_id, int nrhs, const fdfd_complex* src
     if (!find_factor(h, factor_id)) return fail(h, FDFD_ERR_NOTFOUND, "unknown factor id");
     (void)hipSetDevice(h->device);
     const long N = (long)h->g.Nx * h->g.Ny;
-    void *ds = nullptr, *dout = nullptr;
-    HIPCHK(h, hipMalloc(&ds, sizeof(double2) * N * nrhs));
-    if (hipMalloc(&dout, sizeof(double2) * N * 3 * nrhs) != hipSuccess) {
-        dfree(ds);
-        return fail(h, FDFD_ERR_OOM, "output allocation failed");
+    // device staging: sources then fields, in the handle's buffer (grown, not
+    // reallocated per call)
+    const size_t need = sizeof(double2) * N * 4 * (size_t)nrhs;
+    if (h->io_bytes < need) {
+        dfree(h->io);
+        h->io = nullptr;
+        h->io_bytes = 0;
+        HIPCHK(h, hipMalloc(&h->io, need));
+        h->io_bytes = need;
     }
-    int rc = FDFD_OK;
-    rc = stream_copy(h, ds, src, sizeof(double2) * N * nrhs, hipMemcpyHostToDevice);
+    double2* ds = (double2*)h->io;
+    double2* dout = ds + N * nrhs;
+    int rc = stream_copy(h, ds, src, sizeof(double2) * N * nrhs, hipMemcpyHostToDevice);
     if (!rc) rc = fdfd_solve_d(hh, factor_id, nrhs, ds, dout, tol, stats);
     if (!rc) rc = stream_copy(h, out, dout, sizeof(double2) * N * 3 * nrhs, hipMemcpyDeviceToHost);
-    dfree(ds);
-    dfree(dout);
     return rc;
 }
 

@@ -133,6 +133,10 @@ struct Handle {
     // slows the GPU work that follows for seconds); freed by fdfd_destroy
     char* bslab = nullptr;
     size_t bslab_size = 0;
+    // device staging of the host-pointer solve (sources, then fields), kept
+    // between calls and grown as needed
+    void* io = nullptr;
+    size_t io_bytes = 0;
     long eps_n = 0;      // cells the eps buffer holds
 };
 
