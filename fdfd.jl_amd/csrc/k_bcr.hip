@@ -652,25 +652,26 @@ template void launch_gemm<float>(hipStream_t, const GemmJob*, int, int, int, int
 // transposed, strip rows are block columns c = 16 rb + r and k runs over the
 // block rows through the gather, (r, k) = src[map[k]][c].  The strip is read
 // once into registers in the tile layout -- wave wv holds tiles kb = wv,
-// wv + 4, .. (at most SP_TPW for n <= MAX_LINE), lane l row l % 16 and
+// wv + SP_W, .. (at most SP_TPW for n <= MAX_LINE), lane l row l % 16 and
 // k = 32 kb + 8 (l / 16) + [0, 8) -- then each row's largest |re|, |im|
-// (lanes l, l ^ 16, l ^ 32, l ^ 48 of all four waves) gives its exponent and
+// (lanes l, l ^ 16, l ^ 32, l ^ 48 of all SP_W waves) gives its exponent and
 // the scaled fp16 planes are written, one wave per 1 KB tile.
-constexpr int SP_TPW = (MAX_LINE / 32 + 3) / 4;
+constexpr int SP_W = 8;  // waves per strip
+constexpr int SP_TPW = (MAX_LINE / 32 + SP_W - 1) / SP_W;
 
-__global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
+__global__ __launch_bounds__(64 * SP_W) void k_split(const SplitJob* jobs, int n) {
     const SplitJob J = jobs[blockIdx.y];
     const int rb = blockIdx.x, r0 = 16 * rb;
     const int nkb = b3_nkb(n);
     const float2* src = (const float2*)J.src;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int r = lane & 15, row = r0 + r, kq = 8 * (lane >> 4);
-    __shared__ float wmax[4][16];
+    __shared__ float wmax[SP_W][16];
     float2 v[SP_TPW][8];
     // loads: clamped addresses, no branches (entries past n are masked below)
 #pragma unroll
     for (int q = 0; q < SP_TPW; ++q) {
-        const int kb = wv + 4 * q;
+        const int kb = wv + SP_W * q;
         if (kb < nkb) {
             if (!J.trans) {
 #pragma unroll
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     float m = 0.f;
 #pragma unroll
     for (int q = 0; q < SP_TPW; ++q) {
-        const int kb = wv + 4 * q;
+        const int kb = wv + SP_W * q;
         if (kb < nkb) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -712,7 +713,9 @@ __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     m = fmaxf(m, __shfl_xor(m, 32, 64));
     if (lane < 16) wmax[wv][lane] = m;
     __syncthreads();
-    const float mx = fmaxf(fmaxf(wmax[0][r], wmax[1][r]), fmaxf(wmax[2][r], wmax[3][r]));
+    float mx = wmax[0][r];
+#pragma unroll
+    for (int w = 1; w < SP_W; ++w) mx = fmaxf(mx, wmax[w][r]);
     const int ex = split_exp(mx);
     if (t < 16) J.exps[r0 + t] = ex;
     const float sc = exp2i(ex);
@@ -720,7 +723,7 @@ __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
     _Float16* dst = (_Float16*)J.dst;
 #pragma unroll
     for (int q = 0; q < SP_TPW; ++q) {
-        const int kb = wv + 4 * q;
+        const int kb = wv + SP_W * q;
         if (kb < nkb) {
             h8v rh, rl, ih, il;
 #pragma unroll
@@ -743,7 +746,7 @@ __global__ __launch_bounds__(256) void k_split(const SplitJob* jobs, int n) {
 }
 
 void launch_split(hipStream_t s, const SplitJob* jobs_d, int njobs, int n) {
-    k_split<<<dim3(b3_nrb(n), njobs), 256, 0, s>>>(jobs_d, n);
+    k_split<<<dim3(b3_nrb(n), njobs), 64 * SP_W, 0, s>>>(jobs_d, n);
 }
 
 // GEMM on split operands (tiled planes, b3_plane): A as [row][k], B^T as
