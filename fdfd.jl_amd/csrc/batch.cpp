@@ -237,7 +237,7 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     // Two groups are held at once (one factored while the other is solved),
     // so the free memory is shared between two slabs.
     double budget = 32.0;
-    if (const char* eb = std::getenv("FDFD_BATCH_MEMORY_GB")) budget = std::max(1.0, std::atof(eb));
+    if (const char* eb = std::getenv("FDFD_BATCH_MEMORY_GB")) budget = std::max(1e-3, std::atof(eb));
     const double usable = std::min(0.4 * (double)freeb, budget * 1073741824.0);
     std::vector<int> group(b->jobs.size(), 1);
     std::vector<size_t> arena_bytes(b->jobs.size(), 0);
@@ -261,8 +261,12 @@ int fdfd_batch_run(fdfd_batch* b, int precision, double tol, fdfd_stats* stats) 
     // one slab: no overlap; none: the factors allocate their arenas one by one
     // (the slab of an earlier batch on this handle is reused when it is large
     // enough, and kept for the next one)
+    // FDFD_BATCH_LANES=1 runs the lanes in sequence (one slab half)
+    const char* el = std::getenv("FDFD_BATCH_LANES");
+    const bool one_lane = el && el[0] == '1';
     int halves = 0;
     for (int want : {2, 1}) {
+        if (one_lane && want == 2) continue;
         const size_t bytes = (size_t)want * slab_need;
         if (h->bslab_size >= bytes) {
             halves = want;
