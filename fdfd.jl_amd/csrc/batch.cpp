@@ -59,6 +59,7 @@ struct Builder {
                         (void*)b.eps})
             if (p) (void)hipFree(p);
         arena_clear(&b);
+        buf_clear(&b);
     }
 };
 

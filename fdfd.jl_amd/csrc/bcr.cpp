@@ -59,17 +59,39 @@ void rt_factor_arena_sizes(int n, int m, int precision, bool b3, size_t* blocks,
 // device tables for a set of factors built together (same grid and shape)
 // ---------------------------------------------------------------------------
 struct Tables {
+    Handle* h;
+    void* base = nullptr;  // one buffer holds every table
     SplitJob* split = nullptr;
     TriJob* tri = nullptr;
     FormJob* form = nullptr;
     LineJob* lj = nullptr;
     GemmJob* gemm = nullptr;
-    ~Tables() {
-        if (split) (void)hipFree(split);
-        if (tri) (void)hipFree(tri);
-        if (form) (void)hipFree(form);
-        if (lj) (void)hipFree(lj);
-        if (gemm) (void)hipFree(gemm);
+    explicit Tables(Handle* hh) : h(hh) {}
+    ~Tables() { buf_free(h, base); }
+};
+
+// Several host job tables packed into one device buffer (256-byte aligned
+// parts) and uploaded with one copy: add() the tables, then commit().
+struct TablePack {
+    std::vector<char> bytes;
+    std::vector<std::pair<void**, size_t>> parts;  // destination pointer, offset
+    template <typename T> void add(T** dst, const std::vector<T>& v) {
+        *dst = nullptr;
+        if (v.empty()) return;
+        const size_t off = (bytes.size() + 255) & ~(size_t)255;
+        bytes.resize(off + sizeof(T) * v.size());
+        std::memcpy(bytes.data() + off, v.data(), sizeof(T) * v.size());
+        parts.push_back({(void**)dst, off});
+    }
+    // *base receives the buffer (nullptr when every table is empty)
+    int commit(Handle* h, void** base) {
+        *base = nullptr;
+        if (bytes.empty()) return FDFD_OK;
+        void* p = buf_alloc(h, bytes.size());
+        if (!p) return fail(h, FDFD_ERR_OOM, "job table storage does not fit in device memory");
+        *base = p;
+        for (auto& q : parts) *q.first = (char*)p + q.second;
+        return stream_copy(h, p, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
     }
 };
 
@@ -158,10 +180,12 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 return fail(h, FDFD_ERR_OOM, "split GEMM operand storage does not fit in device memory");
             F->blocks_b_bytes = bb;
         }
-        HIPCHK(h, hipMalloc(&F->triw, sizeof(double) * P.tris.size() * (size_t)tri::work_doubles(n)));
-        HIPCHK(h, hipMalloc(&F->prow, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->iprow, sizeof(int) * (size_t)m * n));
-        HIPCHK(h, hipMalloc(&F->pivstat, sizeof(double) * 3 * m));
+        F->triw = (double*)buf_alloc(h, sizeof(double) * P.tris.size() * (size_t)tri::work_doubles(n));
+        F->prow = (int*)buf_alloc(h, sizeof(int) * (size_t)m * n);
+        F->iprow = (int*)buf_alloc(h, sizeof(int) * (size_t)m * n);
+        F->pivstat = (double*)buf_alloc(h, sizeof(double) * 3 * m);
+        if (!F->triw || !F->prow || !F->iprow || !F->pivstat)
+            return fail(h, FDFD_ERR_OOM, "pivot map storage does not fit in device memory");
     }
     // --- factorization job tables: per step, the jobs of factor 0, 1, ...
     std::vector<TriJob> tris;
@@ -296,16 +320,16 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             gemms.insert(gemms.end(), second.begin(), second.end());
         }
     }
-    Tables T;
-    auto upload = [&](auto** dst, const auto& v) -> int {
-        if (v.empty()) return FDFD_OK;
-        const size_t bytes = sizeof(v[0]) * v.size();
-        HIPCHK(h, hipMalloc((void**)dst, bytes));
-        return stream_copy(h, (void*)*dst, v.data(), bytes, hipMemcpyHostToDevice);
-    };
-    if ((rc = upload(&T.tri, tris)) || (rc = upload(&T.form, forms)) || (rc = upload(&T.lj, ljs)) ||
-        (rc = upload(&T.gemm, gemms)) || (rc = upload(&T.split, splits)))
-        return rc;
+    Tables T(h);
+    {
+        TablePack pk;
+        pk.add(&T.tri, tris);
+        pk.add(&T.form, forms);
+        pk.add(&T.lj, ljs);
+        pk.add(&T.gemm, gemms);
+        pk.add(&T.split, splits);
+        if ((rc = pk.commit(h, &T.base))) return rc;
+    }
 
     const bool fp64 = (prec == FDFD_FP64);
     const int NB = h->panel_width ? h->panel_width : PMAX;
@@ -580,9 +604,11 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
             J.si = F->si;
             trs.push_back(J);
         }
-        if ((rc = upload(&F->mv_tab, mvs)) || (rc = upload(&F->dg_tab, dgs)) ||
-            (rc = upload(&F->trs_tab, trs)))
-            return rc;
+        TablePack pk;
+        pk.add(&F->mv_tab, mvs);
+        pk.add(&F->dg_tab, dgs);
+        pk.add(&F->trs_tab, trs);
+        if ((rc = pk.commit(h, &F->tabs))) return rc;
         F->ssteps.clear();
         for (const Step& s : P.ssteps) {
             int mats = 0;

@@ -64,6 +64,28 @@ __device__ inline void split_f16(float x, _Float16& h, _Float16& l) {
     l = (_Float16)(x - (float)h);
 }
 
+// 1 / z for a complex pivot with one hardware reciprocal: z = (a + ib) 2^e
+// with max(|a|, |b|) in [0.5, 1) (exact power-of-two scaling, so no
+// overflow or underflow in |.|^2), 1 / z = (a - ib) r 2^-e with r = 1 / d,
+// d = a^2 + b^2 in [0.25, 2), r from v_rcp refined by Newton steps
+// r <- r + r (1 - d r) (one for FP32, two for FP64: the FP64 estimate has
+// about half the bits).  Relative error a few units in the last place, the
+// accuracy of the three-division (Smith) form it replaces on short chains.
+// z = 0 gives non-finite values; callers select them away.
+__device__ inline float rcp_hw(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ inline double rcp_hw(double d) { return __builtin_amdgcn_rcp(d); }
+template <typename C> __device__ inline C crcp_scaled(C z) {
+    using R = decltype(C{}.x);
+    int e;
+    (void)frexp(fmax(fabs(z.x), fabs(z.y)), &e);
+    const R a = ldexp(z.x, -e), b = ldexp(z.y, -e);
+    const R d = fma(a, a, b * b);
+    R r = rcp_hw(d);
+    r = fma(r, fma(-d, r, R(1)), r);
+    if constexpr (sizeof(R) == 8) r = fma(r, fma(-d, r, R(1)), r);
+    return mk<C>(ldexp(a * r, -e), ldexp(-b * r, -e));
+}
+
 // compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
 template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (B < E) {

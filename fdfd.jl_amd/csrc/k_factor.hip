@@ -70,6 +70,21 @@ __device__ inline unsigned long long mag_key(double m) {
 __device__ inline unsigned long long mag_key(float m) {
     return (unsigned long long)((unsigned)__float_as_int(m) & 0x7fffffffu);
 }
+// the same keys in the narrowest integer that holds them
+__device__ inline unsigned long long wave_max_key(unsigned long long k) { return wave_max_u64(k); }
+__device__ inline unsigned long long gepp_key(double m) { return mag_key(m); }
+__device__ inline unsigned gepp_key(float m) { return (unsigned)__float_as_int(m) & 0x7fffffffu; }
+__device__ inline unsigned wave_max_key(unsigned k) {
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0xB1, 0xF, 0xF, false));
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x4E, 0xF, 0xF, false));
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x124, 0xF, 0xF, false));
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x128, 0xF, 0xF, false));
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)k, 0);
+    const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)k, 16);
+    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)k, 32);
+    const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)k, 48);
+    return max(max(a, b), max(c, d));
+}
 
 // lane k of this lane's half-wave (lanes 0-31 or 32-63), k wave-uniform
 template <typename C> __device__ inline C bcast_half(C v, int k) {
@@ -88,10 +103,24 @@ template <typename C> __device__ inline C bcast_half(C v, int k) {
 // broadcast with readlane.  M is then
 // formed as the product U^-1 L^-1 through LDS (Lis, Uis: scratch).  Called by
 // all nthr threads of the block (a multiple of 64) with Ls, Us complete.
-// UNROLL unrolls the substitutions (lane indices and LDS columns become
-// constants, so the row reads are issued ahead of the chain; needs the
-// register room of at most two waves per SIMD).
-template <bool UNROLL, typename C>
+template <typename C>
+__device__ __attribute__((always_inline)) inline void inverse_product(C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1],
+                                                                      int t, int nthr, C* LUg) {
+    using R = decltype(C{}.x);
+    const C zero = mk<C>(R(0), R(0));
+    C* Mg = LUg;
+    C* Ug = LUg + PMAX * PMAX;
+    for (int e = t; e < PMAX * PMAX; e += nthr) {
+        const int r = e / PMAX, c = e % PMAX;
+        C sacc = zero;
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) sacc = cadd(sacc, cmul(Uis[r][k], Lis[k][c]));
+        gstore(Mg + e, sacc);
+        gstore(Ug + e, Uis[r][c]);
+    }
+}
+
+template <typename C>
 __device__ __attribute__((always_inline)) inline void store_pivot_inverses(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
                                             C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], int nb,
                                             int t, int nthr, C* LUg) {
@@ -104,44 +133,68 @@ __device__ __attribute__((always_inline)) inline void store_pivot_inverses(C (*L
         const int j = j0 + ((t >> 5) & 1);
         C x = csel(i == j, one, zero);  // L^-1 e_j
         C y = csel(i == j, one, zero);  // U^-1 e_j
-        // forward over k, backward over kb; rows and columns >= nb of Ls, Us
-        // are never used
-        auto fwd = [&](int k) {
+        // rows and columns >= nb of Ls, Us are never used
+        for (int k = 0; k < nb; ++k) {
             const C xk = bcast_half(x, k);
             x = csel(i > k && rok, csub(x, cmul(Ls[i][k], xk)), x);
-        };
-        auto bwd = [&](int kb) {
+            const int kb = nb - 1 - k;
             y = csel(i == kb, cmul(y, dinv), y);
             const C yk = bcast_half(y, kb);
             y = csel(i < kb, csub(y, cmul(Us[i][kb], yk)), y);
-        };
-        if constexpr (UNROLL) {
-            static_for<0, PMAX>([&](auto ik) {
-                constexpr int k = decltype(ik)::value;
-                if (k < nb) fwd(k);
-                if (PMAX - 1 - k < nb) bwd(PMAX - 1 - k);
-            });
-        } else {
-            for (int k = 0; k < nb; ++k) {
-                fwd(k);
-                bwd(nb - 1 - k);
-            }
         }
         const bool ok = rok && j < nb;
         Lis[i][j] = csel(ok, x, zero);
         Uis[i][j] = csel(ok, y, zero);
     }
     __syncthreads();
-    C* Mg = LUg;
-    C* Ug = LUg + PMAX * PMAX;
-    for (int e = t; e < PMAX * PMAX; e += nthr) {
-        const int r = e / PMAX, c = e % PMAX;
-        C sacc = zero;
+    inverse_product(Lis, Uis, t, nthr, LUg);
+}
+
+// The same for the window panel, NTHR threads: Ls and Us must be zero outside
+// their leading nb x nb block, so that every substitution step can run
+// unconditionally (steps past nb add exact zeros to the rows below nb).
+// Half-wave h solves for the columns h, h + NTHR / 32, ... of both inverses
+// at once (independent chains interleaved), fully unrolled so that lane
+// indices and LDS columns are constants; the entries for rows < nb are those
+// of the substitution restricted to nb.
+template <int NTHR, typename C>
+__device__ __attribute__((always_inline)) inline void store_pivot_inverses_w(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
+                                              C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], int nb,
+                                              int t, C* LUg) {
+    using R = decltype(C{}.x);
+    constexpr int NH = NTHR / 32, NJ = PMAX / NH;
+    static_assert(NJ * NH == PMAX, "columns per half-wave");
+    const C zero = mk<C>(R(0), R(0)), one = mk<C>(R(1), R(0));
+    const int i = t & 31, h = t >> 5;
+    const bool rok = i < nb;
+    const C dinv = csel(rok, crcp(Us[i][i]), zero);
+    C x[NJ], y[NJ];  // L^-1 e_j, U^-1 e_j for j = h + NH q
 #pragma unroll
-        for (int k = 0; k < PMAX; ++k) sacc = cadd(sacc, cmul(Uis[r][k], Lis[k][c]));
-        gstore(Mg + e, sacc);
-        gstore(Ug + e, Uis[r][c]);
+    for (int q = 0; q < NJ; ++q) {
+        x[q] = csel(i == h + NH * q, one, zero);
+        y[q] = x[q];
     }
+    static_for<0, PMAX>([&](auto ik) {
+        constexpr int k = decltype(ik)::value, kb = PMAX - 1 - k;
+        const C lik = Ls[i][k], uikb = Us[i][kb];
+#pragma unroll
+        for (int q = 0; q < NJ; ++q) {
+            const C xk = bcast_half(x[q], k);
+            x[q] = csel(i > k, csub(x[q], cmul(lik, xk)), x[q]);
+            y[q] = csel(i == kb, cmul(y[q], dinv), y[q]);
+            const C yk = bcast_half(y[q], kb);
+            y[q] = csel(i < kb, csub(y[q], cmul(uikb, yk)), y[q]);
+        }
+    });
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) {
+        const int j = h + NH * q;
+        const bool ok = rok && j < nb;
+        Lis[i][j] = csel(ok, x[q], zero);
+        Uis[i][j] = csel(ok, y[q], zero);
+    }
+    __syncthreads();
+    inverse_product(Lis, Uis, t, NTHR, LUg);
 }
 
 // lane k of this lane's half-wave (lanes 0-31 or 32-63)
@@ -181,6 +234,12 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     __shared__ int wrow[WIN];
     static_assert(NB == PMAX && WIN == 2 * PMAX, "window and panel shapes");
     static_assert(WIN * (NB + 1) >= 2 * PMAX * (PMAX + 1), "the inverse scratch reuses the window");
+    if (nb < PMAX) {  // the inverses need L and U zero outside nb x nb
+        for (int e = t; e < PMAX * (PMAX + 1); e += WP_THREADS) {
+            (&Ls[0][0])[e] = mk<C>(R(0), R(0));
+            (&Us[0][0])[e] = mk<C>(R(0), R(0));
+        }
+    }
     int row = -1;
     if (w0) {
         row = (lane < nwin) ? gload_s(J.win + lane) : -1;
@@ -215,19 +274,25 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     __syncthreads();
     {
         // GEPP on the window by all waves: lane l of every wave holds window
-        // row wrow[l], wave w its columns CPW w .. CPW w + CPW - 1.  At step
-        // cc the wave owning column cc finds the pivot (argmax of |.|_1 over
-        // the active rows; the lowest lane among exact ties, i.e. the
-        // smallest row index, GEPP's tie rule) and publishes the column and
-        // the pivot lane through LDS (double-buffered, one barrier per step);
-        // every wave then forms the multipliers and updates its own columns
-        // with the pivot row's entries (readlane within the wave).  The
-        // arithmetic per entry is that of a single-wave elimination.
+        // row wrow[l], wave w its columns CPW w .. CPW w + CPW - 1, and the
+        // steps run in blocks of CPW, block w by wave w.  The owner takes its
+        // block's steps back to back on its own columns: at step cc it finds
+        // the pivot (argmax of |.|_1 over the active rows; the lowest lane
+        // among exact ties, i.e. the smallest row index, GEPP's tie rule),
+        // forms the multipliers m = x / x[pivot] of the other active rows
+        // (zero for the pivot and inactive rows), publishes them with the
+        // pivot lane through LDS and updates its remaining columns.  After
+        // one barrier per block (LDS double-buffered by block) every other
+        // wave applies the block's steps to its own columns with the pivot
+        // row's entries (readlane within the wave).  The arithmetic per entry
+        // is that of a single-wave elimination, in the same order.  The wave
+        // index is made wave-uniform so that the column tests are scalar
+        // branches.
         constexpr int NW = WP_THREADS / 64, CPW = NB / NW;
         static_assert(CPW * NW == NB, "columns per wave");
-        __shared__ C colb[2][WIN];
-        __shared__ int plb[2];
-        const int wv = t >> 6;
+        __shared__ C colb[2][CPW][WIN];
+        __shared__ int plb[2][CPW];
+        const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
         if (!w0) {
             row = wrow[lane];
             act = row >= 0;
@@ -239,46 +304,68 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
             W[j] = mk<C>(R(0), R(0));
             if (act && c < nb) W[j] = Ws[lane][c];
         }
-        for (int cc = 0; cc < nb; ++cc) {
-            const int b = cc & 1;
-            if (wv == cc / CPW) {
-                const int jj = cc % CPW;
-                C x = W[0];
+        const int nblk = (nb + CPW - 1) / CPW;
+        for (int bk = 0; bk < nblk; ++bk) {
+            const int bb = bk & 1;
+            if (wv == bk) {
 #pragma unroll
-                for (int j = 1; j < CPW; ++j)
-                    if (j == jj) x = W[j];
-                const R mag = cabs1(x);
-                const unsigned long long key = act ? mag_key(mag) : 0ULL;
-                const unsigned long long kmax = wave_max_u64(key);
-                unsigned long long hit = __ballot(act && key == kmax);
-                if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
-                const int pl = (int)__ffsll((long long)hit) - 1;  // wave-uniform
-                colb[b][lane] = x;
-                if (lane == 0) plb[b] = pl;
-                if (lane == pl) {
-                    prs[cc] = row;
-                    pmags[cc] = act ? mag : R(-1);
+                for (int jj = 0; jj < CPW; ++jj) {
+                    const int cc = CPW * bk + jj;
+                    if (cc >= nb) break;  // uniform
+                    const C x = W[jj];
+                    const R mag = cabs1(x);
+                    const auto key = act ? gepp_key(mag) : decltype(gepp_key(mag))(0);
+                    const auto kmax = wave_max_key(key);
+                    unsigned long long hit = __ballot(act && key == kmax);
+                    if (hit == 0ULL) hit = __ballot(act);  // (cannot happen: some row is active)
+                    const int pl = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)hit) - 1);
+                    const C ucc = mk<C>(bcast(x.x, pl), bcast(x.y, pl));
+                    const bool live = act && lane != pl && cabs1(ucc) > R(0);
+                    const C m = csel(live, cmul(x, crcp_scaled(ucc)), mk<C>(R(0), R(0)));
+                    colb[bb][jj][lane] = m;
+                    if (lane == 0) plb[bb][jj] = pl;
+                    if (lane == pl) {
+                        prs[cc] = row;
+                        pmags[cc] = act ? mag : R(-1);
+                        act = false;
+                        mystep = cc;
+                    }
+                    // the pivot lane's registers are not modified in this step
+                    W[jj] = csel(act, m, W[jj]);
+#pragma unroll
+                    for (int j = jj + 1; j < CPW; ++j) {
+                        const C uc = mk<C>(bcast(W[j].x, pl), bcast(W[j].y, pl));
+                        W[j] = csel(act, csub(W[j], cmul(m, uc)), W[j]);
+                    }
                 }
             }
             __syncthreads();
-            const int pl = plb[b];
-            if (lane == pl) {
-                act = false;
-                mystep = cc;
-            }
-            const C ucc = colb[b][pl];
-            const R pivmag = cabs1(ucc);
-            const C m = csel(act && pivmag > R(0), cmul(colb[b][lane], crcp(ucc)), mk<C>(R(0), R(0)));
-            // pivot-row entries are broadcast one at a time (uniform control
-            // flow); the pivot lane's registers are not modified in this step
+            if (wv != bk) {
+                const int nst = min(CPW, nb - CPW * bk);
+                int pls[CPW];
+                C ms[CPW];
 #pragma unroll
-            for (int j = 0; j < CPW; ++j) {
-                const int c = CPW * wv + j;
-                if (c == cc) {
-                    if (act) W[j] = m;
-                } else if (c > cc) {
-                    const C uc = mk<C>(bcast(W[j].x, pl), bcast(W[j].y, pl));
-                    if (act) W[j] = csub(W[j], cmul(m, uc));
+                for (int jj = 0; jj < CPW; ++jj) {
+                    pls[jj] = plb[bb][jj];
+                    ms[jj] = colb[bb][jj][lane];
+                }
+#pragma unroll
+                for (int jj = 0; jj < CPW; ++jj) {
+                    if (jj >= nst) break;  // uniform
+                    const int pl = __builtin_amdgcn_readfirstlane(pls[jj]);
+                    if (lane == pl) {
+                        act = false;
+                        mystep = CPW * bk + jj;
+                    }
+                    // columns right of the block are brought up to date;
+                    // those left of it hold final multipliers
+                    if (wv > bk) {
+#pragma unroll
+                        for (int j = 0; j < CPW; ++j) {
+                            const C uc = mk<C>(bcast(W[j].x, pl), bcast(W[j].y, pl));
+                            W[j] = csel(act, csub(W[j], cmul(ms[jj], uc)), W[j]);
+                        }
+                    }
                 }
             }
         }
@@ -299,7 +386,7 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     __syncthreads();
     // the window tile is dead: its LDS holds the inverse scratch
     C(*Lis)[PMAX + 1] = (C(*)[PMAX + 1]) & Ws[0][0];
-    store_pivot_inverses<true, C>(Ls, Us, Lis, Lis + PMAX, nb, t, WP_THREADS, (C*)J.LU);
+    store_pivot_inverses_w<WP_THREADS, C>(Ls, Us, Lis, Lis + PMAX, nb, t, (C*)J.LU);
     if (!w0) return;
     if (lane < nb) {
         const int p = prs[lane];
@@ -421,7 +508,7 @@ __global__ __launch_bounds__(1024) void k_panel(const LineJob* jobs, int k0) {
         buf ^= 1;
     }
     __syncthreads();
-    store_pivot_inverses<false, C>(Ls, Us, Lis, Lis + PMAX, NB, t, 1024, (C*)J.LU);
+    store_pivot_inverses<C>(Ls, Us, Lis, Lis + PMAX, NB, t, 1024, (C*)J.LU);
     if (t < NB) {
         const int p = prs[t];
         J.piv[t] = p;

@@ -171,19 +171,65 @@ void arena_clear(Handle* h) {
     h->arenas.clear();
 }
 
+void* buf_alloc(Handle* h, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    auto it = h->bpool.find(bytes);
+    void* p = nullptr;
+    if (it != h->bpool.end()) {
+        p = it->second;
+        h->bpool.erase(it);
+        h->bpool_bytes -= bytes;
+    } else if (hipMalloc(&p, bytes) != hipSuccess) {  // retry once without the pools
+        (void)hipGetLastError();
+        buf_clear(h);
+        arena_clear(h);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    h->bpool_live[p] = bytes;
+    return p;
+}
+
+void buf_free(Handle* h, void* p) {
+    if (!p) return;
+    auto it = h->bpool_live.find(p);
+    if (it == h->bpool_live.end()) {
+        // another handle's buffer (a factor a batch's builder handle made)
+        (void)hipFree(p);
+        return;
+    }
+    const size_t bytes = it->second;
+    h->bpool_live.erase(it);
+    if (h->bpool_bytes + bytes > BUF_POOL_MAX) {
+        (void)hipFree(p);
+        return;
+    }
+    h->bpool.emplace(bytes, p);
+    h->bpool_bytes += bytes;
+}
+
+void buf_clear(Handle* h) {
+    for (auto& kv : h->bpool) (void)hipFree(kv.second);
+    h->bpool.clear();
+    h->bpool_bytes = 0;
+}
+
 void rt_release_factor_storage(Handle* h, Factor* F) {
     if (!F) return;
     (void)hipStreamSynchronize(h->stream);  // the arena may be reused by the next factor
-    dfree(F->coef); dfree(F->sxf); dfree(F->sxb); dfree(F->syf); dfree(F->syb);
+    for (void* b : {(void*)F->coef, (void*)F->sxf, (void*)F->sxb, (void*)F->syf, (void*)F->syb,
+                    (void*)F->prow, (void*)F->iprow, (void*)F->pivstat, F->tabs, (void*)F->triw})
+        buf_free(h, b);
     arena_give(h, F->blocks, F->blocks_bytes);
     arena_give(h, F->blocks_b, F->blocks_b_bytes);
-    dfree(F->prow); dfree(F->iprow); dfree(F->pivstat);
-    dfree(F->mv_tab); dfree(F->dg_tab); dfree(F->trs_tab); dfree(F->triw);
     if (F->solve_exec) (void)hipGraphExecDestroy(F->solve_exec);
     F->coef = F->sxf = F->sxb = F->syf = F->syb = nullptr;
     F->blocks = F->blocks_b = nullptr;
     F->prow = F->iprow = nullptr;
     F->pivstat = nullptr;
+    F->tabs = nullptr;
     F->mv_tab = nullptr;
     F->dg_tab = nullptr;
     F->trs_tab = nullptr;
@@ -205,11 +251,13 @@ int rt_assemble(Handle* h, Factor* F) {
     const long N = (long)g.Nx * g.Ny;
     const double eps0n = EPS0 * g.L0, mu0n = MU0 * g.L0;
     const double eta0 = std::sqrt(MU0 / EPS0);
-    HIPCHK(h, hipMalloc(&F->coef, sizeof(double2) * 5 * N));
-    HIPCHK(h, hipMalloc(&F->sxf, sizeof(double2) * g.Nx));
-    HIPCHK(h, hipMalloc(&F->sxb, sizeof(double2) * g.Nx));
-    HIPCHK(h, hipMalloc(&F->syf, sizeof(double2) * g.Ny));
-    HIPCHK(h, hipMalloc(&F->syb, sizeof(double2) * g.Ny));
+    F->coef = (double2*)buf_alloc(h, sizeof(double2) * 5 * N);
+    F->sxf = (double2*)buf_alloc(h, sizeof(double2) * g.Nx);
+    F->sxb = (double2*)buf_alloc(h, sizeof(double2) * g.Nx);
+    F->syf = (double2*)buf_alloc(h, sizeof(double2) * g.Ny);
+    F->syb = (double2*)buf_alloc(h, sizeof(double2) * g.Ny);
+    if (!F->coef || !F->sxf || !F->sxb || !F->syf || !F->syb)
+        return fail(h, FDFD_ERR_OOM, "operator coefficient storage does not fit in device memory");
     if (F->form == 1 && F->pol != FDFD_TM)
         return fail(h, FDFD_ERR_INVALID, "the modulated (sideband) operator is TM only");
     if (F->form == 2 || F->form == 3) {
@@ -494,6 +542,7 @@ int fdfd_destroy(fdfd_handle* hh) {
     (void)hipStreamSynchronize(h->stream);
     for (auto& kv : h->factors) free_factor(h, kv.second);
     arena_clear(h);
+    buf_clear(h);
     free_multi_work(h);
     void* bufs[] = {h->eps, h->used, h->piv, h->win, h->verr, h->Vbuf, h->LU, h->Zb,
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
@@ -522,7 +571,10 @@ int fdfd_set_grid(fdfd_handle* hh, int Nx, int Ny, double dx, double dy, int npm
     // and reallocating gigabytes makes the driver clear the memory again,
     // which slows the GPU work that follows.
     const bool same_shape = h->has_grid && h->g.Nx == Nx && h->g.Ny == Ny;
-    if (!same_shape) arena_clear(h);
+    if (!same_shape) {
+        arena_clear(h);
+        buf_clear(h);
+    }
     h->g = GridDesc{Nx, Ny, dx, dy, npml_x, npml_y, L0};
     h->has_grid = true;
     h->has_eps = false;

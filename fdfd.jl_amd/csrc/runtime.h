@@ -63,6 +63,7 @@ struct Factor {
     // blocks the step streams
     struct SolveStep { int kind, first, count, mats; };
     std::vector<SolveStep> ssteps;
+    void* tabs = nullptr;  // one buffer holding the three tables below
     MvJob* mv_tab = nullptr;
     DiagJob* dg_tab = nullptr;
     TrsJob* trs_tab = nullptr;
@@ -133,6 +134,13 @@ struct Handle {
     // slows the GPU work that follows for seconds); freed by fdfd_destroy
     char* bslab = nullptr;
     size_t bslab_size = 0;
+    // per-factor buffers (coefficient planes, stretch vectors, pivot maps,
+    // line-solve work, job tables) kept for reuse by exact size: every
+    // factorization of a grid allocates the same set, and hipMalloc/hipFree
+    // pairs cost about a millisecond per factorization
+    std::multimap<size_t, void*> bpool;      // free buffers by size
+    std::map<void*, size_t> bpool_live;      // handed-out buffers and their sizes
+    size_t bpool_bytes = 0;                  // bytes held free in bpool
     // device staging of the host-pointer solve (sources, then fields), kept
     // between calls and grown as needed
     void* io = nullptr;
@@ -200,6 +208,14 @@ size_t arena_pooled_bytes(const Handle* h);
 void* arena_take(Handle* h, size_t bytes);
 void arena_give(Handle* h, void* p, size_t bytes);
 void arena_clear(Handle* h);
+// pooled per-factor buffers: buf_alloc returns nullptr when device memory is
+// short; buf_free takes nullptr and pointers from buf_alloc (of this handle,
+// or of another, whose buffer is then freed); buf_clear frees the free list
+// (BUF_POOL_MAX bytes at most are kept)
+constexpr size_t BUF_POOL_MAX = (size_t)1 << 30;
+void* buf_alloc(Handle* h, size_t bytes);
+void buf_free(Handle* h, void* p);
+void buf_clear(Handle* h);
 void rt_free_factor(Handle* h, Factor* F);
 void rt_release_factor_storage(Handle* h, Factor* F);  // device storage only, F stays
 // FGMRES for nr <= KMULTI sources on one factor in lockstep (multi.cpp):
