@@ -150,48 +150,88 @@ __device__ __attribute__((always_inline)) inline void store_pivot_inverses(C (*L
     inverse_product(Lis, Uis, t, nthr, LUg);
 }
 
-// The same for the window panel, NTHR threads: Ls and Us must be zero outside
-// their leading nb x nb block, so that every substitution step can run
+// The same for the window panel, NTHR threads, with Ls holding the strictly
+// lower part of L (zero diagonal) and both Ls and Us zero outside their
+// leading nb x nb block, so that every substitution step runs
 // unconditionally (steps past nb add exact zeros to the rows below nb).
-// Half-wave h solves for the columns h, h + NTHR / 32, ... of both inverses
-// at once (independent chains interleaved), fully unrolled so that lane
-// indices and LDS columns are constants; the entries for rows < nb are those
-// of the substitution restricted to nb.
+// U = D (I + N) with N = D^-1 (U - D) strictly upper: the rows of U are
+// scaled by 1 / U[i][i] in place first (Dv keeps the 1 / U[i][i]), so both
+// substitutions are unit triangular, x <- x - N[:, k] x_k, and U^-1 e_j is
+// (I + N)^-1 e_j times 1 / U[j][j].  Each wave carries whole complex
+// columns: lane i holds the real part of row i, lane 32 + i its imaginary
+// part, so a step is two readlanes (x_k) and two FMAs with the per-lane
+// coefficients (re lanes: Re N, -Im N; im lanes: Im N, Re N) for each of
+// its PMAX / NW columns of L^-1 and of U^-1 (j = w, w + NW, ..), all
+// interleaved and fully unrolled (lane indices and LDS columns constant).
 template <int NTHR, typename C>
 __device__ __attribute__((always_inline)) inline void store_pivot_inverses_w(C (*Ls)[PMAX + 1], C (*Us)[PMAX + 1],
-                                              C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], int nb,
-                                              int t, C* LUg) {
+                                              C (*Lis)[PMAX + 1], C (*Uis)[PMAX + 1], decltype(C{}.x)* Dv,
+                                              int nb, int t, C* LUg) {
     using R = decltype(C{}.x);
-    constexpr int NH = NTHR / 32, NJ = PMAX / NH;
-    static_assert(NJ * NH == PMAX, "columns per half-wave");
-    const C zero = mk<C>(R(0), R(0)), one = mk<C>(R(1), R(0));
-    const int i = t & 31, h = t >> 5;
-    const bool rok = i < nb;
-    const C dinv = csel(rok, crcp(Us[i][i]), zero);
-    C x[NJ], y[NJ];  // L^-1 e_j, U^-1 e_j for j = h + NH q
+    constexpr int NW = NTHR / 64, NJ = PMAX / NW;
+    static_assert(NJ * NW == PMAX, "columns per wave");
+    const C zero = mk<C>(R(0), R(0));
+    // U rows scaled by their inverse diagonal (values read before the barrier)
+    {
+        constexpr int EPT = PMAX * PMAX / NTHR;
+        C nv[EPT];
+        R dre = R(0), dim = R(0);
+#pragma unroll
+        for (int e_ = 0; e_ < EPT; ++e_) {
+            const int e = t + NTHR * e_, r = e / PMAX, c = e % PMAX;
+            const C d = (r < nb) ? crcp_scaled(Us[r][r]) : zero;
+            nv[e_] = csel(c > r, cmul(d, Us[r][c]), zero);
+            if (c == r) {
+                dre = d.x;
+                dim = d.y;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e_ = 0; e_ < EPT; ++e_) {
+            const int e = t + NTHR * e_, r = e / PMAX, c = e % PMAX;
+            Us[r][c] = nv[e_];
+            if (c == r) {
+                Dv[2 * r] = dre;
+                Dv[2 * r + 1] = dim;
+            }
+        }
+        __syncthreads();
+    }
+    const int lane = t & 63, i = lane & 31;
+    const bool him = lane >= 32;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    R x[NJ], y[NJ];  // this lane's part of L^-1 e_j, (I + N)^-1 e_j, j = w + NW q
 #pragma unroll
     for (int q = 0; q < NJ; ++q) {
-        x[q] = csel(i == h + NH * q, one, zero);
+        x[q] = (!him && i == w + NW * q) ? R(1) : R(0);
         y[q] = x[q];
     }
     static_for<0, PMAX>([&](auto ik) {
         constexpr int k = decltype(ik)::value, kb = PMAX - 1 - k;
-        const C lik = Ls[i][k], uikb = Us[i][kb];
+        const C lv = Ls[i][k], uv = Us[i][kb];
+        const R lp = him ? lv.y : lv.x, lq = him ? lv.x : -lv.y;
+        const R up = him ? uv.y : uv.x, uq = him ? uv.x : -uv.y;
 #pragma unroll
         for (int q = 0; q < NJ; ++q) {
-            const C xk = bcast_half(x[q], k);
-            x[q] = csel(i > k, csub(x[q], cmul(lik, xk)), x[q]);
-            y[q] = csel(i == kb, cmul(y[q], dinv), y[q]);
-            const C yk = bcast_half(y[q], kb);
-            y[q] = csel(i < kb, csub(y[q], cmul(uikb, yk)), y[q]);
+            const R xr = bcast(x[q], k), xi = bcast(x[q], 32 + k);
+            x[q] = fma(-lp, xr, fma(-lq, xi, x[q]));
+            const R yr = bcast(y[q], kb), yi = bcast(y[q], 32 + kb);
+            y[q] = fma(-up, yr, fma(-uq, yi, y[q]));
         }
     });
+    R* Lr = (R*)&Lis[0][0];
+    R* Ur = (R*)&Uis[0][0];
 #pragma unroll
     for (int q = 0; q < NJ; ++q) {
-        const int j = h + NH * q;
-        const bool ok = rok && j < nb;
-        Lis[i][j] = csel(ok, x[q], zero);
-        Uis[i][j] = csel(ok, y[q], zero);
+        const int j = w + NW * q;
+        const bool ok = i < nb && j < nb;
+        // U^-1 e_j = y / U[j][j]: the partner lane holds the other part
+        const R yo = __shfl_xor(y[q], 32, 64);
+        const R dr = Dv[2 * j], di = Dv[2 * j + 1];
+        const R u = him ? fma(y[q], dr, yo * di) : fma(y[q], dr, -(yo * di));
+        Lr[2 * (i * (PMAX + 1) + j) + (him ? 1 : 0)] = ok ? x[q] : R(0);
+        Ur[2 * (i * (PMAX + 1) + j) + (him ? 1 : 0)] = ok ? u : R(0);
     }
     __syncthreads();
     inverse_product(Lis, Uis, t, NTHR, LUg);
@@ -377,7 +417,7 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
                 const int c = CPW * wv + j;
                 if (c < nb) {
                     const bool lo = c < mystep;
-                    Ls[mystep][c] = csel(lo, W[j], mk<C>(c == mystep ? R(1) : R(0), R(0)));
+                    Ls[mystep][c] = csel(lo, W[j], mk<C>(R(0), R(0)));  // strictly lower
                     Us[mystep][c] = csel(lo, mk<C>(R(0), R(0)), W[j]);
                 }
             }
@@ -386,7 +426,8 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     __syncthreads();
     // the window tile is dead: its LDS holds the inverse scratch
     C(*Lis)[PMAX + 1] = (C(*)[PMAX + 1]) & Ws[0][0];
-    store_pivot_inverses_w<WP_THREADS, C>(Ls, Us, Lis, Lis + PMAX, nb, t, (C*)J.LU);
+    __shared__ R dvs[2 * PMAX];  // 1 / U[i][i]
+    store_pivot_inverses_w<WP_THREADS, C>(Ls, Us, Lis, Lis + PMAX, dvs, nb, t, (C*)J.LU);
     if (!w0) return;
     if (lane < nb) {
         const int p = prs[lane];
