@@ -390,8 +390,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                 launch_gjreset(st, J, nj, n);
                 // Panels of NB columns.  With the windowed search, two panels are
                 // factored before one trailing pass applies both (the second panel's
-                // columns and pivot rows are brought up to date by the thin update
-                // and the Z build); the full search runs single panels.
+                // columns, as the first panel leaves them, are formed by the window
+                // panel for its window and by the Z build's check blocks for every
+                // row, and its pivot rows by the Z build); the full search runs
+                // single panels.
                 for (int k0 = 0; k0 < n;) {
                     int nb1, nb2 = 0;
                     const int fresh = std::min(n, WIN + k0);
@@ -421,11 +423,10 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
                         const int k2 = k0 + nb1;
                         nb2 = std::min(NB, n - k2);
                         const int fresh2 = std::min(n, WIN + k2);
-                        ProfScope* ps = scope(P_ZBUILD, 0, 0);
-                        if (fp64) launch_thin<double>(st, J, nj, n, k0, nb1, nb2);
-                        else launch_thin<float>(st, J, nj, n, k0, nb1, nb2);
-                        delete ps;
-                        ps = scope(P_PANEL, 0, 0);
+                        // (the second panel's values as the first panel leaves
+                        // them are formed by the window panel, for its window,
+                        // and by the Z build's check blocks, for every row)
+                        ProfScope* ps = scope(P_PANEL, 0, 0);
                         if (fp64) launch_wpanel<double>(st, J, nj, n, k2, nb2, fresh2, 1);
                         else launch_wpanel<float>(st, J, nj, n, k2, nb2, fresh2, 1);
                         delete ps;

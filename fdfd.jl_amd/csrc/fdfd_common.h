@@ -336,8 +336,6 @@ template <typename R>
 void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int verify, int second, int k1, int nb1);
 template <typename R>
-void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1, int nb1, int nb2);
-template <typename R>
 void launch_trailing(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb1,
                      int nb2);
 // mixed path, n % 4 == 0: the trailing update on fp16x3 products

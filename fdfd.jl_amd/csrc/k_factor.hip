@@ -244,6 +244,26 @@ template <typename C> __device__ inline C shfl_half(C v, int k) {
 }
 
 // ---------------------------------------------------------------------------
+// Value of the second panel of a fused pair (first panel K1 = [k1, k1 + PMAX),
+// second K2) at a row r that is no pivot of the first panel, column c of K2:
+//   V2[r][c] = A[r][c] - sum_k V1[r][k] Z1[k][c],
+// the V1 row (PMAX entries) in LDS, this column of Z1[:, K2] in registers
+// half a column at a time: pair_second_step subtracts the terms k in
+// [kh, kh + PMAX / 2) with zh = Z1[kh.., c], called for kh = 0 then PMAX / 2.
+// The window panel and the check blocks of the Z build use this one
+// routine, so both see the same values.
+template <typename C>
+__device__ inline C pair_second_step(C w, const C* v1row, const C (&zh)[PMAX / 2], int kh) {
+#pragma unroll
+    for (int k = 0; k < PMAX / 2; k += 2) {
+        const C va = v1row[kh + k], vb = v1row[kh + k + 1];
+        w = csub(w, cmul(va, zh[k]));
+        w = csub(w, cmul(vb, zh[k + 1]));
+    }
+    return w;
+}
+
+// ---------------------------------------------------------------------------
 // Fast panel: GEPP on the candidate window, the first WIN unpivoted rows by
 // index, by wave 0 (lane l owns window row win[l]); the whole workgroup
 // (WP_THREADS) loads the window and forms the inverse factors.  Every chosen
@@ -288,22 +308,53 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
     bool act = row >= 0;
     int mystep = -1;  // panel step at which this lane's row became the pivot
     // panel values: the block's columns, or (second panel of a pair) the
-    // columns brought up to date by the first panel, in the panel buffer
-    const C* src = second ? (const C*)J.Vbuf + PMAX : (const C*)J.Y + k0;
-    const long ld = second ? VMAX : n;
+    // columns as the first panel leaves them: the window's rows are no
+    // pivots of the first panel, so V2 = A[:, K2] - V1 Z1[:, K2]
     const int poff = second ? PMAX : 0;
     __syncthreads();
     {
         // 32 lanes per row, WIN / 16 rows per thread; all loads in flight
         // before the first LDS store (lanes past the window or the panel
-        // read src[0] and store zero)
+        // read entry 0 and store zero)
         constexpr int RPT = WIN * 32 / WP_THREADS;
         C v[RPT];
         const int c = t & 31, r0 = t >> 5;
+        const C* Y = (const C*)J.Y;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int rw = wrow[r0 + 16 * q];
-            v[q] = gload(src + ((rw >= 0 && c < nb) ? (long)rw * ld + c : 0));
+            v[q] = gload(Y + ((rw >= 0 && c < nb) ? (long)rw * n + k0 + c : 0));
+        }
+        if (second) {  // uniform
+            const C* Vb = (const C*)J.Vbuf;
+            const C* Zg = (const C*)J.Z;
+            C v1[RPT], zh[PMAX / 2];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int rw = wrow[r0 + 16 * q];
+                v1[q] = gload(Vb + (rw >= 0 ? (long)rw * VMAX + c : 0));
+            }
+#pragma unroll
+            for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)k * n + k0 + c : 0));
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int rw = wrow[r0 + 16 * q];
+                Ws[r0 + 16 * q][c] = csel(rw >= 0, v1[q], mk<C>(R(0), R(0)));  // V1 rows first
+                v[q] = csel(c < nb, v[q], mk<C>(R(0), R(0)));
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int kh = 0; kh < PMAX; kh += PMAX / 2) {
+                if (kh > 0) {
+#pragma unroll
+                    for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)(kh + k) * n + k0 + c : 0));
+                }
+#pragma unroll
+                for (int k = 0; k < PMAX / 2; ++k) zh[k] = csel(c < nb, zh[k], mk<C>(R(0), R(0)));
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) v[q] = pair_second_step(v[q], &Ws[r0 + 16 * q][0], zh, kh);
+            }
+            __syncthreads();
         }
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
@@ -599,8 +650,10 @@ template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, in
 // Second panel of a fused pair (columns K2 after K1 = [k1, k1 + nb1)): its
 // pivot rows are taken after the first panel's update,
 //   T[k][col] = [col not in K1] A[p2_k][col] - sum_q V1[p2_k][q] Z1[q][col],
-// Z goes to rows PMAX.. of the Z buffer, and the panel values (already in the
-// panel buffer, columns PMAX..) are only checked.
+// Z goes to rows PMAX.. of the Z buffer, and the check blocks form the
+// panel's values for every row as the first panel leaves them (the first
+// panel's pivots take their Z1 row, every other row A - V1 Z1[:, K2]), write
+// them to columns PMAX.. of the panel buffer and check them.
 // Operand layout of v_mfma_*_16x16x4: A[i = lane%16][k = lane/16],
 // B[k = lane/16][j = lane%16], result D[M::row(lane, v)][lane%16].  In Z = M T
 // the contraction index of step (rb, s) for lane group g is the row T holds
@@ -619,7 +672,11 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
     __shared__ R Fr[PMAX][PMAX + 1], Fi[PMAX][PMAX + 1];  // M (Z blocks) or U^-1 (check blocks)
-    __shared__ R Vr[PMAX][PMAX + 1], Vi[PMAX][PMAX + 1];  // V1[piv2, :] (Z) or the V tile (check)
+    // V1[piv2, :] (Z) or the V tile (check); in the check blocks of a second
+    // panel first the rows' V1 as one complex tile
+    __shared__ __align__(16) R Vri[2][PMAX][PMAX + 1];
+    R(*Vr)[PMAX + 1] = Vri[0];
+    R(*Vi)[PMAX + 1] = Vri[1];
     __shared__ int pv[PMAX];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int lr = lane & 15, lg = lane >> 4;
@@ -632,13 +689,30 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     const int vc = t & 31, h8 = t >> 5;
     const int rbase = ((int)blockIdx.x - nzb) * 32;
     C vrow[4];
+    // second panel of a pair: the check blocks form the panel's values as
+    // the first panel leaves them (its pivots take their Z1 row, every other
+    // row A - V1 Z1[:, K2]) and write them to the panel buffer
+    C v1r[4], zh[PMAX / 2];
+    int slot[4];
     if (!zpart) {
-        const C* vsrc = second ? (const C*)J.Vbuf + poff : (const C*)J.Y + k0;
+        const C* Y = (const C*)J.Y;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = rbase + h8 + 8 * i;
             const bool ok = r < n && vc < nb;
-            vrow[i] = gload(vsrc + (ok ? (second ? (long)r * VMAX : (long)r * n) + vc : 0));
+            vrow[i] = gload(Y + (ok ? (long)r * n + k0 + vc : 0));
+        }
+        if (second) {  // uniform
+            const C* Vb = (const C*)J.Vbuf;
+            const C* Zg = (const C*)J.Z;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = rbase + h8 + 8 * i;
+                slot[i] = gload_s(J.iprow + (r < n ? r : 0)) - k1;
+                v1r[i] = gload(Vb + (r < n ? (long)r * VMAX + vc : 0));
+            }
+#pragma unroll
+            for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (vc < nb ? (long)k * n + k0 + vc : 0));
         }
     }
     {
@@ -761,12 +835,44 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
             }
     } else {
         const int c = vc;
+        if (second) {  // uniform
+            const C* Zg = (const C*)J.Z;
+            C(*V1s)[PMAX + 1] = (C(*)[PMAX + 1]) & Vri[0][0][0];
+            C zs[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool piv1 = slot[i] >= 0 && slot[i] < nb1;
+                zs[i] = gload(Zg + (piv1 && c < nb ? (long)slot[i] * n + k0 + c : 0));
+                V1s[h8 + 8 * i][c] = (c < nb1) ? v1r[i] : mk<C>(R(0), R(0));
+            }
+            C w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w[i] = csel(c < nb, vrow[i], mk<C>(R(0), R(0)));
+            __syncthreads();
+#pragma unroll 1
+            for (int kh = 0; kh < PMAX; kh += PMAX / 2) {
+                if (kh > 0) {
+#pragma unroll
+                    for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)(kh + k) * n + k0 + c : 0));
+                }
+#pragma unroll
+                for (int k = 0; k < PMAX / 2; ++k) zh[k] = csel(c < nb, zh[k], mk<C>(R(0), R(0)));
+#pragma unroll
+                for (int i = 0; i < 4; ++i) w[i] = pair_second_step(w[i], &V1s[h8 + 8 * i][0], zh, kh);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool piv1 = slot[i] >= 0 && slot[i] < nb1;
+                vrow[i] = piv1 ? zs[i] : w[i];
+            }
+            __syncthreads();  // the V1 tile is read: its LDS takes the V tile
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = rbase + h8 + 8 * i;
             const bool ok = r < n && c < nb;
             const C v = csel(ok, vrow[i], mk<C>(R(0), R(0)));
-            if (!second && ok) gstore((C*)J.Vbuf + (long)r * VMAX + c, v);
+            if (ok) gstore((C*)J.Vbuf + (long)r * VMAX + poff + c, v);
             Vr[h8 + 8 * i][c] = v.x;
             Vi[h8 + 8 * i][c] = v.y;
         }
@@ -813,96 +919,6 @@ template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, 
                                     int, int, int);
 template void launch_zbuild<float>(hipStream_t, const LineJob*, int, int, int, int, int, int,
                                    int, int, int);
-
-// ---------------------------------------------------------------------------
-// Thin update of a fused pair: the second panel's columns K2 = [k1 + nb1,
-// k1 + nb1 + nb2) after the first panel, 32 lanes per row (lane c = column),
-// 32 rows per workgroup:
-//   V2[r] = Z1[slot, K2]                        for the first panel's pivots,
-//   V2[r] = A[r, K2] - V1[r] Z1[:, K2]           for every other row,
-// into columns PMAX.. of the panel buffer.
-// ---------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_thin(const LineJob* jobs, int k1, int nb1, int nb2) {
-    using C = cplx<R>;
-    const LineJob J = jobs[blockIdx.y];
-    const int n = J.n;
-    __shared__ C Z1k[PMAX][PMAX + 1];        // Z1[q][K2 column c]
-    __shared__ __align__(16) C V1s[32][PMAX + 2];  // V1 of the workgroup's 32 rows
-    const C* Zg = (const C*)J.Z;
-    const int k2 = k1 + nb1;
-    constexpr int EPT = PMAX * PMAX / 256;  // blockDim.x == 256
-    const int c = threadIdx.x & 31, half = threadIdx.x >> 5;
-    const int rbase = blockIdx.x * 32;
-    // the four rows of this half-wave (rows half + 8 i): slots and row loads
-    // are issued first, so their latency overlaps the Z1 staging below;
-    // out-of-range lanes read entry 0
-    const C* Y = (const C*)J.Y;
-    const C* Vb = (const C*)J.Vbuf;
-    int slot[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = rbase + half + 8 * i;
-        slot[i] = gload_s(J.iprow + (r < n ? r : 0)) - k1;
-    }
-    C wv[4], v1v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = rbase + half + 8 * i;
-        wv[i] = gload(Y + (r < n && c < nb2 ? (long)r * n + k2 + c : 0));
-        v1v[i] = gload(Vb + (r < n && c < nb1 ? (long)r * VMAX + c : 0));
-    }
-    {
-        C v[EPT];
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, r = e / PMAX, cc = e % PMAX;
-            v[q] = gload(Zg + ((r < nb1 && cc < nb2) ? (long)r * n + k2 + cc : 0));
-        }
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = threadIdx.x + 256 * q, r = e / PMAX, cc = e % PMAX;
-            Z1k[r][cc] = csel(r < nb1 && cc < nb2, v[q], mk<C>(R(0), R(0)));
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        V1s[half + 8 * i][c] = csel(c < nb1, v1v[i], mk<C>(R(0), R(0)));
-    __syncthreads();
-    // this lane's column of Z1 in registers, reused for its four rows; the
-    // rows of V1 are read from LDS two entries at a time (one address per
-    // half-wave: a broadcast), so a step costs half an LDS read
-    C z[PMAX];
-#pragma unroll
-    for (int q = 0; q < PMAX; ++q) z[q] = Z1k[q][c];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = rbase + half + 8 * i;  // the same for the 32 lanes of a half-wave
-        if (r >= n) break;
-        C w;
-        if (slot[i] >= 0 && slot[i] < nb1) {
-            w = Z1k[slot[i]][c];  // (a dynamic index into z would put z on the stack)
-        } else {
-            w = (c < nb2) ? wv[i] : mk<C>(R(0), R(0));
-            const C* vr = &V1s[half + 8 * i][0];
-#pragma unroll
-            for (int q = 0; q < PMAX; q += 2) {
-                const C va = vr[q], vb = vr[q + 1];  // zero past nb1
-                w = csub(w, cmul(va, z[q]));
-                w = csub(w, cmul(vb, z[q + 1]));
-            }
-        }
-        if (c < nb2) gstore((C*)J.Vbuf + (long)r * VMAX + PMAX + c, w);
-    }
-}
-
-template <typename R>
-void launch_thin(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k1, int nb1,
-                 int nb2) {
-    k_thin<R><<<dim3((n + 31) / 32, njobs), 256, 0, s>>>(jobs_d, k1, nb1, nb2);
-}
-template void launch_thin<double>(hipStream_t, const LineJob*, int, int, int, int, int);
-template void launch_thin<float>(hipStream_t, const LineJob*, int, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Trailing update on MFMA.  64x64 output tile per block, 4 waves of 32x32,
