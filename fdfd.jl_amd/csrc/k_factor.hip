@@ -247,18 +247,17 @@ template <typename C> __device__ inline C shfl_half(C v, int k) {
 // Value of the second panel of a fused pair (first panel K1 = [k1, k1 + PMAX),
 // second K2) at a row r that is no pivot of the first panel, column c of K2:
 //   V2[r][c] = A[r][c] - sum_k V1[r][k] Z1[k][c],
-// the V1 row (PMAX entries) in LDS, this column of Z1[:, K2] in registers
-// half a column at a time: pair_second_step subtracts the terms k in
-// [kh, kh + PMAX / 2) with zh = Z1[kh.., c], called for kh = 0 then PMAX / 2.
-// The window panel and the check blocks of the Z build use this one
-// routine, so both see the same values.
+// the V1 row (PMAX entries) and the tile Z1[:, K2] (zt[k][c], zero past the
+// panel) in LDS.  The window panel and the check blocks of the Z build use
+// this one routine, so both see the same values.
 template <typename C>
-__device__ inline C pair_second_step(C w, const C* v1row, const C (&zh)[PMAX / 2], int kh) {
+__device__ inline C pair_second_value(C a, const C* v1row, const C (*zt)[PMAX + 1], int c) {
+    C w = a;
 #pragma unroll
-    for (int k = 0; k < PMAX / 2; k += 2) {
-        const C va = v1row[kh + k], vb = v1row[kh + k + 1];
-        w = csub(w, cmul(va, zh[k]));
-        w = csub(w, cmul(vb, zh[k + 1]));
+    for (int k = 0; k < PMAX; k += 2) {
+        const C va = v1row[k], vb = v1row[k + 1];
+        w = csub(w, cmul(va, zt[k][c]));
+        w = csub(w, cmul(vb, zt[k + 1][c]));
     }
     return w;
 }
@@ -328,32 +327,34 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
         if (second) {  // uniform
             const C* Vb = (const C*)J.Vbuf;
             const C* Zg = (const C*)J.Z;
-            C v1[RPT], zh[PMAX / 2];
+            // V1 rows of the window into the window tile, Z1[:, K2] into the
+            // (not yet used) L factor tile
+            constexpr int ZPT = PMAX * PMAX / WP_THREADS;
+            C v1[RPT], zt[ZPT];
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int rw = wrow[r0 + 16 * q];
                 v1[q] = gload(Vb + (rw >= 0 ? (long)rw * VMAX + c : 0));
             }
 #pragma unroll
-            for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)k * n + k0 + c : 0));
+            for (int q = 0; q < ZPT; ++q) {
+                const int e = t + WP_THREADS * q, k = e / PMAX, cc = e % PMAX;
+                zt[q] = gload(Zg + (cc < nb ? (long)k * n + k0 + cc : 0));
+            }
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int rw = wrow[r0 + 16 * q];
-                Ws[r0 + 16 * q][c] = csel(rw >= 0, v1[q], mk<C>(R(0), R(0)));  // V1 rows first
+                Ws[r0 + 16 * q][c] = csel(rw >= 0, v1[q], mk<C>(R(0), R(0)));
                 v[q] = csel(c < nb, v[q], mk<C>(R(0), R(0)));
             }
-            __syncthreads();
-#pragma unroll 1
-            for (int kh = 0; kh < PMAX; kh += PMAX / 2) {
-                if (kh > 0) {
 #pragma unroll
-                    for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)(kh + k) * n + k0 + c : 0));
-                }
-#pragma unroll
-                for (int k = 0; k < PMAX / 2; ++k) zh[k] = csel(c < nb, zh[k], mk<C>(R(0), R(0)));
-#pragma unroll
-                for (int q = 0; q < RPT; ++q) v[q] = pair_second_step(v[q], &Ws[r0 + 16 * q][0], zh, kh);
+            for (int q = 0; q < ZPT; ++q) {
+                const int e = t + WP_THREADS * q, k = e / PMAX, cc = e % PMAX;
+                Ls[k][cc] = csel(cc < nb, zt[q], mk<C>(R(0), R(0)));
             }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) v[q] = pair_second_value(v[q], &Ws[r0 + 16 * q][0], Ls, c);
             __syncthreads();
         }
 #pragma unroll
@@ -671,7 +672,11 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     using acc_t = typename M::acc;
     const LineJob J = jobs[blockIdx.y];
     const int n = J.n;
-    __shared__ R Fr[PMAX][PMAX + 1], Fi[PMAX][PMAX + 1];  // M (Z blocks) or U^-1 (check blocks)
+    // M (Z blocks) or U^-1 (check blocks); in the check blocks of a second
+    // panel first the tile Z1[:, K2]
+    __shared__ __align__(16) R Fri[2][PMAX][PMAX + 1];
+    R(*Fr)[PMAX + 1] = Fri[0];
+    R(*Fi)[PMAX + 1] = Fri[1];
     // V1[piv2, :] (Z) or the V tile (check); in the check blocks of a second
     // panel first the rows' V1 as one complex tile
     __shared__ __align__(16) R Vri[2][PMAX][PMAX + 1];
@@ -692,7 +697,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     // second panel of a pair: the check blocks form the panel's values as
     // the first panel leaves them (its pivots take their Z1 row, every other
     // row A - V1 Z1[:, K2]) and write them to the panel buffer
-    C v1r[4], zh[PMAX / 2];
+    C v1r[4];
     int slot[4];
     if (!zpart) {
         const C* Y = (const C*)J.Y;
@@ -711,23 +716,39 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
                 slot[i] = gload_s(J.iprow + (r < n ? r : 0)) - k1;
                 v1r[i] = gload(Vb + (r < n ? (long)r * VMAX + vc : 0));
             }
-#pragma unroll
-            for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (vc < nb ? (long)k * n + k0 + vc : 0));
         }
     }
+    // check blocks of a second panel: U^-1 waits in registers while the
+    // factor tile holds Z1[:, K2]
+    const bool pair2 = second && !zpart;
+    C fv[EPT];
     {
         // M or U^-1 and the pivot rows: every load issued before the first use
         const C* Fg = (const C*)J.LU + (zpart ? 0 : PMAX * PMAX);
         const int p = gload_s(J.piv + poff + (t < nb ? t : 0));
-        C fv[EPT];
 #pragma unroll
         for (int q = 0; q < EPT; ++q) fv[q] = gload(Fg + t + 256 * q);
         if (t < PMAX) pv[t] = (t < nb) ? p : 0;  // rows past nb: a safe row, masked
+        if (pair2) {
+            C zt[EPT];
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e = t + 256 * q;
-            Fr[e / PMAX][e % PMAX] = fv[q].x;
-            Fi[e / PMAX][e % PMAX] = fv[q].y;
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q, k = e / PMAX, cc = e % PMAX;
+                zt[q] = gload((const C*)J.Z + (cc < nb ? (long)k * n + k0 + cc : 0));
+            }
+            C(*Z1s)[PMAX + 1] = (C(*)[PMAX + 1]) & Fri[0][0][0];
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q, k = e / PMAX, cc = e % PMAX;
+                Z1s[k][cc] = csel(cc < nb, zt[q], mk<C>(R(0), R(0)));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q;
+                Fr[e / PMAX][e % PMAX] = fv[q].x;
+                Fi[e / PMAX][e % PMAX] = fv[q].y;
+            }
         }
     }
     __syncthreads();
@@ -836,36 +857,26 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     } else {
         const int c = vc;
         if (second) {  // uniform
-            const C* Zg = (const C*)J.Z;
             C(*V1s)[PMAX + 1] = (C(*)[PMAX + 1]) & Vri[0][0][0];
-            C zs[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const bool piv1 = slot[i] >= 0 && slot[i] < nb1;
-                zs[i] = gload(Zg + (piv1 && c < nb ? (long)slot[i] * n + k0 + c : 0));
-                V1s[h8 + 8 * i][c] = (c < nb1) ? v1r[i] : mk<C>(R(0), R(0));
-            }
-            C w[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) w[i] = csel(c < nb, vrow[i], mk<C>(R(0), R(0)));
+            for (int i = 0; i < 4; ++i) V1s[h8 + 8 * i][c] = (c < nb1) ? v1r[i] : mk<C>(R(0), R(0));
+            const C(*Z1s)[PMAX + 1] = (const C(*)[PMAX + 1]) & Fri[0][0][0];
             __syncthreads();
-#pragma unroll 1
-            for (int kh = 0; kh < PMAX; kh += PMAX / 2) {
-                if (kh > 0) {
-#pragma unroll
-                    for (int k = 0; k < PMAX / 2; ++k) zh[k] = gload(Zg + (c < nb ? (long)(kh + k) * n + k0 + c : 0));
-                }
-#pragma unroll
-                for (int k = 0; k < PMAX / 2; ++k) zh[k] = csel(c < nb, zh[k], mk<C>(R(0), R(0)));
-#pragma unroll
-                for (int i = 0; i < 4; ++i) w[i] = pair_second_step(w[i], &V1s[h8 + 8 * i][0], zh, kh);
-            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const bool piv1 = slot[i] >= 0 && slot[i] < nb1;
-                vrow[i] = piv1 ? zs[i] : w[i];
+                // the first panel's pivots take their row of the Z1 tile
+                const int sl = slot[i];
+                const bool piv1 = sl >= 0 && sl < nb1;
+                const C a = csel(c < nb, vrow[i], mk<C>(R(0), R(0)));
+                vrow[i] = piv1 ? Z1s[piv1 ? sl : 0][c] : pair_second_value(a, &V1s[h8 + 8 * i][0], Z1s, c);
             }
-            __syncthreads();  // the V1 tile is read: its LDS takes the V tile
+            __syncthreads();  // both tiles are read: their LDS takes U^-1 and the V tile
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e = t + 256 * q;
+                Fr[e / PMAX][e % PMAX] = fv[q].x;
+                Fi[e / PMAX][e % PMAX] = fv[q].y;
+            }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
