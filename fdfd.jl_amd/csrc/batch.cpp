@@ -60,6 +60,8 @@ struct Builder {
             if (p) (void)hipFree(p);
         arena_clear(&b);
         buf_clear(&b);
+        if (b.tstage) (void)hipHostFree(b.tstage);
+        if (b.tstage_ev) (void)hipEventDestroy(b.tstage_ev);
     }
 };
 

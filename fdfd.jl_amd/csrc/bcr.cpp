@@ -22,6 +22,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -36,9 +38,20 @@ using namespace bcr;
 
 int rt_assemble(Handle* h, Factor* F);
 
+// The reduction plan depends on m only and never changes: built once per m
+// and shared (a batch's pipeline builds factors from two threads).
+static const Plan& plan_for(int m) {
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<const Plan>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto& p = cache[m];
+    if (!p) p.reset(new Plan(make_plan(m)));
+    return *p;
+}
+
 size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
     const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
-    const Plan P = make_plan(m);
+    const Plan& P = plan_for(m);
     size_t bytes = (size_t)P.nblocks * n * n * elem + sizeof(int) * 2 * (size_t)m * n;
     // Givens QR work of the first-level lines, kept for the reduced solve
     bytes += P.tris.size() * (size_t)tri::work_doubles(n) * sizeof(double);
@@ -50,7 +63,7 @@ size_t rt_factor_bytes(int n, int m, int precision, bool b3) {
 // the two arenas a factor takes: its blocks and (fp16x3 GEMMs) the split operands
 void rt_factor_arena_sizes(int n, int m, int precision, bool b3, size_t* blocks, size_t* split) {
     const size_t elem = (precision == FDFD_FP64) ? sizeof(double2) : sizeof(float2);
-    const Plan P = make_plan(m);
+    const Plan& P = plan_for(m);
     *blocks = (size_t)P.nblocks * n * n * elem;
     *split = (b3 && precision != FDFD_FP64) ? (size_t)P.nblocks * b3_block_bytes(n) : 0;
 }
@@ -83,7 +96,10 @@ struct TablePack {
         std::memcpy(bytes.data() + off, v.data(), sizeof(T) * v.size());
         parts.push_back({(void**)dst, off});
     }
-    // *base receives the buffer (nullptr when every table is empty)
+    // *base receives the buffer (nullptr when every table is empty).  The
+    // copy goes through the handle's pinned staging buffer and is ordered on
+    // the stream before the kernels that read the tables, so the host does
+    // not wait for it (only the next upload waits, for the staging buffer).
     int commit(Handle* h, void** base) {
         *base = nullptr;
         if (bytes.empty()) return FDFD_OK;
@@ -91,7 +107,19 @@ struct TablePack {
         if (!p) return fail(h, FDFD_ERR_OOM, "job table storage does not fit in device memory");
         *base = p;
         for (auto& q : parts) *q.first = (char*)p + q.second;
-        return stream_copy(h, p, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+        if (h->tstage_ev) HIPCHK(h, hipEventSynchronize(h->tstage_ev));  // the previous upload is done
+        if (h->tstage_bytes < bytes.size()) {
+            if (h->tstage) (void)hipHostFree(h->tstage);
+            h->tstage = nullptr;
+            h->tstage_bytes = 0;
+            HIPCHK(h, hipHostMalloc(&h->tstage, bytes.size()));
+            h->tstage_bytes = bytes.size();
+        }
+        if (!h->tstage_ev) HIPCHK(h, hipEventCreateWithFlags(&h->tstage_ev, hipEventDisableTiming));
+        std::memcpy(h->tstage, bytes.data(), bytes.size());
+        HIPCHK(h, hipMemcpyAsync(p, h->tstage, bytes.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipEventRecord(h->tstage_ev, h->stream));
+        return FDFD_OK;
     }
 };
 
@@ -152,7 +180,7 @@ int rt_build_factors(Handle* h, const std::vector<Factor*>& Fs) {
     const bool b3 = (prec != FDFD_FP64) && h->gemm_b3;
     if (n > MAX_LINE)
         return fail(h, FDFD_ERR_INVALID, "line block larger than 1024 cells (min(Nx,Ny) > 1024)");
-    const Plan P = make_plan(m);
+    const Plan& P = plan_for(m);
     int rc;
     const int inject = h->inject;  // one-shot: consumed by this factorization
     if (inject == FDFD_INJECT_PIVOT_FALLBACK || inject == FDFD_INJECT_SINGULAR ||

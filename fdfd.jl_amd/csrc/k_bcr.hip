@@ -10,6 +10,8 @@
 // GEMMs on MFMA (k_gemm) in the order Z = C_sj X_j, then D_s -= Z C_js.
 // The reduced solve applies the same steps to a vector (k_mv, k_diag).
 #include "fdfd_common.h"
+
+#include <atomic>
 #include "k_device.h"
 #include "tri_qr.h"
 
@@ -366,8 +368,20 @@ __global__ __launch_bounds__(128) void k_trisolve(const TrsJob* jobs, int n) {
 static size_t trisolve_lds(int n) { return sizeof(double) * (2 * (size_t)n + 2 * TS_CH * TS_STEP); }
 
 int trisolve_prepare(int n) {
-    return (int)hipFuncSetAttribute((const void*)k_trisolve,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)trisolve_lds(n));
+    // set once per device for the largest line length seen (a launch for a
+    // shorter line uses less)
+    static std::atomic<int> done[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
+    if (dev >= 0 && done[dev].load() >= n) return (int)hipSuccess;
+    const hipError_t e = hipFuncSetAttribute((const void*)k_trisolve,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)trisolve_lds(n));
+    if (e == hipSuccess && dev >= 0) {
+        int cur = done[dev].load();
+        while (cur < n && !done[dev].compare_exchange_weak(cur, n)) {
+        }
+    }
+    return (int)e;
 }
 
 void launch_trisolve(hipStream_t s, const TrsJob* jobs_d, int njobs, int n) {

@@ -548,6 +548,8 @@ int fdfd_destroy(fdfd_handle* hh) {
                     h->g_mid, h->x_out, h->V, h->Zk, h->tmp, h->tmp2, h->bvec, h->xvec,
                     h->dpart, h->rpart, h->hdev, h->bslab, h->io};
     for (void* b : bufs) dfree(b);
+    if (h->tstage) (void)hipHostFree(h->tstage);
+    if (h->tstage_ev) (void)hipEventDestroy(h->tstage_ev);
     prof_resolve(h->prof);
     for (auto e : h->prof.pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(h->stream);

@@ -139,6 +139,10 @@ struct Handle {
     // factorization of a grid allocates the same set, and hipMalloc/hipFree
     // pairs cost about a millisecond per factorization
     std::multimap<size_t, void*> bpool;      // free buffers by size
+    // pinned staging of job-table uploads and the event of the last upload
+    void* tstage = nullptr;
+    size_t tstage_bytes = 0;
+    hipEvent_t tstage_ev = nullptr;
     std::map<void*, size_t> bpool_live;      // handed-out buffers and their sizes
     size_t bpool_bytes = 0;                  // bytes held free in bpool
     // device staging of the host-pointer solve (sources, then fields), kept
