@@ -663,10 +663,12 @@ template void launch_panel<float>(hipStream_t, const LineJob*, int, int, int, in
 // ---------------------------------------------------------------------------
 constexpr int ZB_COLS = 64;  // columns per Z workgroup
 
-template <typename R>
+// SECOND: the second panel of a pair (a template parameter, so that the
+// first panel's launches carry none of its registers)
+template <typename R, bool SECOND>
 __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int nb, int fresh,
-                                                int verify, int nzb, int second, int k1,
-                                                int nb1) {
+                                                int verify, int nzb, int k1, int nb1) {
+    constexpr bool second = SECOND;
     using C = cplx<R>;
     using M = mfma_t<R>;
     using acc_t = typename M::acc;
@@ -707,7 +709,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
             const bool ok = r < n && vc < nb;
             vrow[i] = gload(Y + (ok ? (long)r * n + k0 + vc : 0));
         }
-        if (second) {  // uniform
+        if constexpr (second) {
             const C* Vb = (const C*)J.Vbuf;
             const C* Zg = (const C*)J.Z;
 #pragma unroll
@@ -756,7 +758,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
     C* Zg = (C*)J.Z;
     const C* Vb = (const C*)J.Vbuf;
     if (zpart) {
-        if (second) {
+        if constexpr (second) {
             C v[EPT];
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
@@ -799,7 +801,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
                     tim[rb][v] = (ldA && k < nb) ? tv[rb][v].y : R(0);
                 }
         }
-        if (second) {
+        if constexpr (second) {
             // T -= V1[piv2, :] Z1 (columns outside K2): A = V1[k][q], B = Z1[q][col],
             // q = 4 s + lane group
             const bool zok = colok && !inK;
@@ -856,7 +858,7 @@ __global__ __launch_bounds__(256) void k_zbuild(const LineJob* jobs, int k0, int
             }
     } else {
         const int c = vc;
-        if (second) {  // uniform
+        if constexpr (second) {
             C(*V1s)[PMAX + 1] = (C(*)[PMAX + 1]) & Vri[0][0][0];
 #pragma unroll
             for (int i = 0; i < 4; ++i) V1s[h8 + 8 * i][c] = (c < nb1) ? v1r[i] : mk<C>(R(0), R(0));
@@ -923,8 +925,10 @@ void launch_zbuild(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k
                    int fresh, int verify, int second, int k1, int nb1) {
     const int nzb = (n + ZB_COLS - 1) / ZB_COLS;
     const int nvb = (n + 31) / 32;
-    k_zbuild<R><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb,
-                                                        second, k1, nb1);
+    if (second)
+        k_zbuild<R, true><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb, k1, nb1);
+    else
+        k_zbuild<R, false><<<dim3(nzb + nvb, njobs), 256, 0, s>>>(jobs_d, k0, nb, fresh, verify, nzb, k1, nb1);
 }
 template void launch_zbuild<double>(hipStream_t, const LineJob*, int, int, int, int, int, int,
                                     int, int, int);
