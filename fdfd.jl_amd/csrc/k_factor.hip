@@ -276,9 +276,10 @@ __device__ inline C pair_second_value(C a, const C* v1row, const C (*zt)[PMAX + 
 // ---------------------------------------------------------------------------
 constexpr int WP_THREADS = 512;
 
-template <typename R, int NB>
+template <typename R, int NB, bool SECOND>
 __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, int k0, int nb,
-                                                        int fresh, int second) {
+                                                        int fresh) {
+    constexpr bool second = SECOND;  // second panel of a pair
     using C = cplx<R>;
     const LineJob J = jobs[blockIdx.x];
     const int n = J.n;
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(WP_THREADS, 2) void k_wpanel(const LineJob* jobs, i
             const int rw = wrow[r0 + 16 * q];
             v[q] = gload(Y + ((rw >= 0 && c < nb) ? (long)rw * n + k0 + c : 0));
         }
-        if (second) {  // uniform
+        if constexpr (second) {
             const C* Vb = (const C*)J.Vbuf;
             const C* Zg = (const C*)J.Z;
             // V1 rows of the window into the window tile, Z1[:, K2] into the
@@ -519,7 +520,8 @@ template <typename R>
 void launch_wpanel(hipStream_t s, const LineJob* jobs_d, int njobs, int n, int k0, int nb,
                    int fresh, int second) {
     (void)n;
-    k_wpanel<R, PMAX><<<njobs, WP_THREADS, 0, s>>>(jobs_d, k0, nb, fresh, second);
+    if (second) k_wpanel<R, PMAX, true><<<njobs, WP_THREADS, 0, s>>>(jobs_d, k0, nb, fresh);
+    else k_wpanel<R, PMAX, false><<<njobs, WP_THREADS, 0, s>>>(jobs_d, k0, nb, fresh);
 }
 template void launch_wpanel<double>(hipStream_t, const LineJob*, int, int, int, int, int, int);
 template void launch_wpanel<float>(hipStream_t, const LineJob*, int, int, int, int, int, int);
