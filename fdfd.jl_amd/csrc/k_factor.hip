@@ -1297,9 +1297,10 @@ __global__ __launch_bounds__(256) void k_trailing_f32(const LineJob* jobs, int k
 // dependent load.
 //
 // One workgroup per 64-row strip and TPW consecutive 64 x 64 tiles of it:
-// the strip's V rows are split once, and while a tile's products and
-// epilogue run the next tile's Z columns are already loading; its old rows
-// are issued right after the epilogue.  LDS: the V planes (32 KB, kept for
+// the strip's V rows are split once (with the first tile's loads already in
+// flight), and while a tile's products and epilogue run the next tile's Z
+// columns are already loading; its old rows are issued right after the
+// epilogue.  LDS: the V planes (32 KB, kept for
 // the strip) and one 36 KB region used in turn by a tile's Z planes and its
 // product tile (two workgroups per CU).
 constexpr int H3_PROW = 66;  // product tile row stride (complex): 528 bytes
@@ -1340,6 +1341,31 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3s(const LineJob* jobs, in
         const int i = i0 + (t >> 5) + 8 * q;
         rsl[q] = (i < n) ? gload_s(J.iprow + i) - k0 : -1;
     }
+    // ---- tiles
+    C pz[2][8];
+    auto load_z = [&](int j0) {
+        const int jz = j0 + 16 * wv + lr;
+#pragma unroll
+        for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                pz[ch][e] = gload(Zg + (jz < n && ch < nchunk ? (ch * PMAX + 8 * lk + e) * n + jz : 0));
+    };
+    f4v old[8];
+    auto load_old = [&](int j0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = i0 + (t >> 5) + 8 * q, j = j0 + rc;
+            const int sl = rsl[q];
+            const bool zsrc = fused ? (sl >= 0 && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
+            // element offsets fit in 32 bits (n <= MAX_LINE)
+            const int off = (i < n && j < n) ? (zsrc ? sl * n + j : i * n + j) : 0;
+            old[q] = gload2c((zsrc ? Zg : A) + off);
+        }
+    };
+    // the first tile's loads are in flight while the V rows are split
+    load_z(tj0 * 64);
+    load_old(tj0 * 64);
     // ---- V rows of the strip (both chunks), split once
     {
         const int slv = (iv < n) ? gload_s(J.iprow + iv) - k0 : -1;
@@ -1384,30 +1410,6 @@ __global__ __launch_bounds__(256, 2) void k_trailing_h3s(const LineJob* jobs, in
             sV[ch][3][wv][lane] = il;
         }
     }
-    // ---- tiles
-    C pz[2][8];
-    auto load_z = [&](int j0) {
-        const int jz = j0 + 16 * wv + lr;
-#pragma unroll
-        for (int ch = 0; ch < 2; ++ch)
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                pz[ch][e] = gload(Zg + (jz < n && ch < nchunk ? (ch * PMAX + 8 * lk + e) * n + jz : 0));
-    };
-    f4v old[8];
-    auto load_old = [&](int j0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int i = i0 + (t >> 5) + 8 * q, j = j0 + rc;
-            const int sl = rsl[q];
-            const bool zsrc = fused ? (sl >= 0 && sl < PMAX + nb2) : (sl >= 0 && sl < nb1);
-            // element offsets fit in 32 bits (n <= MAX_LINE)
-            const int off = (i < n && j < n) ? (zsrc ? sl * n + j : i * n + j) : 0;
-            old[q] = gload2c((zsrc ? Zg : A) + off);
-        }
-    };
-    load_z(tj0 * 64);
-    load_old(tj0 * 64);
     for (int tl = 0; tl < ntl; ++tl) {
         const int j0 = (tj0 + tl) * 64;
         const int jz = j0 + 16 * wv + lr;
