@@ -7,9 +7,15 @@
 //   * k_panel    : GEPP pivot selection on the not-yet-pivoted rows of V
 //                  (one workgroup, one row per thread in registers), giving
 //                  the factors of the pivot block V[p, :] = L U;
+//   * k_wpanel   : the same on a window of the first 64 unpivoted rows (the
+//                  default; every pivot is then checked against all other
+//                  rows by k_zbuild, threshold partial pivoting);
 //   * k_zbuild   : Z = U^-1 L^-1 [A[p, C] | I_K]            (nb x n);
 //   * k_trailing : A[i, :] = A'[i, :] - V[i, :] Z  (i not in p), A[p, :] = Z,
 //                  a rank-nb complex update on MFMA (A' = A with K zeroed).
+// With the window, two panels form a pair applied by one trailing pass: the
+// second panel works on its columns as the first leaves them (formed by the
+// second k_wpanel for its window and by the second k_zbuild for every row).
 // After the last panel the stored Y holds the inverse X = D^-1 as
 // X[a, b] = Y[prow[a], iprow[b]].
 #include "fdfd_common.h"
